@@ -1,0 +1,629 @@
+// C ABI of libmadi_hip.so: argument checks, geometry upload, jump matrices,
+// and launches.  See include/madi_hip.h for the contract of each function.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "launch.hpp"
+#include "phase.hpp"
+#include "rng.hpp"
+
+namespace madi {
+
+static thread_local char g_err[512] = "";
+
+void set_error(const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int check_launch(const char* what)
+{
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) {
+        set_error("%s launch failed: %s", what, hipGetErrorString(e));
+        return MADI_ERR_HIP;
+    }
+    return MADI_OK;
+}
+
+#define HIP_TRY(expr)                                                                  \
+    do {                                                                               \
+        hipError_t _e = (expr);                                                        \
+        if (_e != hipSuccess) {                                                        \
+            set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+            return _e == hipErrorOutOfMemory ? MADI_ERR_OOM : MADI_ERR_HIP;            \
+        }                                                                              \
+    } while (0)
+
+// ---------------------------------------------------- GF(2) jump matrices
+
+struct HostState {
+    uint64_t s0, s1;
+};
+
+static uint64_t host_rotl(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+static void host_next(HostState& s)
+{
+    uint64_t a = s.s0, b = s.s1;
+    b ^= a;
+    s.s0 = host_rotl(a, 55) ^ b ^ (b << 14);
+    s.s1 = host_rotl(b, 36);
+}
+
+// Published xoroshiro128+ jump (2^64 steps).
+static HostState host_jump(HostState s)
+{
+    static const uint64_t J[2] = {0xbeac0467eba5facbull, 0xd86b048b86aa9922ull};
+    uint64_t a = 0, b = 0;
+    for (int w = 0; w < 2; ++w)
+        for (int bit = 0; bit < 64; ++bit) {
+            if (J[w] & (1ull << bit)) {
+                a ^= s.s0;
+                b ^= s.s1;
+            }
+            host_next(s);
+        }
+    return HostState{a, b};
+}
+
+static HostState mat_apply(const Gf2Col* M, HostState v)
+{
+    uint64_t lo = 0, hi = 0;
+    for (int j = 0; j < 128; ++j) {
+        const uint64_t bit = j < 64 ? (v.s0 >> j) & 1 : (v.s1 >> (j - 64)) & 1;
+        if (bit) {
+            lo ^= M[j].lo;
+            hi ^= M[j].hi;
+        }
+    }
+    return HostState{lo, hi};
+}
+
+struct DeviceTables {
+    Gf2Col* jump_pow = nullptr;
+};
+
+static std::mutex g_tables_mu;
+static DeviceTables g_tables[64];
+
+static int get_jump_pow(const Gf2Col** out)
+{
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_error("device index %d out of range", dev);
+        return MADI_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_tables_mu);
+    if (!g_tables[dev].jump_pow) {
+        std::vector<Gf2Col> P((size_t)kJumpPowers * 128);
+        for (int j = 0; j < 128; ++j) {
+            HostState e{j < 64 ? 1ull << j : 0ull, j < 64 ? 0ull : 1ull << (j - 64)};
+            const HostState c = host_jump(e);
+            P[j] = Gf2Col{c.s0, c.s1};
+        }
+        for (int b = 1; b < kJumpPowers; ++b) {
+            const Gf2Col* A = &P[(size_t)(b - 1) * 128];
+            for (int j = 0; j < 128; ++j) {
+                const HostState col = mat_apply(A, HostState{A[j].lo, A[j].hi});
+                P[(size_t)b * 128 + j] = Gf2Col{col.s0, col.s1};
+            }
+        }
+        // self-check: matrix of one jump agrees with the jump routine
+        HostState t{0x0123456789abcdefull, 0xfedcba9876543210ull};
+        const HostState x = host_jump(t), y = mat_apply(&P[0], t);
+        if (x.s0 != y.s0 || x.s1 != y.s1) {
+            set_error("GF(2) jump matrix self-check failed");
+            return MADI_ERR_HIP;
+        }
+        Gf2Col* d = nullptr;
+        HIP_TRY(hipMalloc(&d, P.size() * sizeof(Gf2Col)));
+        HIP_TRY(hipMemcpy(d, P.data(), P.size() * sizeof(Gf2Col), hipMemcpyHostToDevice));
+        g_tables[dev].jump_pow = d;
+    }
+    *out = g_tables[dev].jump_pow;
+    return MADI_OK;
+}
+
+// ------------------------------------------------------------- geometry
+
+}  // namespace madi
+
+struct madi_geom {
+    madi::GeomDev dev;
+    std::vector<void*> allocs;
+    int64_t bytes = 0;
+    int device = 0;
+};
+
+namespace madi {
+
+template <typename T>
+static int upload(madi_geom* g, const T* host, size_t count, const T** out)
+{
+    *out = nullptr;
+    if (count == 0) return MADI_OK;
+    void* d = nullptr;
+    HIP_TRY(hipMalloc(&d, count * sizeof(T)));
+    g->allocs.push_back(d);
+    g->bytes += (int64_t)(count * sizeof(T));
+    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    *out = (const T*)d;
+    return MADI_OK;
+}
+
+static void free_geom(madi_geom* g)
+{
+    for (void* p : g->allocs) (void)hipFree(p);
+    g->allocs.clear();
+}
+
+static int build_geom(const madi_geom_desc* d, madi_geom* g)
+{
+    GeomDev& D = g->dev;
+    memset(&D, 0, sizeof D);
+    D.L = d->L;
+    D.source_lo = d->source_lo;
+    D.source_hi = d->source_hi;
+    D.periodic = d->periodic;
+    D.free_water = d->free_water;
+    int rc;
+    if (d->free_water) return MADI_OK;
+
+    // exact KD classifier: relabel the pre-order tree breadth-first
+    const int n = d->n_kd;
+    if (n < 1 || !d->kd_xyz || !d->kd_annulus || !d->node_seed || !d->node_axis || !d->node_left ||
+        !d->node_right || !d->node_parent) {
+        set_error("geometry: KD arrays are required for a cellular geometry (n_kd=%d)", n);
+        return MADI_ERR_ARG;
+    }
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<int> new_id(n, -1);
+    order.push_back(0);
+    new_id[0] = 0;
+    for (size_t h = 0; h < order.size(); ++h) {
+        const int o = order[h];
+        for (int c : {d->node_left[o], d->node_right[o]}) {
+            if (c < 0) continue;
+            if (c >= n || new_id[c] >= 0) {
+                set_error("geometry: KD tree is not a tree (node %d)", c);
+                return MADI_ERR_ARG;
+            }
+            new_id[c] = (int)order.size();
+            order.push_back(c);
+        }
+    }
+    if ((int)order.size() != n) {
+        set_error("geometry: KD tree reaches %zu of %d nodes", order.size(), n);
+        return MADI_ERR_ARG;
+    }
+    std::vector<KdNodeF> nf(n);
+    std::vector<KdNodeI> ni(n);
+    for (int k = 0; k < n; ++k) {
+        const int o = order[k];
+        const int sid = d->node_seed[o];
+        if (sid < 0 || sid >= n) {
+            set_error("geometry: node_seed out of range");
+            return MADI_ERR_ARG;
+        }
+        nf[k] = KdNodeF{d->kd_xyz[3 * sid], d->kd_xyz[3 * sid + 1], d->kd_xyz[3 * sid + 2], 0.0};
+        const int L = d->node_left[o], R = d->node_right[o], P = d->node_parent[o];
+        ni[k] = KdNodeI{L >= 0 ? new_id[L] : -1, R >= 0 ? new_id[R] : -1, P >= 0 ? new_id[P] : -1,
+                        (sid << 2) | (d->node_axis[o] & 3)};
+    }
+    std::vector<int32_t> label(n);
+    for (int k = 0; k < n; ++k) label[k] = d->kd_label ? d->kd_label[k] : k;
+    if ((rc = upload(g, nf.data(), nf.size(), &D.kd_f))) return rc;
+    if ((rc = upload(g, ni.data(), ni.size(), &D.kd_i))) return rc;
+    if ((rc = upload(g, d->kd_annulus, (size_t)n, &D.kd_annulus))) return rc;
+    if ((rc = upload(g, label.data(), label.size(), &D.kd_label))) return rc;
+    if ((rc = upload(g, d->kd_xyz, (size_t)n * 3, &D.kd_seed_xyz))) return rc;
+    D.n_kd = n;
+    D.n_kd_lds = n < 511 ? n : 511;
+
+    // Delaunay graph (optional: walks that only use the exact classifier may omit it)
+    const int nc = d->n_cells;
+    if (nc > 0 && d->nb_offsets && d->n_edges > 0) {
+        const int64_t E = d->n_edges;
+        if (d->nb_offsets[0] != 0 || d->nb_offsets[nc] != E) {
+            set_error("geometry: nb_offsets must run 0..n_edges");
+            return MADI_ERR_ARG;
+        }
+        std::vector<int32_t> off(nc + 1);
+        for (int c = 0; c <= nc; ++c) off[c] = (int32_t)d->nb_offsets[c];
+        std::vector<float4> rec(E);
+        std::vector<float> alpha(nc), hmax(nc, 0.f);
+        std::vector<double> alpha64(nc);
+        float amin = INFINITY;
+        for (int c = 0; c < nc; ++c) {
+            alpha64[c] = d->cell_annulus[c];
+            alpha[c] = (float)d->cell_annulus[c];
+            amin = fminf(amin, alpha[c]);
+            for (int64_t k = off[c]; k < off[c + 1]; ++k) {
+                const int nb = d->nb_ids[k];
+                if (nb < 0 || nb >= nc) {
+                    set_error("geometry: neighbour id out of range");
+                    return MADI_ERR_ARG;
+                }
+                const double ex = d->nb_vec[3 * k], ey = d->nb_vec[3 * k + 1], ez = d->nb_vec[3 * k + 2];
+                const double len = std::sqrt(ex * ex + ey * ey + ez * ez);
+                if (!(len > 0.0)) {
+                    set_error("Poisson seed realization contains coincident seeds");
+                    return MADI_ERR_CLASSIFIER;
+                }
+                rec[k] = make_float4((float)(ex / len), (float)(ey / len), (float)(ez / len), (float)(0.5 * len));
+                hmax[c] = fmaxf(hmax[c], (float)(0.5 * len));
+            }
+        }
+        if ((rc = upload(g, off.data(), off.size(), &D.nb_off))) return rc;
+        if ((rc = upload(g, rec.data(), rec.size(), &D.nb_rec))) return rc;
+        if ((rc = upload(g, d->nb_ids, (size_t)E, &D.nb_id))) return rc;
+        if ((rc = upload(g, d->nb_vec, (size_t)E * 3, &D.nb_e64))) return rc;
+        if ((rc = upload(g, alpha.data(), alpha.size(), &D.cell_alpha))) return rc;
+        if ((rc = upload(g, alpha64.data(), alpha64.size(), &D.cell_alpha64))) return rc;
+        if ((rc = upload(g, hmax.data(), hmax.size(), &D.cell_hmax))) return rc;
+        if ((rc = upload(g, d->cell_xyz, (size_t)nc * 3, &D.cell_xyz))) return rc;
+        if (d->hint_n < 1 || !d->hint) {
+            set_error("geometry: a hint grid is required with the Delaunay graph");
+            return MADI_ERR_ARG;
+        }
+        const size_t nh = (size_t)d->hint_n * d->hint_n * d->hint_n;
+        for (size_t k = 0; k < nh; ++k)
+            if (d->hint[k] < 0 || d->hint[k] >= nc) {
+                set_error("geometry: hint cell out of range");
+                return MADI_ERR_ARG;
+            }
+        if ((rc = upload(g, d->hint, nh, &D.hint))) return rc;
+        D.n_cells = nc;
+        D.hint_n = d->hint_n;
+        D.hint_h = d->L / d->hint_n;
+        D.alpha_min = amin;
+    }
+    return MADI_OK;
+}
+
+static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != nullptr; }
+
+static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Events& ev)
+{
+    memset(&P, 0, sizeof P);
+    memset(&ev, 0, sizeof ev);
+    if (!plan || plan->n_pairs < 1 || plan->n_pairs > MADI_MAX_PAIRS || plan->n_b < 1 || plan->n_b > MADI_MAX_B ||
+        plan->n_dirs < 1 || plan->n_dirs > MADI_MAX_DIRS) {
+        set_error("phase plan: need 1..%d pairs, 1..%d b values, 1..%d directions", MADI_MAX_PAIRS, MADI_MAX_B,
+                  MADI_MAX_DIRS);
+        return MADI_ERR_ARG;
+    }
+    P.n_pairs = plan->n_pairs;
+    P.n_b = plan->n_b;
+    P.n_dirs = plan->n_dirs;
+    P.n_cols = plan->n_pairs * plan->n_b * plan->n_dirs;
+    for (int p = 0; p < plan->n_pairs; ++p) {
+        const int js[3] = {plan->j_delta[p], plan->j_Delta[p], plan->j_sum[p]};
+        for (int t = 0; t < 3; ++t)
+            if (js[t] < 0 || js[t] >= n_grid) {
+                set_error("phase plan: grid index %d outside [0, %d)", js[t], n_grid);
+                return MADI_ERR_ARG;
+            }
+        P.j_delta[p] = js[0];
+        P.j_Delta[p] = js[1];
+        P.j_sum[p] = js[2];
+    }
+    for (int k = 0; k < plan->n_pairs * plan->n_b; ++k) {
+        P.pc[k] = plan->pc[k];
+        P.pc32[k] = (float)plan->pc[k];
+    }
+    for (int k = 0; k < plan->n_dirs * 3; ++k) {
+        P.dirs[k] = plan->dirs[k];
+        P.dirs32[k] = (float)plan->dirs[k];
+    }
+    // events sorted by grid index (stable in pair order)
+    for (int j = 0; j < n_grid; ++j)
+        for (int p = 0; p < plan->n_pairs; ++p) {
+            const int js[3] = {P.j_delta[p], P.j_Delta[p], P.j_sum[p]};
+            const int sg[3] = {1, 1, -1};
+            for (int t = 0; t < 3; ++t)
+                if (js[t] == j) {
+                    ev.j[ev.n] = j;
+                    ev.pair[ev.n] = p;
+                    ev.sign[ev.n] = sg[t];
+                    ++ev.n;
+                }
+        }
+    return MADI_OK;
+}
+
+static uint64_t reject_threshold(double thr)
+{
+    // smallest k with k * 2^-53 >= thr (k < 2^53); 2^53 never triggers
+    if (!(thr > 0.0)) return 0;
+    if (thr >= 1.0) return 1ull << 53;
+    const double k = std::ceil(std::ldexp(thr, 53));
+    return (uint64_t)k;
+}
+
+static int make_walk(const madi_walk_params* p, WalkDev& W)
+{
+    const bool shape_ok = p && p->n_walkers >= 0 && p->n_steps >= 0 && p->steps_per_h >= 1 && p->n_grid >= 1 &&
+                          p->n_steps / p->steps_per_h < p->n_grid;
+    if (!shape_ok) {
+        set_error("walk params: need n_walkers >= 0, steps_per_h >= 1 and n_steps / steps_per_h < n_grid");
+        return MADI_ERR_ARG;
+    }
+    if (!(p->pp >= 0.0 && p->pp <= 1.0)) {
+        set_error("p_p must be finite and in [0, 1], got %g", p->pp);
+        return MADI_ERR_ARG;
+    }
+    memset(&W, 0, sizeof W);
+    W.n_walkers = p->n_walkers;
+    W.n_steps = p->n_steps;
+    W.steps_per_h = p->steps_per_h;
+    W.n_grid = p->n_grid;
+    W.sigma = p->sigma;
+    W.ts = p->ts;
+    W.pp = p->pp;
+    W.seed = p->seed;
+    W.first_walker = p->first_walker;
+    if (p->pp >= 1.0) {
+        W.rej1 = W.rej2 = 1ull << 53;
+    } else {
+        W.rej1 = reject_threshold(p->pp);
+        W.rej2 = reject_threshold(p->pp * p->pp);
+    }
+    return MADI_OK;
+}
+
+__global__ void k_zero_i64(long long* a, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = 0;
+}
+
+static int zero_i64(long long* a, int n, hipStream_t s)
+{
+    if (!a || n <= 0) return MADI_OK;
+    hipLaunchKernelGGL(k_zero_i64, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
+    return check_launch("k_zero_i64");
+}
+
+}  // namespace madi
+
+using namespace madi;
+
+extern "C" {
+
+int madi_abi_version(void) { return MADI_ABI_VERSION; }
+
+const char* madi_last_error(void) { return g_err; }
+
+int madi_get_device_info(madi_device_info* out)
+{
+    if (!out) {
+        set_error("madi_get_device_info: out is NULL");
+        return MADI_ERR_ARG;
+    }
+    memset(out, 0, sizeof *out);
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, dev));
+    out->device = dev;
+    out->cu_count = prop.multiProcessorCount;
+    out->clock_khz = prop.clockRate;
+    out->xcc_count = 0;
+    size_t fr = 0, tot = 0;
+    HIP_TRY(hipMemGetInfo(&fr, &tot));
+    out->total_mem = (int64_t)tot;
+    out->free_mem = (int64_t)fr;
+    snprintf(out->name, sizeof out->name, "%s", prop.name);
+    snprintf(out->arch, sizeof out->arch, "%s", prop.gcnArchName);
+    return MADI_OK;
+}
+
+int madi_geom_create(const madi_geom_desc* desc, madi_geom** out)
+{
+    if (!desc || !out) {
+        set_error("madi_geom_create: NULL argument");
+        return MADI_ERR_ARG;
+    }
+    *out = nullptr;
+    if (!(desc->L > 0.0)) {
+        set_error("madi_geom_create: L must be positive");
+        return MADI_ERR_ARG;
+    }
+    madi_geom* g = new madi_geom();
+    (void)hipGetDevice(&g->device);
+    const int rc = build_geom(desc, g);
+    if (rc) {
+        free_geom(g);
+        delete g;
+        return rc;
+    }
+    *out = g;
+    return MADI_OK;
+}
+
+int madi_geom_destroy(madi_geom* g)
+{
+    if (!g) return MADI_OK;
+    free_geom(g);
+    delete g;
+    return MADI_OK;
+}
+
+int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes)
+{
+    if (!g || !bytes) {
+        set_error("madi_geom_device_bytes: NULL argument");
+        return MADI_ERR_ARG;
+    }
+    *bytes = g->bytes;
+    return MADI_OK;
+}
+
+int madi_classify(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside, int8_t* d_ok,
+                  void* stream)
+{
+    if (!g || n < 0 || (n > 0 && (!d_pts || !d_cell || !d_inside || !d_ok))) {
+        set_error("madi_classify: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    return launch_classify_f64(g->dev, d_pts, n, d_cell, d_inside, d_ok, nullptr, (hipStream_t)stream);
+}
+
+int madi_classify_graph(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside,
+                        int8_t* d_ok, int64_t* d_counters, void* stream)
+{
+    if (!g || n < 0 || (n > 0 && (!d_pts || !d_cell || !d_inside || !d_ok))) {
+        set_error("madi_classify_graph: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (!graph_ready(g->dev)) {
+        set_error("madi_classify_graph: geometry has no Delaunay graph");
+        return MADI_ERR_ARG;
+    }
+    return launch_classify_graph(g->dev, d_pts, n, d_cell, d_inside, d_ok, (unsigned long long*)d_counters,
+                                 (hipStream_t)stream);
+}
+
+int madi_rng_init(uint64_t seed, uint64_t first, int64_t n, uint64_t* d_state, void* stream)
+{
+    if (n < 0 || (n > 0 && !d_state) || first + (uint64_t)n >= (1ull << kJumpPowers)) {
+        set_error("madi_rng_init: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const Gf2Col* jp = nullptr;
+    int rc = get_jump_pow(&jp);
+    if (rc) return rc;
+    return launch_rng_init(seed, first, n, d_state, jp, (hipStream_t)stream);
+}
+
+int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d_gold_init, const double* d_gold_incr,
+                  const double* d_gold_u, double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err,
+                  int64_t* d_counters, void* stream)
+{
+    WalkDev W;
+    int rc = make_walk(p, W);
+    if (rc) return rc;
+    if (!g || (W.n_walkers > 0 && (!d_Y || !d_inside_trace || !d_escaped || !d_err))) {
+        set_error("madi_walk_f64: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const int golden = d_gold_init != nullptr;
+    if (golden && (!d_gold_incr || !d_gold_u)) {
+        set_error("madi_walk_f64: an injected stream needs initial positions, increments and uniforms");
+        return MADI_ERR_ARG;
+    }
+    W.use_golden = golden;
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) return rc;
+    return launch_walk_f64(g->dev, W, jp, d_gold_init, d_gold_incr, d_gold_u, d_Y, d_inside_trace, d_escaped, d_err,
+                           (unsigned long long*)d_counters, (hipStream_t)stream);
+}
+
+int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
+                    const int32_t* d_js, const double* d_pc, int32_t n_cols, double* d_cos, double* d_sin,
+                    void* stream)
+{
+    if (n < 0 || n_grid < 1 || n_cols < 0 || (n_cols > 0 && (!d_jd || !d_jD || !d_js || !d_pc || !d_cos || !d_sin)) ||
+        (n > 0 && !d_Y)) {
+        set_error("madi_reduce_f64: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (n_cols == 0) return MADI_OK;
+    int32_t chunks = (int32_t)((n * 3 + 8191) / 8192);
+    if (chunks < 1) chunks = 1;
+    if (chunks > 512) chunks = 512;
+    double* part = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&part, (size_t)n_cols * chunks * 2 * sizeof(double), (hipStream_t)stream));
+    int rc = launch_reduce_f64(d_Y, n, n_grid, d_jd, d_jD, d_js, d_pc, n_cols, part, chunks, d_cos, d_sin,
+                               (hipStream_t)stream);
+    (void)hipFreeAsync(part, (hipStream_t)stream);
+    return rc;
+}
+
+int madi_walk_fused_workspace(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes)
+{
+    if (!plan || !bytes || n_walkers < 0) {
+        set_error("madi_walk_fused_workspace: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const int64_t tiles = (n_walkers + kBlock - 1) / kBlock;
+    const int64_t cols = (int64_t)plan->n_pairs * plan->n_b * plan->n_dirs;
+    *bytes = (int64_t)sizeof(PlanDev) + tiles * cols * 2 * (int64_t)sizeof(double) + 256;
+    return MADI_OK;
+}
+
+int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
+                    void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
+                    int64_t* d_flags, int64_t* d_counters, void* stream)
+{
+    WalkDev W;
+    int rc = make_walk(p, W);
+    if (rc) return rc;
+    if (!g || !d_cos || !d_sin || !d_occupancy || !d_flags || !d_workspace) {
+        set_error("madi_walk_fused: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (precision != 32 && precision != 64) {
+        set_error("madi_walk_fused: precision must be 32 or 64");
+        return MADI_ERR_ARG;
+    }
+    if (precision == 32 && !graph_ready(g->dev)) {
+        set_error("madi_walk_fused: the FP32 walk needs the geometry's Delaunay graph");
+        return MADI_ERR_ARG;
+    }
+    int64_t need = 0;
+    madi_walk_fused_workspace(W.n_walkers, plan, &need);
+    if (workspace_bytes < need) {
+        set_error("madi_walk_fused: workspace of %lld bytes is smaller than the %lld required",
+                  (long long)workspace_bytes, (long long)need);
+        return MADI_ERR_ARG;
+    }
+    PlanDev P;
+    Events ev;
+    if ((rc = make_plan(plan, W.n_grid, P, ev))) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    PlanDev* dplan = (PlanDev*)d_workspace;
+    double* partial = (double*)((char*)d_workspace + ((sizeof(PlanDev) + 255) / 256) * 256);
+    HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
+    if ((rc = zero_i64((long long*)d_occupancy, W.n_grid, s))) return rc;
+    if ((rc = zero_i64((long long*)d_flags, MADI_N_FLAGS, s))) return rc;
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) return rc;
+    FusedOut fo;
+    fo.plan = dplan;
+    fo.ev = ev;
+    fo.partial = partial;
+    fo.tile_base = 0;
+    fo.occupancy = (long long*)d_occupancy;
+    fo.flags = (long long*)d_flags;
+    if (precision == 32)
+        rc = launch_walk_f32_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
+    else
+        rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
+    if (rc) return rc;
+    const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
+    if (tiles == 0) {
+        std::vector<double> z(P.n_cols, 0.0);
+        HIP_TRY(hipMemcpyAsync(d_cos, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(d_sin, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return MADI_OK;
+    }
+    return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Host-side launch declarations shared between the kernel translation units
+// and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+#include "phase.hpp"
+#include "rng.hpp"
+
+namespace madi {
+
+struct WalkDev {
+    int64_t n_walkers;
+    int32_t n_steps;
+    int32_t steps_per_h;
+    int32_t n_grid;
+    int32_t use_golden;
+    double sigma;
+    double ts;
+    double pp;
+    uint64_t seed;
+    uint64_t first_walker;
+    // Acceptance as integer thresholds on the 53-bit uniform mantissa k:
+    // u = k * 2^-53 >= thr  <=>  k >= rej[m-1].  rej = 2^53 + 1 disables.
+    uint64_t rej1;
+    uint64_t rej2;
+};
+
+struct FusedOut {
+    const PlanDev* plan;   // device copy
+    Events ev;
+    double* partial;       // tile slots: n_tiles x n_cols x 2
+    int64_t tile_base;     // slot index of this launch's first tile
+    long long* occupancy;  // n_grid
+    long long* flags;      // MADI_N_FLAGS
+};
+
+void set_error(const char* fmt, ...);
+int check_launch(const char* what);
+
+int launch_classify_f64(const GeomDev& g, const double* pts, int64_t n, int32_t* cell, int8_t* inside, int8_t* ok,
+                        unsigned long long* counters, hipStream_t s);
+int launch_rng_init(uint64_t seed, uint64_t first, int64_t n, uint64_t* out, const Gf2Col* jump_pow, hipStream_t s);
+int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const double* gi, const double* gd,
+                    const double* gu, double* Y, int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters,
+                    hipStream_t s);
+int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
+                          unsigned long long* counters, hipStream_t s);
+int launch_reduce_f64(const double* Y, int64_t n, int32_t G, const int32_t* jd, const int32_t* jD, const int32_t* js,
+                      const double* pc, int32_t n_cols, double* part, int32_t n_chunks, double* cos_out,
+                      double* sin_out, hipStream_t s);
+
+int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_t* cell, int8_t* inside, int8_t* ok,
+                          unsigned long long* counters, hipStream_t s);
+int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
+                          unsigned long long* counters, hipStream_t s);
+int launch_finalize_tiles(const double* partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols, double* cos_out,
+                          double* sin_out, hipStream_t s);
+
+}  // namespace madi

@@ -1,0 +1,219 @@
+/*
+ * madi_hip.h -- C ABI of the MI355X (gfx950) MADI random-walk library.
+ *
+ * Conventions
+ *  - Every function returns an int status: MADI_OK (0) or a MADI_ERR_* code;
+ *    madi_last_error() returns a thread-local message for the last failure.
+ *  - Pointers named d_* are caller-owned device buffers (for example
+ *    torch-ROCm tensors' data_ptr()); all other pointers are host memory.
+ *  - void* stream is the caller's hipStream_t (NULL = the null stream).
+ *    Functions enqueue work on it and return without synchronising unless
+ *    the comment says otherwise.
+ *  - No function aborts the process; device-side failures (fatal escape,
+ *    classifier failure) come back as per-walker flags or counters.
+ *
+ * Each entry point names the reference interface it replaces; paths are
+ * relative to the reference tree (jadenfigger/Custom_MADI).
+ */
+#ifndef MADI_HIP_H
+#define MADI_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MADI_ABI_VERSION 1
+
+#define MADI_OK 0
+#define MADI_ERR_ARG 1
+#define MADI_ERR_HIP 2
+#define MADI_ERR_OOM 3
+#define MADI_ERR_CLASSIFIER 4
+#define MADI_ERR_ESCAPE 5
+#define MADI_ERR_BUSY 6
+
+#define MADI_MAX_PAIRS 8
+#define MADI_MAX_B 16
+#define MADI_MAX_DIRS 64
+
+/* Counters returned by the walk kernels (int64 each). */
+#define MADI_CNT_CLASSIFY 0      /* endpoint classifications run (graph or KD) */
+#define MADI_CNT_SAFE_HIT 1      /* endpoints answered by the certified safe ball */
+#define MADI_CNT_GRAPH_ITERS 2   /* Delaunay-graph anchor moves */
+#define MADI_CNT_FP64_RECHECK 3  /* FP32 decisions re-decided in FP64 */
+#define MADI_CNT_FACETS 4        /* facet (edge) evaluations */
+#define MADI_CNT_KD_NODES 5      /* KD nodes visited (FP64 exact classifier) */
+#define MADI_CNT_CROSSINGS 6     /* proposals with m > 0 */
+#define MADI_CNT_REJECTS 7       /* rejected crossings */
+#define MADI_N_COUNTERS 8
+
+/* Flags summed over walkers (int64 each). */
+#define MADI_FLAG_ESCAPED 0
+#define MADI_FLAG_CLASSIFIER_ERR 1
+#define MADI_N_FLAGS 2
+
+int madi_abi_version(void);
+const char* madi_last_error(void);
+
+typedef struct {
+    int32_t device;
+    int32_t cu_count;
+    int32_t clock_khz;
+    int32_t xcc_count;
+    int64_t total_mem;
+    int64_t free_mem;
+    char name[64];
+    char arch[32];
+} madi_device_info;
+
+/* Properties of the current device (no reference counterpart; used by the
+ * benchmark's roofline and by admission control). */
+int madi_get_device_info(madi_device_info* out);
+
+/* ------------------------------------------------------------ geometry */
+
+/* Host description of one ensemble geometry.  Replaces the per-walk device
+ * upload of Ensemble arrays in madi/walker_gpu.py:871-877 (seeds, annulus
+ * and the five KD arrays of madi/ensemble.py:525-563); the handle keeps one
+ * device copy for every walk and k_io that uses the geometry.
+ *
+ * kd_*: the exact classifier's point set and its balanced KD tree in
+ *   pre-order (node_seed/axis/left/right/parent as madi/ensemble.py builds
+ *   them).  kd_label maps each KD point to its cell (identity for the
+ *   reference's non-periodic sets; periodic images map to their cell).
+ * cell_*, nb_*: the Delaunay neighbour graph of the cells (CSR over cells,
+ *   nb_vec = c_k + shift - c_a in FP64), used by the FP32 production walk.
+ * hint: hint_n^3 ints over [0, L)^3, a cell near each box centre. */
+typedef struct {
+    int32_t n_kd;
+    const double* kd_xyz;
+    const double* kd_annulus;
+    const int32_t* kd_label;
+    const int32_t* node_seed;
+    const int8_t* node_axis;
+    const int32_t* node_left;
+    const int32_t* node_right;
+    const int32_t* node_parent;
+    int32_t n_cells;
+    const double* cell_xyz;
+    const double* cell_annulus;
+    int64_t n_edges;
+    const int64_t* nb_offsets;
+    const int32_t* nb_ids;
+    const double* nb_vec;
+    int32_t hint_n;
+    const int32_t* hint;
+    double L;
+    double source_lo;
+    double source_hi;
+    int32_t periodic;
+    int32_t free_water;
+} madi_geom_desc;
+
+typedef struct madi_geom madi_geom;
+
+int madi_geom_create(const madi_geom_desc* desc, madi_geom** out);
+int madi_geom_destroy(madi_geom* g);
+/* Device bytes held by the handle (admission control). */
+int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes);
+
+/* --------------------------------------------------------- classifiers */
+
+/* Exact SI Eq. S2 label of n points (d_pts: n x 3 FP64).  FP64 stackless KD
+ * traversal: nearest seed, then every seed within d1 + 2*alpha1.  Replaces
+ * _classify_full_facet (madi/walker_gpu.py:272-335) and
+ * Ensemble.classify_cpu (madi/ensemble.py:213-256).  d_ok[i] = 0 marks
+ * coincident seeds (the reference's classifier failure). */
+int madi_classify(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside,
+                  int8_t* d_ok, void* stream);
+
+/* The same labels from the FP32 Delaunay-graph classifier of the production
+ * walk, certified in FP64 near every decision boundary.  d_counters may be
+ * NULL (else MADI_N_COUNTERS int64, accumulated).  No reference counterpart
+ * beyond madi_classify; it exists so tests can hold the production
+ * classifier to the exact one. */
+int madi_classify_graph(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside,
+                        int8_t* d_ok, int64_t* d_counters, void* stream);
+
+/* ----------------------------------------------------------------- RNG */
+
+/* d_state[2i], d_state[2i+1] = xoroshiro128+ state of walker first+i for
+ * seed (splitmix64 seeding, jump^(first+i)).  Replaces
+ * create_xoroshiro128p_states (madi/walker_gpu.py:911, numba-src:224-291),
+ * computed on the device. */
+int madi_rng_init(uint64_t seed, uint64_t first, int64_t n, uint64_t* d_state, void* stream);
+
+/* ---------------------------------------------------------------- walk */
+
+typedef struct {
+    int64_t n_walkers;
+    int32_t n_steps;
+    int32_t steps_per_h;
+    int32_t n_grid;
+    int32_t reserved;
+    double sigma;
+    double ts;
+    double pp;
+    uint64_t seed;          /* walk seed (device RNG) */
+    uint64_t first_walker;  /* RNG subsequence index of walker 0 */
+} madi_walk_params;
+
+/* Reference-semantics FP64 walk with the exact classifier at every endpoint.
+ * Replaces _walk_kernel (madi/walker_gpu.py:520-656).  Random inputs come
+ * from the device RNG (d_gold_* all NULL; states from seed/first_walker) or
+ * from an injected stream (WalkRandomStream, madi/walker_gpu.py:86-127):
+ * d_gold_init n x 3, d_gold_incr n_steps x n x 3, d_gold_u n_steps x n.
+ * Outputs: d_Y n x n_grid x 3, d_inside_trace n x n_grid, d_escaped n,
+ * d_err n.  d_counters may be NULL. */
+int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d_gold_init, const double* d_gold_incr,
+                  const double* d_gold_u, double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err,
+                  int64_t* d_counters, void* stream);
+
+/* Sigma cos / sigma sin over walkers and axes of pc*(Y[jd]+Y[jD]-Y[js]) per
+ * column, FP64.  Replaces _reduce_kernel (madi/walker_gpu.py:658-695). */
+int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
+                    const int32_t* d_js, const double* d_pc, int32_t n_cols, double* d_cos, double* d_sin,
+                    void* stream);
+
+/* Phase plan of the fused walk: n_pairs (delta, Delta) pairs given as Y-grid
+ * indices, n_b phase coefficients per pair (gamma*G*1e-9), n_dirs unit
+ * gradient directions.  Output column (pair, b, dir) has index
+ * (pair * n_b + b) * n_dirs + dir. */
+typedef struct {
+    int32_t n_pairs;
+    int32_t n_b;
+    int32_t n_dirs;
+    int32_t reserved;
+    int32_t j_delta[MADI_MAX_PAIRS];
+    int32_t j_Delta[MADI_MAX_PAIRS];
+    int32_t j_sum[MADI_MAX_PAIRS];
+    double pc[MADI_MAX_PAIRS * MADI_MAX_B];
+    double dirs[MADI_MAX_DIRS * 3];
+} madi_phase_plan;
+
+/* Bytes of scratch the fused walk needs for n_walkers (fixed-slot per-tile
+ * partial sums). */
+int madi_walk_fused_workspace(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes);
+
+/* Production walk with the phase reduction fused in: FP32 seed-relative
+ * state, Delaunay-graph classifier certified in FP64, compensated Y, FP64
+ * cross-walker sums in fixed tile order.  precision 32 selects this kernel;
+ * precision 64 runs the same fused reduction on the FP64 exact-classifier
+ * walk (the FP64 oracle run on the device).  Replaces the walk + reduce
+ * pair of _walk_and_reduce_one_ensemble (madi/walker_gpu.py:989-1063) for
+ * column sets that fit the plan.
+ * Outputs: d_cos, d_sin (n_cols FP64 sums), d_occupancy (n_grid int64 sums of
+ * the inside label at each Y-grid time), d_flags (MADI_N_FLAGS int64),
+ * d_counters (MADI_N_COUNTERS int64, may be NULL).  All outputs are
+ * overwritten. */
+int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
+                    void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
+                    int64_t* d_flags, int64_t* d_counters, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MADI_HIP_H */

@@ -1,0 +1,353 @@
+"""CPU ORACLE for the MADI walk -- TEST INFRASTRUCTURE ONLY.
+
+Loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+leg, as the checker; the shipped package never imports it.
+
+Contents
+  * ctypes binding of oracle/libmadi_oracle.so (madi_oracle.c): the FP64 C
+    restatement of the reference walk, classifier, reduce and RNG.
+  * numpy restatements used as independent checks:
+      classify_brute       all-seed SI Eq. S2 labels
+                           (tests/physics_audit/test_geometry_diagnostics.py:119-140)
+      walk_cpu_reference   the reference CPU walk with cKDTree classification
+                           (madi/walker_gpu.py:698-840 + madi/ensemble.py:213-256)
+      reduce_reference     madi/walker_gpu.py:954-968
+      Xoroshiro128p        pure-Python integer stream (numba-src:45-125)
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import subprocess
+from pathlib import Path
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "libmadi_oracle.so"
+
+_f64p = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i8p = C.POINTER(C.c_int8)
+_u64p = C.POINTER(C.c_uint64)
+
+
+class OGeom(C.Structure):
+    _fields_ = [
+        ("seeds", _f64p), ("annulus", _f64p), ("node_seed", _i32p), ("node_axis", _i8p), ("node_left", _i32p),
+        ("node_right", _i32p), ("node_parent", _i32p), ("orig_id", _i32p), ("n", C.c_int64), ("L", C.c_double),
+        ("periodic", C.c_int32), ("free_water", C.c_int32),
+    ]
+
+
+class OWalk(C.Structure):
+    _fields_ = [
+        ("n_walkers", C.c_int64), ("n_steps", C.c_int32), ("steps_per_h", C.c_int32), ("n_grid", C.c_int32),
+        ("use_golden", C.c_int32), ("sigma", C.c_double), ("ts", C.c_double), ("pp", C.c_double),
+        ("source_lo", C.c_double), ("source_hi", C.c_double), ("seed", C.c_uint64), ("first_walker", C.c_uint64),
+    ]
+
+
+class OState(C.Structure):
+    _fields_ = [("s0", C.c_uint64), ("s1", C.c_uint64)]
+
+
+_lib = None
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB.is_file():
+            build()
+        L = C.CDLL(str(LIB))
+        L.og_seed_state.restype = OState
+        L.og_seed_state.argtypes = [C.c_uint64]
+        L.og_next.restype = C.c_uint64
+        L.og_next.argtypes = [C.POINTER(OState)]
+        L.og_jump.restype = None
+        L.og_jump.argtypes = [C.POINTER(OState)]
+        L.og_rng_states.restype = None
+        L.og_rng_states.argtypes = [C.c_uint64, C.c_uint64, C.c_int64, _u64p]
+        L.og_draw.restype = None
+        L.og_draw.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int64, _f64p]
+        L.og_classify.restype = None
+        L.og_classify.argtypes = [C.POINTER(OGeom), _f64p, C.c_int64, _i32p, _i8p, _i8p]
+        L.og_classify_brute.restype = None
+        L.og_classify_brute.argtypes = [_f64p, _f64p, C.c_int64, _f64p, C.c_int64, _i32p, _i8p]
+        L.og_walk.restype = None
+        L.og_walk.argtypes = [C.POINTER(OGeom), C.POINTER(OWalk), _u64p, _f64p, _f64p, _f64p, _f64p, _i8p, _i8p,
+                              _i8p]
+        L.og_reduce.restype = None
+        L.og_reduce.argtypes = [_f64p, C.c_int64, C.c_int32, _i32p, _i32p, _i32p, _f64p, C.c_int32, _f64p, _f64p]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t)) if a is not None else None
+
+
+class Geometry:
+    """Oracle view of a geometry: KD point set + tree (+ labels / periodic flag)."""
+
+    def __init__(self, kd_xyz, kd_annulus, nodes, L, periodic=False, label=None, free_water=False):
+        self.kd_xyz = np.ascontiguousarray(kd_xyz, dtype=np.float64)
+        self.kd_annulus = np.ascontiguousarray(kd_annulus, dtype=np.float64)
+        self.nodes = [np.ascontiguousarray(nodes[0], dtype=np.int32), np.ascontiguousarray(nodes[1], dtype=np.int8),
+                      np.ascontiguousarray(nodes[2], dtype=np.int32), np.ascontiguousarray(nodes[3], dtype=np.int32),
+                      np.ascontiguousarray(nodes[4], dtype=np.int32)]
+        self.label = None if label is None else np.ascontiguousarray(label, dtype=np.int32)
+        g = OGeom()
+        g.seeds, g.annulus = _p(self.kd_xyz, C.c_double), _p(self.kd_annulus, C.c_double)
+        g.node_seed, g.node_axis = _p(self.nodes[0], C.c_int32), _p(self.nodes[1], C.c_int8)
+        g.node_left, g.node_right = _p(self.nodes[2], C.c_int32), _p(self.nodes[3], C.c_int32)
+        g.node_parent = _p(self.nodes[4], C.c_int32)
+        g.orig_id = _p(self.label, C.c_int32)
+        g.n, g.L, g.periodic, g.free_water = len(self.kd_xyz), float(L), int(periodic), int(free_water)
+        self.c = g
+        self.L = float(L)
+
+    @classmethod
+    def from_arrays(cls, arr):
+        """From custom_madi_amd.ensemble.GeometryArrays."""
+        if arr.free_water:
+            z = np.zeros((1, 3))
+            return cls(z, np.zeros(1), (np.zeros(1), np.zeros(1), -np.ones(1), -np.ones(1), -np.ones(1)), arr.L,
+                       free_water=True)
+        return cls(arr.kd_xyz, arr.kd_annulus, arr.nodes, arr.L, arr.periodic, arr.kd_label)
+
+    def classify(self, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64).reshape(-1, 3)
+        m = len(pts)
+        cell = np.empty(m, np.int32)
+        ins = np.empty(m, np.int8)
+        ok = np.empty(m, np.int8)
+        lib().og_classify(C.byref(self.c), _p(pts, C.c_double), m, _p(cell, C.c_int32), _p(ins, C.c_int8),
+                          _p(ok, C.c_int8))
+        return cell, ins.astype(bool), ok.astype(bool)
+
+
+def classify_brute_c(seeds, annulus, pts):
+    seeds = np.ascontiguousarray(seeds, dtype=np.float64)
+    annulus = np.ascontiguousarray(annulus, dtype=np.float64)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cell = np.empty(len(pts), np.int32)
+    ins = np.empty(len(pts), np.int8)
+    lib().og_classify_brute(_p(seeds, C.c_double), _p(annulus, C.c_double), len(seeds), _p(pts, C.c_double),
+                            len(pts), _p(cell, C.c_int32), _p(ins, C.c_int8))
+    return cell, ins.astype(bool)
+
+
+def classify_brute(seeds, annulus, points, block=128):
+    """All-seed SI Eq. S2 labels in numpy (tests/physics_audit/test_geometry_diagnostics.py:119-140)."""
+    seeds = np.asarray(seeds, dtype=np.float64)
+    ids = np.empty(len(points), dtype=np.int32)
+    inside = np.empty(len(points), dtype=bool)
+    for lo in range(0, len(points), block):
+        blk = np.asarray(points[lo:lo + block], dtype=np.float64)
+        diff = seeds[None, :, :] - blk[:, None, :]
+        d2 = np.einsum("...i,...i->...", diff, diff)
+        near = np.argmin(d2, axis=1)
+        rows = np.arange(len(blk))
+        d1 = d2[rows, near]
+        sep = np.linalg.norm(seeds[None, :, :] - seeds[near][:, None, :], axis=2)
+        sep[rows, near] = 1.0
+        margin = (d2 - d1[:, None]) / (2.0 * sep)
+        margin[rows, near] = np.inf
+        ids[lo:lo + len(blk)] = near
+        inside[lo:lo + len(blk)] = np.all(margin >= annulus[near][:, None], axis=1)
+    return ids, inside
+
+
+def periodic_brute(seeds, annulus, L, points):
+    """Labels of a periodic box from the 27-image tiling (cell id = image's cell)."""
+    shifts = np.array([(a, b, c) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)], dtype=float)
+    tiled = np.concatenate([seeds + s * L for s in shifts])
+    ann = np.tile(annulus, len(shifts))
+    wrapped = np.mod(points, L)
+    ids, inside = classify_brute(tiled, ann, wrapped)
+    return (ids % len(seeds)).astype(np.int32), inside
+
+
+def rng_states(seed, first, n):
+    out = np.empty(2 * n, dtype=np.uint64)
+    lib().og_rng_states(int(seed), int(first), int(n), _p(out, C.c_uint64))
+    return out.reshape(n, 2)
+
+
+def draws(s0, s1, kind, n):
+    """kind 0 uniform_f64, 1 uniform_f32, 2 normal_f64, 3 raw uint64 (as bits)."""
+    out = np.empty(n, dtype=np.float64)
+    lib().og_draw(int(s0), int(s1), int(kind), int(n), _p(out, C.c_double))
+    return out.view(np.uint64) if kind == 3 else out
+
+
+def walk(geom: Geometry, n_walkers, n_steps, steps_per_h, n_grid, sigma, ts, pp, source_lo, source_hi,
+         seed=0, first_walker=0, stream=None):
+    """Oracle FP64 walk (reference GPU semantics). Returns Y, inside_trace, escaped, err."""
+    P = OWalk()
+    P.n_walkers, P.n_steps, P.steps_per_h, P.n_grid = n_walkers, n_steps, steps_per_h, n_grid
+    P.sigma, P.ts, P.pp, P.source_lo, P.source_hi = sigma, ts, pp, source_lo, source_hi
+    P.seed, P.first_walker = int(seed), int(first_walker)
+    states = gi = gd = gu = None
+    if stream is None:
+        P.use_golden = 0
+        states = np.ascontiguousarray(rng_states(seed, first_walker, n_walkers).ravel())
+    else:
+        P.use_golden = 1
+        gi = np.ascontiguousarray(stream[0], dtype=np.float64)
+        gd = np.ascontiguousarray(stream[1], dtype=np.float64)
+        gu = np.ascontiguousarray(stream[2], dtype=np.float64)
+    Y = np.empty((n_walkers, n_grid, 3))
+    itr = np.empty((n_walkers, n_grid), np.int8)
+    esc = np.empty(n_walkers, np.int8)
+    err = np.empty(n_walkers, np.int8)
+    lib().og_walk(C.byref(geom.c), C.byref(P), _p(states, C.c_uint64), _p(gi, C.c_double), _p(gd, C.c_double),
+                  _p(gu, C.c_double), _p(Y, C.c_double), _p(itr, C.c_int8), _p(esc, C.c_int8), _p(err, C.c_int8))
+    return Y, itr, esc, err
+
+
+def reduce(Y, jd, jD, js, pc):
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    jd, jD, js = (np.ascontiguousarray(a, dtype=np.int32) for a in (jd, jD, js))
+    pc = np.ascontiguousarray(pc, dtype=np.float64)
+    cs = np.empty(len(pc))
+    sn = np.empty(len(pc))
+    lib().og_reduce(_p(Y, C.c_double), Y.shape[0], Y.shape[1], _p(jd, C.c_int32), _p(jD, C.c_int32),
+                    _p(js, C.c_int32), _p(pc, C.c_double), len(pc), _p(cs, C.c_double), _p(sn, C.c_double))
+    return cs, sn
+
+
+def reduce_reference(Y, jd, jD, js, pc):
+    """numpy restatement of _reduce_cpu (madi/walker_gpu.py:954-968)."""
+    cs = np.empty(len(pc))
+    sn = np.empty(len(pc))
+    for c in range(len(pc)):
+        ph = pc[c] * (Y[:, jd[c], :] + Y[:, jD[c], :] - Y[:, js[c], :])
+        cs[c] = np.cos(ph).sum()
+        sn[c] = np.sin(ph).sum()
+    return cs, sn
+
+
+def classify_ckdtree(seeds, annulus, pts, tree=None):
+    """numpy/scipy restatement of Ensemble.classify_cpu (madi/ensemble.py:213-256)."""
+    tree = tree or cKDTree(seeds)
+    d1, s1 = tree.query(pts, k=1)
+    s1 = np.asarray(s1, dtype=np.int32)
+    a1 = annulus[s1]
+    lists = tree.query_ball_point(pts, d1 + 2.0 * a1)
+    inside = np.ones(len(pts), dtype=bool)
+    for i, cand in enumerate(lists):
+        ids = np.asarray(cand, dtype=np.int32)
+        ids = ids[ids != s1[i]]
+        if len(ids) == 0:
+            continue
+        sep = np.linalg.norm(seeds[ids] - seeds[s1[i]], axis=1)
+        if np.any(sep == 0.0):
+            raise RuntimeError("Poisson seed realization contains coincident seeds")
+        diff = seeds[ids] - pts[i]
+        dj2 = np.einsum("ij,ij->i", diff, diff)
+        inside[i] = bool(np.all((dj2 - d1[i] ** 2) / (2.0 * sep) >= a1[i]))
+    return s1, inside
+
+
+def walk_cpu_reference(seeds, annulus, L, source_lo, source_hi, n_walkers, n_steps, steps_per_h, n_grid, sigma, ts,
+                       pp, seed=0, stream=None, free_water=False):
+    """numpy restatement of the reference CPU walk (madi/walker_gpu.py:698-840), exact classifier.
+
+    This is the CPU algorithm the benchmark times as the reference's CPU
+    fallback: vectorised draws and proposals, cKDTree nearest and ball
+    queries, and the per-point facet loop.
+    """
+    rng = np.random.default_rng(seed) if stream is None else None
+    n = int(n_walkers)
+    tree = None if free_water else cKDTree(seeds)
+    pos = (rng.uniform(source_lo, source_hi, size=(n, 3)) if stream is None
+           else np.asarray(stream[0], dtype=np.float64).copy())
+
+    def classify(p):
+        if free_water:
+            return np.zeros(len(p), np.int32), np.zeros(len(p), bool)
+        return classify_ckdtree(seeds, annulus, p, tree)
+
+    cell, inside = classify(pos)
+    Y = np.zeros((n, n_grid, 3))
+    itr = np.zeros((n, n_grid), np.int8)
+    itr[:, 0] = inside
+    escaped = np.zeros(n, bool)
+    prev = pos - L / 2.0
+    run = np.zeros((n, 3))
+    for step in range(n_steps):
+        active = ~escaped
+        if not np.any(active):
+            break
+        if stream is None:
+            inc = rng.normal(0.0, sigma, size=(n, 3))
+            acc = rng.uniform(0.0, 1.0, size=n)
+        else:
+            inc = np.asarray(stream[1][step])
+            acc = np.asarray(stream[2][step])
+        prop = pos + inc
+        out = ((prop < 0.0).any(axis=1) | (prop >= L).any(axis=1)) & active
+        escaped[out] = True
+        prop[out] = pos[out]
+        active = ~escaped
+        ncell, nin = classify(prop)
+        m = np.where(inside != nin, 1, 0)
+        m = np.where(inside & nin & (cell != ncell), 2, m)
+        cross = active & (m > 0)
+        if np.any(cross) and pp < 1.0:
+            rej = np.flatnonzero(cross)[acc[cross] >= pp ** m[cross]]
+            prop[rej] = pos[rej]
+            ncell[rej] = cell[rej]
+            nin[rej] = inside[rej]
+        pos[active] = prop[active]
+        cell[active] = ncell[active]
+        inside[active] = nin[active]
+        cur = pos - L / 2.0
+        run[active] += 0.5 * (prev[active] + cur[active]) * ts
+        prev[active] = cur[active]
+        k = step + 1
+        if k % steps_per_h == 0:
+            Y[:, k // steps_per_h, :] = run
+            itr[:, k // steps_per_h] = inside
+    return Y, itr, escaped.astype(np.int8)
+
+
+class Xoroshiro128p:
+    """Pure-Python xoroshiro128+ / splitmix64 (numba-src:45-125), independent of the C oracle."""
+
+    M = (1 << 64) - 1
+
+    def __init__(self, seed=None, state=None):
+        if state is not None:
+            self.s0, self.s1 = int(state[0]), int(state[1])
+        else:
+            z = (int(seed) + 0x9E3779B97F4A7C15) & self.M
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.M
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.M
+            z ^= z >> 31
+            self.s0 = self.s1 = z
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & Xoroshiro128p.M
+
+    def next(self):
+        a, b = self.s0, self.s1
+        out = (a + b) & self.M
+        b ^= a
+        self.s0 = self._rotl(a, 55) ^ b ^ ((b << 14) & self.M)
+        self.s1 = self._rotl(b, 36)
+        return out
+
+    def step_linear(self):
+        """State update only (the linear part)."""
+        self.next()

@@ -1,0 +1,50 @@
+"""Small seeded geometries for tests (no GPU needed to build them)."""
+
+from __future__ import annotations
+
+import numpy as np
+from scipy.spatial import cKDTree
+
+from custom_madi_amd.ensemble import (
+    Ensemble, GeometryStats, PopulationCertificate, alpha_star_from_vi, build_kdtree,
+    periodic_nearest_distances,
+)
+
+
+def _stats(side):
+    cert = PopulationCertificate(0.0, 0.0, True, 0, float("inf"), True, 0.0, 0)
+    return GeometryStats(0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, side, side,
+                         cert, "", {})
+
+
+def cellular(L=60.0, rho_per_uL=5.0e5, vi=0.6, seed=1, margin_spacings=3.0, alpha=None, mean_av=0.6):
+    """Non-periodic SI-style box: seeds in [-m, L+m)^3, annulus min(alpha*, 0.45 d_nn)."""
+    rng = np.random.default_rng(seed)
+    rho = rho_per_uL * 1e-9
+    m = margin_spacings * rho ** (-1.0 / 3.0)
+    n = rng.poisson(rho * (L + 2 * m) ** 3)
+    seeds = rng.uniform(-m, L + m, size=(n, 3))
+    nn = cKDTree(seeds).query(seeds, k=2)[0][:, 1]
+    a_star = alpha_star_from_vi(vi, rho_per_uL) if alpha is None else alpha
+    ann = np.minimum(a_star, 0.45 * nn)
+    nodes = build_kdtree(seeds)
+    src = 0.4 * L
+    return Ensemble(seeds=seeds, annulus=ann, rho=rho_per_uL, V=vi / rho / 1e3, vi=vi, alpha_star=a_star, L=L,
+                    source_lo=0.5 * (L - src), source_hi=0.5 * (L + src), mean_AV=mean_av, geometry=_stats(L),
+                    rho_requested=rho_per_uL, V_requested=vi / rho / 1e3, kd_node_seed=nodes[0],
+                    kd_node_axis=nodes[1], kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4])
+
+
+def periodic(n_cells=64, rho_per_uL=6.0e5, vi=0.6, seed=0, mean_av=0.6):
+    """Periodic cube of n_cells uniform seeds (the synthetic benchmark boxes)."""
+    rho = rho_per_uL * 1e-9
+    L = (n_cells / rho) ** (1.0 / 3.0)
+    rng = np.random.default_rng(seed)
+    seeds = rng.uniform(0.0, L, size=(n_cells, 3))
+    a_star = alpha_star_from_vi(vi, rho_per_uL)
+    ann = np.minimum(a_star, 0.45 * periodic_nearest_distances(seeds, L))
+    nodes = build_kdtree(seeds)
+    return Ensemble(seeds=seeds, annulus=ann, rho=rho_per_uL, V=vi / rho / 1e3, vi=vi, alpha_star=a_star, L=L,
+                    source_lo=0.0, source_hi=L, mean_AV=mean_av, geometry=_stats(L), rho_requested=rho_per_uL,
+                    V_requested=vi / rho / 1e3, kd_node_seed=nodes[0], kd_node_axis=nodes[1],
+                    kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], periodic=True)

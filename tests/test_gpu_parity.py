@@ -1,0 +1,213 @@
+"""HIP kernels vs the CPU oracle on identical inputs (run on an MI355X).
+
+Bars: labels and RNG states bit-exact; FP64 walks within the reference's
+golden tolerance rtol 5e-11 / atol 5e-12 (scripts/run_gpu_golden_check.py:27-28)
+with identical inside traces; FP64 sums within 1e-10 relative; the FP32
+production walk within |dS| <= 2e-3 and 3 Monte-Carlo standard errors of the
+FP64 walk on the same seeds.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from custom_madi_amd import _hip, engine
+from custom_madi_amd.ensemble import DeviceGeometry, geometry_arrays
+from tests import _geom
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def cell_geom():
+    ens = _geom.cellular(L=50.0, seed=3)
+    arr = geometry_arrays(ens)
+    return ens, arr, DeviceGeometry(arr), O.Geometry.from_arrays(arr)
+
+
+@pytest.fixture(scope="module")
+def big_geom():
+    ens = _geom.cellular(L=150.0, seed=9)
+    arr = geometry_arrays(ens)
+    return ens, arr, DeviceGeometry(arr), O.Geometry.from_arrays(arr)
+
+
+@pytest.fixture(scope="module")
+def periodic_geom():
+    ens = _geom.periodic(n_cells=64, seed=0)
+    arr = geometry_arrays(ens)
+    return ens, arr, DeviceGeometry(arr), O.Geometry.from_arrays(arr)
+
+
+def test_library_abi_version():
+    lib = _hip.load_library()
+    assert lib.madi_abi_version() == 1
+    info = _hip.device_info()
+    say(f"device: {info}")
+    assert info["cu_count"] > 0
+
+
+def test_exact_classifier_bit_exact_vs_oracle_and_brute(cell_geom):
+    ens, arr, dg, og = cell_geom
+    pts = np.random.default_rng(20_260_902).uniform(0.0, ens.L, size=(20_000, 3))
+    cell, inside, ok = dg.classify(pts)
+    ocell, oinside, ook = og.classify(pts)
+    assert ok.all() and ook.all()
+    assert np.array_equal(cell, ocell)
+    assert np.array_equal(inside, oinside)
+    bcell, binside = O.classify_brute_c(ens.seeds, ens.annulus, pts[:4000])
+    assert np.array_equal(cell[:4000], bcell) and np.array_equal(inside[:4000], binside)
+
+
+def test_graph_classifier_equals_exact(cell_geom):
+    ens, arr, dg, og = cell_geom
+    pts = np.random.default_rng(77).uniform(0.0, ens.L, size=(50_000, 3))
+    cell, inside, ok, cnt = dg.classify(pts, graph=True, counters=True)
+    ecell, einside, eok = dg.classify(pts)
+    say(f"graph classifier counters: {cnt}")
+    assert ok.all()
+    assert np.array_equal(inside, einside)
+    assert np.array_equal(cell, ecell)
+
+
+def test_periodic_classifiers_equal_tiling(periodic_geom):
+    ens, arr, dg, og = periodic_geom
+    pts = np.random.default_rng(4).uniform(-0.5 * ens.L, 1.5 * ens.L, size=(20_000, 3))
+    cell, inside, ok = dg.classify(pts)
+    gcell, ginside, gok = dg.classify(pts, graph=True)
+    ocell, oinside, _ = og.classify(pts)
+    assert ok.all() and gok.all()
+    assert np.array_equal(cell, ocell) and np.array_equal(inside, oinside)
+    assert np.array_equal(inside, ginside) and np.array_equal(cell, gcell)
+
+
+def test_rng_states_match_oracle():
+    for seed, first, n in ((0, 0, 1000), (20_260_901, 17, 513), (2 ** 31 - 1, 100_000, 64)):
+        dev = engine.rng_states(seed, first, n)
+        ref = O.rng_states(seed, first, n)
+        assert np.array_equal(dev, ref), (seed, first)
+
+
+def _walk_pair(ens, arr, dg, og, n, steps, sph, G, sigma, ts, pp, seed=0, stream=None):
+    p = engine.walk_params(n, _Cfg(steps, sph, G, sigma, ts), pp, seed)
+    out = engine.walk_y(dg, p, None if stream is None else _Stream(*stream))
+    Y, itr, esc, err = O.walk(og, n, steps, sph, G, sigma, ts, pp, arr.source_lo, arr.source_hi, seed=seed,
+                              stream=stream)
+    return out, (Y, itr, esc, err)
+
+
+class _Cfg:
+    def __init__(self, steps, sph, G, sigma, ts):
+        self.n_steps, self.steps_per_h, self.n_grid, self.sigma, self.ts = steps, sph, G, sigma, ts
+
+
+class _Stream:
+    def __init__(self, a, b, c):
+        self.initial_positions, self.increments, self.acceptance_uniforms = a, b, c
+
+
+def test_fp64_walk_injected_stream_matches_oracle(cell_geom):
+    ens, arr, dg, og = cell_geom
+    n, steps, sph, G = 256, 600, 50, 13
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.05
+    rng = np.random.default_rng(5)
+    stream = (rng.uniform(ens.source_lo, ens.source_hi, size=(n, 3)), rng.normal(0.0, sigma, size=(steps, n, 3)),
+              rng.uniform(0.0, 1.0, size=(steps, n)))
+    out, (Y, itr, esc, err) = _walk_pair(ens, arr, dg, og, n, steps, sph, G, sigma, ts, pp, stream=stream)
+    assert not err.any() and not out.err.any()
+    assert np.array_equal(out.escaped, esc)
+    assert np.array_equal(out.inside_trace, itr)
+    assert np.allclose(out.Y.cpu().numpy(), Y, rtol=5e-11, atol=5e-12)
+
+
+def test_fp64_walk_device_rng_matches_oracle(cell_geom):
+    ens, arr, dg, og = cell_geom
+    n, steps, sph, G = 512, 1000, 50, 21
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.02
+    out, (Y, itr, esc, err) = _walk_pair(ens, arr, dg, og, n, steps, sph, G, sigma, ts, pp, seed=104_729)
+    assert not err.any() and not out.err.any()
+    assert np.array_equal(out.escaped, esc)
+    assert np.array_equal(out.inside_trace, itr)
+    assert np.allclose(out.Y.cpu().numpy(), Y, rtol=5e-11, atol=5e-12)
+
+
+def test_fp64_walk_periodic_matches_oracle(periodic_geom):
+    ens, arr, dg, og = periodic_geom
+    n, steps, sph, G = 256, 1500, 50, 31
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.01
+    out, (Y, itr, esc, err) = _walk_pair(ens, arr, dg, og, n, steps, sph, G, sigma, ts, pp, seed=11)
+    assert not out.err.any() and not out.escaped.any()
+    assert np.array_equal(out.inside_trace, itr)
+    assert np.allclose(out.Y.cpu().numpy(), Y, rtol=5e-11, atol=5e-12)
+
+
+def test_reduce_matches_oracle():
+    torch = _hip.require_gpu()
+    rng = np.random.default_rng(2)
+    Y = rng.normal(0.0, 30.0, size=(5000, 9, 3))
+    jd, jD, js = np.array([1, 1, 2, 6]), np.array([4, 6, 5, 7]), np.array([5, 7, 7, 8])
+    pc = np.array([0.0, 0.013, 0.029, 0.05])
+    cs, sn = engine.reduce_columns(torch.from_numpy(Y).cuda(), jd, jD, js, pc)
+    ocs, osn = O.reduce(Y, jd, jD, js, pc)
+    assert np.allclose(cs, ocs, rtol=1e-11, atol=1e-9)
+    assert np.allclose(sn, osn, rtol=1e-11, atol=1e-9)
+
+
+def _spec(G, sph_ms=1.0, dirs=engine.AXIS_DIRECTIONS):
+    delta, Deltas, bs = 2.0, (5.0, 8.0, 10.0), (0.0, 1000.0, 2500.0, 6000.0)
+    pc = np.array([[engine.phase_coef(b, delta, D) for b in bs] for D in Deltas])
+    return engine.PhaseSpec(
+        j_delta=np.array([2, 2, 2]), j_Delta=np.array([5, 8, 10]), j_sum=np.array([7, 10, 12]), pc=pc, dirs=dirs)
+
+
+def test_fused_fp64_equals_walk_then_reduce(big_geom):
+    ens, arr, dg, og = big_geom
+    n, steps, sph, G = 4096, 650, 50, 14
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.03
+    cfg = _Cfg(steps, sph, G, sigma, ts)
+    spec = _spec(G)
+    p = engine.walk_params(n, cfg, pp, 321)
+    fused = engine.walk_fused(dg, p, spec, precision=64)
+    out = engine.walk_y(dg, p)
+    cols = [(k, b) for k in range(3) for b in range(4)]
+    jd = [spec.j_delta[k] for k, b in cols]
+    jD = [spec.j_Delta[k] for k, b in cols]
+    js = [spec.j_sum[k] for k, b in cols]
+    pcs = [spec.pc[k, b] for k, b in cols]
+    cs, sn = engine.reduce_columns(out.Y, jd, jD, js, pcs)
+    fc = fused.cos.sum(axis=2).ravel()
+    fs = fused.sin.sum(axis=2).ravel()
+    assert fused.n_escaped == 0 and fused.n_error == 0
+    assert np.array_equal(fused.occupancy, out.inside_trace.sum(axis=0))
+    assert np.allclose(fc, cs, rtol=1e-10, atol=1e-8)
+    assert np.allclose(fs, sn, rtol=1e-10, atol=1e-8)
+
+
+@pytest.mark.parametrize("which", ["big_geom", "periodic_geom"])
+def test_fp32_walk_matches_fp64_statistically(which, request):
+    ens, arr, dg, og = request.getfixturevalue(which)
+    n, steps, sph, G = 65_536, 650, 50, 14
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.03
+    cfg = _Cfg(steps, sph, G, sigma, ts)
+    spec = _spec(G)
+    p = engine.walk_params(n, cfg, pp, 987)
+    f64 = engine.walk_fused(dg, p, spec, precision=64)
+    f32 = engine.walk_fused(dg, p, spec, precision=32)
+    say(f"fp32 counters: {f32.counters}")
+    assert f32.n_escaped == 0 and f32.n_error == 0
+    S64 = f64.cos / n
+    S32 = f32.cos / n
+    dS = np.abs(S64 - S32)
+    say(f"max |dS| fp32 vs fp64 = {dS.max():.3e}")
+    assert dS.max() <= 2e-3
+    occ_rel = np.abs(f32.occupancy - f64.occupancy) / n
+    assert occ_rel.max() < 0.01
