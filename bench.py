@@ -1,0 +1,344 @@
+"""Benchmark of the MADI walk on one MI355X.
+
+Default workload (one library entry of the production single-entry config):
+periodic cube of 1,000 contracted Voronoi cells, rho = 5e5 cells/µL,
+V = 1.2 pL (v_i = 0.6), k_io = 22 s^-1 (p_p by Eq. 5), D = 2.0 µm²/ms,
+dt = 0.02 ms for 2,300 steps (46 ms), 1e6 walkers, PGSE delta = 6 ms,
+Delta = 15/25/30/40 ms, b = 0/1000/2500/4000/6000 s/mm², 24 directions:
+480 signals.  The FP32 production walk with the fused phase reduction is the
+timed path; a "step" is one full entry (1e6 walkers x 2,300 steps), inputs
+resident on the device.
+
+Prints one JSON line.  --gpus N > 1 exits: this benchmark uses one GPU.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Algorithmic FP32 operation counts (FMA = 2, other ops = 1) per unit of work:
+# per walker-step (3 Box-Muller normals, step, escape test, safe-ball test,
+# accept/reject, compensated Y), per facet (edge) evaluation, per walker
+# (480-column phase stage).  The facet count comes from the kernel's counters.
+F_STEP = 79
+F_FACET = 13
+F_WALKER = 2400
+PEAK_FP32_TFLOPS = 157.3        # MI355X FP32 vector peak (packed FMA)
+PEAK_FP32_NONPACKED_TFLOPS = 78.6
+PEAK_HBM_GBS = 8000.0
+
+
+def c1_workload(n_walkers=1_000_000):
+    from custom_madi_amd.config import SimConfig
+    from custom_madi_amd.ensemble import create_periodic_ensemble
+    from custom_madi_amd.walker_gpu import kio_to_pp
+    from custom_madi_amd import engine
+
+    rho, V, kio = 5.0e5, 1.2, 22.0
+    vi = rho * V * 1e-6
+    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=46.0, h_ms=1.0, n_walkers=n_walkers, small_deltas=[6.0],
+                    big_deltas=[15.0, 25.0, 30.0, 40.0], b_values=[0.0, 1000.0, 2500.0, 4000.0, 6000.0],
+                    periodic=True, precision="fp32")
+    ens = create_periodic_ensemble(1000, rho, vi, cfg, seed=0)
+    pp = kio_to_pp(kio, ens.mean_AV, cfg)
+    dirs = np.random.default_rng(1).normal(size=(24, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    delta = 6.0
+    Deltas = [15.0, 25.0, 30.0, 40.0]
+    pc = np.array([[engine.phase_coef(b, delta, D) for b in cfg.b_values] for D in Deltas])
+    spec = engine.PhaseSpec(j_delta=np.array([6] * 4), j_Delta=np.array([int(D) for D in Deltas]),
+                            j_sum=np.array([int(D + delta) for D in Deltas]), pc=pc, dirs=dirs)
+    return cfg, ens, pp, spec
+
+
+def cpu_baseline(ens, cfg, pp, threads, target_s=15.0, max_walkers=1_000_000):
+    """The CPU restatement of the reference walk (FP64 exact classifier), timed on host cores.
+
+    A 2,048-walker calibration run sizes the timed sample to about target_s
+    seconds (at most the entry's own walker count).
+    """
+    from oracle import oracle as O
+    from custom_madi_amd.ensemble import geometry_arrays
+    O.set_threads(threads)
+    og = O.Geometry.from_arrays(geometry_arrays(ens))
+
+    def run(n):
+        t0 = time.perf_counter()
+        O.walk(og, n, cfg.n_steps, cfg.steps_per_h, cfg.n_grid, cfg.sigma, cfg.ts, pp, ens.source_lo,
+               ens.source_hi, seed=0)
+        return time.perf_counter() - t0
+
+    dt = run(2048)
+    n = int(min(max_walkers, max(2048, target_s / dt * 2048)))
+    dt = run(n)
+    return n * cfg.n_steps / dt, dt, n
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def _noop(x):
+    return x
+
+
+def run_c2(args):
+    """256 uneven library entries: plain one-entry-per-launch form vs the batched form."""
+    from custom_madi_amd import batch, serve
+    reqs = batch.library_requests(args.entries, seed=2)
+    spec = batch.acquisition_spec(24)
+    pool = serve.make_pool(args.workers)           # spawned before the GPU is initialised
+    pool.map(_noop, range(args.workers))
+    from custom_madi_amd import _hip
+    _hip.require_gpu()
+    info = _hip.device_info()
+    log = lambda m: print(f"[bench c2] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    walker_steps = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs)
+    log(f"{len(reqs)} entries, {walker_steps:.3e} walker-steps, device {info['name']} CUs={info['cu_count']}")
+
+    # warm-up (kernel load, jump tables)
+    warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
+    batch.run_plain(warm, spec)
+
+    # batched / served form: all requests submitted at t0
+    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
+                                max_tiles_per_generation=args.gen_tiles)
+    srv = serve.Server(sch, pool=pool)
+    t0 = time.perf_counter()
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    t_batch = time.perf_counter() - t0
+    res_b = {r.key: r for r in srv.results}
+    log(f"batched: {t_batch:.1f} s, generations {sch.gen}")
+
+    # plain form on the first n_plain entries (sequential host prep, one launch each)
+    n_plain = min(args.plain_entries, len(reqs))
+    t0 = time.perf_counter()
+    res_p = batch.run_plain(reqs[:n_plain], spec)
+    t_plain = time.perf_counter() - t0
+    log(f"plain: {n_plain} entries in {t_plain:.1f} s")
+
+    # plain form with host geometry prepared beforehand (device-only comparison)
+    preps = pool.map(batch.prepare_entry, reqs[:n_plain])
+    t0 = time.perf_counter()
+    res_pg = batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps))
+    t_plain_dev = time.perf_counter() - t0
+    pool.close()
+
+    same = sum(int(np.array_equal(res_b[r.key].cos, r.cos) and np.array_equal(res_b[r.key].sin, r.sin)
+                   and np.array_equal(res_b[r.key].occupancy, r.occupancy)) for r in res_p)
+    ws_plain = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs[:n_plain])
+    eph_b = len(reqs) / t_batch * 3600.0
+    eph_p = n_plain / t_plain * 3600.0
+    eph_pd = n_plain / t_plain_dev * 3600.0
+    lat = [r.latency_s for r in srv.results]
+    esc = sum(r.n_escaped + r.n_error for r in srv.results)
+    out = {
+        "metric": "library entries/h", "value": eph_b, "unit": "entries/h", "n_gpus": 1, "higher_is_better": True,
+        "workload": "c2: 256 uneven entries (k_io log-U 1-130/s, V log-U 0.1-20 pL, v_i U 0.5-0.95, 50-4000 cells, "
+                    "2e5-1e6 walkers, dt 0.02 ms x 2300 steps, 480 signals), FP32 walk, FP64 sums",
+        "batched": {"entries": len(reqs), "wall_s": t_batch, "entries_per_hour": eph_b,
+                    "walker_steps_per_s": walker_steps / t_batch, "generations": sch.gen,
+                    "latency_s_p50": float(np.percentile(lat, 50)), "latency_s_p99": float(np.percentile(lat, 99)),
+                    "host_workers": args.workers, "escapes_or_errors": esc},
+        "plain": {"entries": n_plain, "wall_s": t_plain, "entries_per_hour": eph_p,
+                  "walker_steps_per_s": ws_plain / t_plain},
+        "plain_prebuilt_geometry": {"entries": n_plain, "wall_s": t_plain_dev, "entries_per_hour": eph_pd,
+                                    "walker_steps_per_s": ws_plain / t_plain_dev},
+        "batched_over_plain": eph_b / eph_p, "batched_over_plain_prebuilt": eph_b / eph_pd,
+        "bitwise_equal_plain_vs_batched": f"{same}/{n_plain}",
+        "classifier": sch.counter_dict(),
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def _prepared_lookup(preps):
+    table = {p.req.key: p for p in preps}
+    return lambda req: table[req.key]
+
+
+def run_c3(args):
+    """Serving: Poisson arrivals of entry requests, continuous batching, p50/p99 latency."""
+    from custom_madi_amd import batch, serve
+    arrivals = serve.poisson_arrivals(args.rate, args.duration, seed=3)
+    reqs = batch.library_requests(len(arrivals), seed=3, walkers=(100_000, 1_000_000))
+    spec = batch.acquisition_spec(24)
+    pool = serve.make_pool(args.workers)
+    pool.map(_noop, range(args.workers))
+    from custom_madi_amd import _hip
+    _hip.require_gpu()
+    log = lambda m: print(f"[bench c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
+    batch.run_plain(warm, spec)
+    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
+                                max_tiles_per_generation=args.gen_tiles)
+    stream = serve.LibraryStream(args.library_out)
+    srv = serve.Server(sch, pool=pool, stream=stream)
+    offered = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs) / args.duration
+    log(f"{len(reqs)} requests over {args.duration:.0f} s, offered {offered:.3e} walker-steps/s")
+    rep = serve.serve_poisson(srv, reqs, arrivals, progress=log)
+    stream.close()
+    pool.close()
+    hbm = rep.pop("hbm_used_bytes_samples")
+    out = {"metric": "p50/p99 entry latency", "unit": "s", "n_gpus": 1, "higher_is_better": False,
+           "workload": f"c3: Poisson {args.rate}/s for {args.duration:.0f} s, entries as c2 with 1e5-1e6 walkers, "
+                       "480 signals", "offered_walker_steps_per_s": offered,
+           "hbm_used_gib_max": max((b for _, b in hbm), default=0) / 2 ** 30,
+           "hbm_used_gib_samples": [[round(t, 1), round(b / 2 ** 30, 2)] for t, b in hbm], **rep,
+           "classifier": sch.counter_dict()}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--walkers", type=int, default=1_000_000)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-fp64-check", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--entries", type=int, default=256)
+    ap.add_argument("--plain-entries", type=int, default=256)
+    ap.add_argument("--workers", type=int, default=16)
+    ap.add_argument("--grid-per-cu", type=int, default=0)
+    ap.add_argument("--tier-cap", type=float, default=0.0)
+    ap.add_argument("--gen-tiles", type=int, default=65536)
+    ap.add_argument("--rate", type=float, default=20.0)
+    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--library-out", default=None)
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (entries shard across GPUs "
+              "independently, see DESIGN.md)", file=sys.stderr)
+        return 2
+    if args.workload == "c2":
+        return run_c2(args)
+    if args.workload == "c3":
+        return run_c3(args)
+
+    import torch
+    from custom_madi_amd import _hip, engine
+    _hip.require_gpu()
+    info = _hip.device_info()
+
+    if args.copy_probe:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    cfg, ens, pp, spec = c1_workload(args.walkers)
+    dg = ens.device_geometry()
+    N = args.walkers
+    runner = engine.FusedRunner(N, spec, cfg.n_grid)
+    params = engine.walk_params(N, cfg, pp, seed=0, tier_cap_um=args.tier_cap)
+    print(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}; workload c1 walkers={N} "
+          f"steps={cfg.n_steps} pp={pp:.6g}", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        runner.launch(dg, params, 32)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        starts[k].record(stream)
+        runner.launch(dg, params, 32)
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    res = runner.result(N)
+    kern_ms = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    ms_per_step = 1e3 * wall / args.steps
+    walker_steps = N * cfg.n_steps
+    value = walker_steps * args.steps / wall
+    if res.n_escaped or res.n_error:
+        raise RuntimeError(f"walk reported escapes={res.n_escaped} errors={res.n_error}")
+
+    # per launch counters are accumulated over every launch
+    launches = args.warmup + args.steps
+    facets = res.counters["facets"] / launches
+    flops = walker_steps * F_STEP + facets * F_FACET + N * F_WALKER
+    kavg = float(np.mean(kern_ms)) * 1e-3
+    achieved = flops / kavg / 1e12
+    traffic = None
+    pmc = ROOT / "profiles" / "walk_f32_c1_traffic.json"
+    if pmc.is_file():
+        traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
+
+    out = {
+        "metric": "walker-steps/s", "value": value, "unit": "walker-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 final sums)", "data": "synthetic",
+        "config": {"workload": "c1: periodic 1000-cell box, rho 5e5/uL, V 1.2 pL, k_io 22/s, D 2.0, dt 0.02 ms "
+                               "x 2300 steps, 1e6 walkers, 4 Delta x 5 b x 24 dirs = 480 signals",
+                   "walkers": N, "steps_per_walk": cfg.n_steps, "cells": 1000, "signals": 480},
+        "entries_per_hour": 3600.0 / (ms_per_step / 1e3),
+        "entry_latency_ms_p50": float(np.percentile(kern_ms, 50)),
+        "entry_latency_ms_p99": float(np.percentile(kern_ms, 99)),
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "frac_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
+                     "kernel_ms": kavg * 1e3, "algorithmic_tflop_per_launch": flops / 1e12},
+        "classifier": {k: v / launches for k, v in res.counters.items()},
+    }
+
+    if not args.no_fp64_check:
+        t64 = time.perf_counter()
+        f64 = engine.walk_fused(dg, params, spec, precision=64)
+        t64 = time.perf_counter() - t64
+        S32 = res.cos / N
+        S64 = f64.cos / N
+        dS = np.abs(S32 - S64)
+        se = np.sqrt(np.maximum(1.0 - S64 ** 2, 0.0) / N)
+        per = dS.max(axis=2)  # (Delta, b)
+        out["fp64_check"] = {
+            "max_abs_dS": float(dS.max()), "max_abs_dS_over_3se": float((dS / (3 * se + 1e-300)).max()),
+            "per_Delta_b_max_abs_dS": per.tolist(), "fp64_escaped": f64.n_escaped,
+            "fp64_walk_s": t64, "fp64_kd_nodes": f64.counters["kd_nodes"],
+        }
+
+    if not args.no_cpu_baseline:
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        threads = max(1, min(16, cores))
+        rate, dt, n_cpu = cpu_baseline(ens, cfg, pp, threads, target_s=args.cpu_seconds, max_walkers=N)
+        out["cpu_baseline"] = {
+            "value": rate, "unit": "walker-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{n_cpu} walkers x {cfg.n_steps} steps of the same entry, FP64 exact-classifier "
+                      f"walk (oracle/madi_oracle.c, OpenMP) in {dt:.1f} s on {cpu_model()}",
+        }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -57,9 +57,24 @@ class GeomDesc(C.Structure):
 class WalkParams(C.Structure):
     _fields_ = [
         ("n_walkers", C.c_int64), ("n_steps", C.c_int32), ("steps_per_h", C.c_int32), ("n_grid", C.c_int32),
-        ("reserved", C.c_int32), ("sigma", C.c_double), ("ts", C.c_double), ("pp", C.c_double),
-        ("seed", C.c_uint64), ("first_walker", C.c_uint64),
+        ("tier_cap_um", C.c_float), ("sigma", C.c_double), ("ts", C.c_double), ("pp", C.c_double),
+        ("seed", C.c_uint64), ("first_walker", C.c_uint64), ("scan_chunk", C.c_int32), ("reserved", C.c_int32),
     ]
+
+
+class Work(C.Structure):
+    _fields_ = [
+        ("geom", C.c_void_p), ("walk", WalkParams), ("tile_lo", C.c_int64), ("n_tiles", C.c_int64),
+        ("d_partial", C.c_void_p), ("d_occupancy", C.c_void_p), ("d_flags", C.c_void_p),
+    ]
+
+
+class Tune(C.Structure):
+    _fields_ = [("grid_per_cu", C.c_int32), ("tier_cap_um", C.c_float), ("scan_chunk", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
+TILE_WALKERS = 256
 
 
 class PhasePlan(C.Structure):
@@ -95,6 +110,11 @@ SIGNATURES = {
     "madi_walk_fused_workspace": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_walk_fused": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, C.c_int64,
                                   _p, _p, _p, _p, _p, _p]),
+    "madi_batch_workspace": (C.c_int, [C.c_int32, _i64p]),
+    "madi_batch_launch": (C.c_int, [C.POINTER(Work), C.c_int32, C.POINTER(PhasePlan), C.POINTER(Tune), _p, C.c_int64,
+                                    _p, _p]),
+    "madi_tiles_finalize": (C.c_int, [C.POINTER(PhasePlan), _p, C.c_int64, _p, _p, _p]),
+    "madi_fill_i64": (C.c_int, [_p, C.c_int64, C.c_int64, _p]),
 }
 
 _lib: Optional[C.CDLL] = None

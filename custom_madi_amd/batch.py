@@ -1,0 +1,358 @@
+"""Batched and served library-entry simulation on one GPU.
+
+An entry is one (k_io, rho, V) simulation request with its own cell count and
+walker count.  Its walkers are cut into tiles of 256 (one workgroup each).
+``DeviceScheduler`` keeps admitted entries' geometry and per-tile result slots
+on the device and launches *generations*: one persistent work-stealing kernel
+(madi_batch_launch) over the pending tiles of every active entry.  New entries
+join the next generation as soon as they are admitted, so entries of uneven
+size share launches and the GPU stays full; an entry is finalised (its tile
+slots summed in tile order) when its last tile's generation completes.
+Because every tile writes its own slot, an entry's result does not depend on
+which other entries shared its launches.
+
+Host geometry preparation (seeds, annulus, Delaunay graph, hint grid) is pure
+CPU work and runs in worker processes (``prepare_entry``), overlapped with the
+device work.  The plain form (``run_plain``) is the one-entry-per-launch path:
+prepare, upload, one launch, synchronise, next entry.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+from collections import deque
+from dataclasses import dataclass, field
+from typing import Callable, Iterable, List, Optional
+
+import numpy as np
+
+from . import _hip, engine
+from .config import SimConfig
+from .ensemble import DeviceGeometry, GeometryArrays, create_periodic_ensemble, geometry_arrays
+
+
+@dataclass
+class EntryRequest:
+    """One library entry to simulate (periodic box of n_cells contracted cells)."""
+
+    key: tuple
+    kio: float
+    rho: float        # cells / µL
+    V: float          # pL
+    n_cells: int
+    n_walkers: int
+    seed: int = 0        # geometry seed
+    walk_seed: int = 0
+    D0: float = 2.0
+    ts: float = 0.02
+    T_max_ms: float = 46.0
+    h_ms: float = 1.0
+    submitted: float = 0.0
+
+    def config(self) -> SimConfig:
+        return SimConfig(D0=self.D0, ts=self.ts, T_max_ms=self.T_max_ms, h_ms=self.h_ms, n_walkers=self.n_walkers,
+                         small_deltas=[1.0], big_deltas=[1.0], b_values=[0.0], periodic=True, precision="fp32")
+
+
+@dataclass
+class PreparedEntry:
+    req: EntryRequest
+    arrays: GeometryArrays
+    pp: float
+    mean_AV: float
+    alpha_star: float
+    L: float
+    prep_s: float
+
+
+def prepare_entry(req: EntryRequest) -> PreparedEntry:
+    """Host geometry for one entry (CPU only; safe to run in a worker process)."""
+    from .walker_gpu import kio_to_pp
+
+    t0 = time.perf_counter()
+    cfg = req.config()
+    vi = req.rho * req.V * 1e-6
+    ens = create_periodic_ensemble(int(req.n_cells), req.rho, vi, cfg, seed=req.seed, with_kd=False)
+    arrays = geometry_arrays(ens, with_kd=False)
+    pp = 0.0 if req.kio == 0.0 else kio_to_pp(req.kio, ens.mean_AV, cfg)
+    return PreparedEntry(req, arrays, pp, ens.mean_AV, ens.alpha_star, ens.L, time.perf_counter() - t0)
+
+
+@dataclass
+class EntryResult:
+    key: tuple
+    cos: np.ndarray        # (n_pairs, n_b, n_dirs) sums over walkers
+    sin: np.ndarray
+    occupancy: np.ndarray  # (n_grid,)
+    n_walkers: int
+    n_escaped: int
+    n_error: int
+    pp: float
+    submitted: float
+    admitted: float
+    finished: float
+    prep_s: float
+
+    @property
+    def signal(self) -> np.ndarray:
+        return self.cos / self.n_walkers
+
+    @property
+    def latency_s(self) -> float:
+        return self.finished - self.submitted
+
+
+@dataclass
+class _Active:
+    prep: PreparedEntry
+    geom: DeviceGeometry
+    params: _hip.WalkParams
+    n_tiles: int
+    partial: object
+    occ: object
+    flags: object
+    next_tile: int = 0
+    last_gen: int = -1
+    admitted: float = 0.0
+    device_bytes: int = 0
+
+
+class DeviceScheduler:
+    """Admission, generations and finalisation of entries on the current GPU."""
+
+    def __init__(self, spec: engine.PhaseSpec, grid_per_cu: int = 0, tier_cap_um: float = 0.0,
+                 max_tiles_per_generation: int = 65536, hbm_reserve_bytes: int = 8 << 30,
+                 max_inflight: int = 2):
+        torch = _hip.require_gpu()
+        self.torch = torch
+        self.lib = _hip.load_library()
+        self.spec = spec
+        self.plan = spec.to_ctypes()
+        self.n_cols = int(np.prod(spec.shape))
+        self.tune = _hip.Tune()
+        self.tune.grid_per_cu = int(grid_per_cu)
+        self.tune.tier_cap_um = float(tier_cap_um)
+        self.max_tiles = int(max_tiles_per_generation)
+        self.reserve = int(hbm_reserve_bytes)
+        self.max_inflight = int(max_inflight)
+        self.active: List[_Active] = []
+        self.inflight: deque = deque()   # (gen id, event, workspace, [entries])
+        self.gen = 0
+        self.counters = torch.zeros(_hip.N_COUNTERS, dtype=torch.int64, device="cuda")
+        self.hbm_samples: list = []
+        # finalisation runs on its own stream (default priority) so it does not
+        # wait behind generations queued after the one that completed
+        self.side = torch.cuda.Stream()
+
+    # ------------------------------------------------------------ admission
+    def entry_bytes(self, prep: PreparedEntry) -> int:
+        tiles = -(-prep.req.n_walkers // _hip.TILE_WALKERS)
+        a = prep.arrays
+        geom = (a.nb_vec.nbytes + a.nb_ids.nbytes * 2 + a.nb_offsets.nbytes + a.cell_xyz.nbytes * 2
+                + a.hint.nbytes + a.cell_annulus.nbytes * 3)
+        return tiles * self.n_cols * 16 + geom + 4096
+
+    def can_admit(self, prep: PreparedEntry) -> bool:
+        free, _ = self.torch.cuda.mem_get_info()
+        return free - self.entry_bytes(prep) > self.reserve
+
+    def admit(self, prep: PreparedEntry) -> bool:
+        """Upload an entry if free HBM allows; returns False otherwise."""
+        if not self.can_admit(prep):
+            return False
+        torch = self.torch
+        req = prep.req
+        cfg = req.config()
+        geom = DeviceGeometry(prep.arrays)
+        params = engine.walk_params(req.n_walkers, cfg, prep.pp, req.walk_seed)
+        tiles = -(-req.n_walkers // _hip.TILE_WALKERS)
+        partial = torch.empty(tiles * self.n_cols * 2, dtype=torch.float64, device="cuda")
+        occ = torch.empty(cfg.n_grid, dtype=torch.int64, device="cuda")
+        flags = torch.empty(_hip.N_FLAGS, dtype=torch.int64, device="cuda")
+        s = _hip.stream_ptr()
+        _hip.check(self.lib.madi_fill_i64(_hip.dptr(occ), cfg.n_grid, 0, s), "madi_fill_i64")
+        _hip.check(self.lib.madi_fill_i64(_hip.dptr(flags), _hip.N_FLAGS, 0, s), "madi_fill_i64")
+        self.active.append(_Active(prep, geom, params, tiles, partial, occ, flags, admitted=time.perf_counter(),
+                                   device_bytes=self.entry_bytes(prep)))
+        return True
+
+    # ---------------------------------------------------------- generations
+    def pending_tiles(self) -> int:
+        return sum(a.n_tiles - a.next_tile for a in self.active)
+
+    def launch_generation(self) -> bool:
+        """Launch the pending tiles of active entries (FIFO, up to the budget)."""
+        if len(self.inflight) >= self.max_inflight:
+            return False
+        items, owners = [], []
+        budget = self.max_tiles
+        for a in self.active:
+            if budget <= 0:
+                break
+            left = a.n_tiles - a.next_tile
+            if left <= 0:
+                continue
+            take = min(left, budget)
+            w = _hip.Work()
+            w.geom = a.geom.handle.value
+            w.walk = a.params
+            w.tile_lo = a.next_tile
+            w.n_tiles = take
+            w.d_partial = _hip.dptr(a.partial)
+            w.d_occupancy = _hip.dptr(a.occ)
+            w.d_flags = _hip.dptr(a.flags)
+            items.append(w)
+            owners.append(a)
+            a.next_tile += take
+            a.last_gen = self.gen
+            budget -= take
+        if not items:
+            return False
+        arr = (_hip.Work * len(items))(*items)
+        need = C.c_int64()
+        _hip.check(self.lib.madi_batch_workspace(len(items), C.byref(need)), "madi_batch_workspace")
+        ws = self.torch.empty(int(need.value), dtype=torch_uint8(self.torch), device="cuda")
+        s = _hip.stream_ptr()
+        rc = self.lib.madi_batch_launch(arr, len(items), C.byref(self.plan), C.byref(self.tune), _hip.dptr(ws),
+                                        int(need.value), _hip.dptr(self.counters), s)
+        _hip.check(rc, "madi_batch_launch")
+        ev = self.torch.cuda.Event()
+        ev.record()
+        self.inflight.append((self.gen, ev, ws, owners))
+        self.gen += 1
+        return True
+
+    def _finalize(self, a: _Active) -> EntryResult:
+        """Sum the entry's tile slots and copy its results (its generation has completed)."""
+        torch = self.torch
+        with torch.cuda.stream(self.side):
+            cs = torch.empty(self.n_cols, dtype=torch.float64, device="cuda")
+            sn = torch.empty(self.n_cols, dtype=torch.float64, device="cuda")
+            rc = self.lib.madi_tiles_finalize(C.byref(self.plan), _hip.dptr(a.partial), a.n_tiles, _hip.dptr(cs),
+                                              _hip.dptr(sn), _hip.stream_ptr())
+            _hip.check(rc, "madi_tiles_finalize")
+            shape = self.spec.shape
+            flags = a.flags.cpu().numpy()
+            cos, sin, occ = cs.cpu().numpy().reshape(shape), sn.cpu().numpy().reshape(shape), a.occ.cpu().numpy()
+        req = a.prep.req
+        return EntryResult(
+            key=req.key, cos=cos, sin=sin, occupancy=occ, n_walkers=req.n_walkers, n_escaped=int(flags[0]),
+            n_error=int(flags[1]), pp=a.prep.pp, submitted=req.submitted, admitted=a.admitted,
+            finished=time.perf_counter(), prep_s=a.prep.prep_s,
+        )
+
+    def poll(self, block: bool = False) -> List[EntryResult]:
+        """Finalise entries whose last generation has completed."""
+        done_gen = -1
+        while self.inflight:
+            gen, ev, ws, owners = self.inflight[0]
+            if block:
+                ev.synchronize()
+            elif not ev.query():
+                break
+            done_gen = gen
+            self.inflight.popleft()
+            block = False
+        if done_gen < 0:
+            return []
+        out, keep = [], []
+        for a in self.active:
+            if a.next_tile >= a.n_tiles and a.last_gen <= done_gen:
+                out.append(self._finalize(a))
+            else:
+                keep.append(a)
+        self.active = keep
+        if out:
+            free, total = self.torch.cuda.mem_get_info()
+            self.hbm_samples.append((time.perf_counter(), total - free))
+        return out
+
+    def drain(self) -> List[EntryResult]:
+        out = []
+        while self.active or self.inflight:
+            self.launch_generation()
+            out += self.poll(block=True)
+        return out
+
+    def counter_dict(self) -> dict:
+        return dict(zip(_hip.COUNTER_NAMES, self.counters.cpu().numpy().tolist()))
+
+
+def torch_uint8(torch):
+    return torch.uint8
+
+
+def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, **kw) -> List[EntryResult]:
+    """Admit every prepared entry as it arrives and run generations until all finish."""
+    sch = DeviceScheduler(spec, **kw)
+    waiting: deque = deque()
+    results: List[EntryResult] = []
+    for prep in preps:
+        waiting.append(prep)
+        while waiting and sch.admit(waiting[0]):
+            waiting.popleft()
+        if sch.pending_tiles() >= sch.max_tiles or len(sch.inflight) == 0:
+            sch.launch_generation()
+        results += sch.poll()
+    while waiting or sch.active or sch.inflight:
+        while waiting and sch.admit(waiting[0]):
+            waiting.popleft()
+        if not sch.launch_generation() and not sch.inflight and waiting:
+            raise MemoryError("an entry does not fit in free HBM even with the device idle")
+        results += sch.poll(block=True)
+    return results
+
+
+def run_plain(requests: Iterable[EntryRequest], spec: engine.PhaseSpec,
+              prepare: Callable[[EntryRequest], PreparedEntry] = prepare_entry) -> List[EntryResult]:
+    """One entry per launch: prepare, upload, launch, synchronise, in sequence."""
+    results = []
+    for req in requests:
+        t_admit = time.perf_counter()
+        prep = prepare(req)
+        geom = DeviceGeometry(prep.arrays)
+        params = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed)
+        res = engine.walk_fused(geom, params, spec, precision=32)
+        results.append(EntryResult(
+            key=req.key, cos=res.cos, sin=res.sin, occupancy=res.occupancy, n_walkers=req.n_walkers,
+            n_escaped=res.n_escaped, n_error=res.n_error, pp=prep.pp, submitted=req.submitted, admitted=t_admit,
+            finished=time.perf_counter(), prep_s=prep.prep_s,
+        ))
+        del geom
+    return results
+
+
+def library_requests(n: int, seed: int = 2, walkers=(200_000, 1_000_000), cells=(50, 4000), kio=(1.0, 130.0),
+                     V=(0.1, 20.0), vi=(0.5, 0.95), ts: float = 0.02, D0: float = 2.0,
+                     T_max_ms: float = 46.0) -> List[EntryRequest]:
+    """Uneven library entries: k_io and V log-uniform, v_i uniform, cells and walkers uniform."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n):
+        k = float(np.exp(rng.uniform(np.log(kio[0]), np.log(kio[1]))))
+        v = float(np.exp(rng.uniform(np.log(V[0]), np.log(V[1]))))
+        f = float(rng.uniform(vi[0], vi[1]))
+        rho = f / (v * 1e-6)
+        c = int(rng.integers(cells[0], cells[1] + 1))
+        w = int(rng.integers(walkers[0], walkers[1] + 1))
+        out.append(EntryRequest(key=(i, round(k, 4), round(rho, 1), round(v, 6)), kio=k, rho=rho, V=v, n_cells=c,
+                                n_walkers=w, seed=1000 + i, walk_seed=104_729 * (i + 1) & 0x7FFFFFFF, ts=ts, D0=D0,
+                                T_max_ms=T_max_ms))
+    return out
+
+
+def acquisition_spec(n_dirs: int = 24, delta: float = 6.0, Deltas=(15.0, 25.0, 30.0, 40.0),
+                     b_values=(0.0, 1000.0, 2500.0, 4000.0, 6000.0), h_ms: float = 1.0,
+                     dir_seed: int = 1) -> engine.PhaseSpec:
+    """PGSE acquisition: one delta, several Delta, b values and n_dirs directions uniform on the sphere."""
+    if n_dirs == 3:
+        dirs = np.eye(3)
+    else:
+        dirs = np.random.default_rng(dir_seed).normal(size=(n_dirs, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    pc = np.array([[engine.phase_coef(b, delta, D) for b in b_values] for D in Deltas])
+    j = lambda t: int(round(t / h_ms))  # noqa: E731
+    return engine.PhaseSpec(j_delta=np.array([j(delta)] * len(Deltas)), j_Delta=np.array([j(D) for D in Deltas]),
+                            j_sum=np.array([j(D + delta) for D in Deltas]), pc=pc, dirs=dirs)

@@ -9,10 +9,16 @@ packed FP32 instructions.
 from __future__ import annotations
 
 import os
-import shutil
-import subprocess
 import sys
-from pathlib import Path
+
+# Run as a script, this directory would shadow the standard library's
+# ``signal`` module with the package's signal.py.
+if sys.path and os.path.abspath(sys.path[0]) == os.path.dirname(os.path.abspath(__file__)):
+    sys.path.pop(0)
+
+import shutil  # noqa: E402
+import subprocess  # noqa: E402
+from pathlib import Path  # noqa: E402
 
 HERE = Path(__file__).resolve().parent
 CSRC = HERE / "csrc"

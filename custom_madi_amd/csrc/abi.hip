@@ -6,7 +6,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <algorithm>
 #include <mutex>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -169,25 +171,13 @@ static void free_geom(madi_geom* g)
     g->allocs.clear();
 }
 
-static int build_geom(const madi_geom_desc* d, madi_geom* g)
+// Exact KD classifier arrays: the pre-order tree relabelled breadth-first so
+// its first levels form a contiguous prefix (staged in LDS by the kernels).
+static int build_kd(const madi_geom_desc* d, madi_geom* g)
 {
     GeomDev& D = g->dev;
-    memset(&D, 0, sizeof D);
-    D.L = d->L;
-    D.source_lo = d->source_lo;
-    D.source_hi = d->source_hi;
-    D.periodic = d->periodic;
-    D.free_water = d->free_water;
-    int rc;
-    if (d->free_water) return MADI_OK;
-
-    // exact KD classifier: relabel the pre-order tree breadth-first
     const int n = d->n_kd;
-    if (n < 1 || !d->kd_xyz || !d->kd_annulus || !d->node_seed || !d->node_axis || !d->node_left ||
-        !d->node_right || !d->node_parent) {
-        set_error("geometry: KD arrays are required for a cellular geometry (n_kd=%d)", n);
-        return MADI_ERR_ARG;
-    }
+    int rc;
     std::vector<int> order;
     order.reserve(n);
     std::vector<int> new_id(n, -1);
@@ -232,6 +222,33 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
     if ((rc = upload(g, d->kd_xyz, (size_t)n * 3, &D.kd_seed_xyz))) return rc;
     D.n_kd = n;
     D.n_kd_lds = n < 511 ? n : 511;
+    return MADI_OK;
+}
+
+static int build_geom(const madi_geom_desc* d, madi_geom* g)
+{
+    GeomDev& D = g->dev;
+    memset(&D, 0, sizeof D);
+    D.L = d->L;
+    D.source_lo = d->source_lo;
+    D.source_hi = d->source_hi;
+    D.periodic = d->periodic;
+    D.free_water = d->free_water;
+    int rc;
+    if (d->free_water) return MADI_OK;
+
+    // exact KD classifier: relabel the pre-order tree breadth-first
+    const int n = d->n_kd;
+    if (n > 0 && (!d->kd_xyz || !d->kd_annulus || !d->node_seed || !d->node_axis || !d->node_left ||
+                  !d->node_right || !d->node_parent)) {
+        set_error("geometry: incomplete KD arrays (n_kd=%d)", n);
+        return MADI_ERR_ARG;
+    }
+    if (n < 1 && !(d->n_cells > 0 && d->nb_offsets)) {
+        set_error("geometry: a cellular geometry needs a KD tree or a Delaunay graph");
+        return MADI_ERR_ARG;
+    }
+    if (n > 0 && (rc = build_kd(d, g))) return rc;
 
     // Delaunay graph (optional: walks that only use the exact classifier may omit it)
     const int nc = d->n_cells;
@@ -243,14 +260,21 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
         }
         std::vector<int32_t> off(nc + 1);
         for (int c = 0; c <= nc; ++c) off[c] = (int32_t)d->nb_offsets[c];
+        // each cell's edges sorted by half length (the FP32 walk stops its edge
+        // loop once h_j - |p| passes the bound it needs)
         std::vector<float4> rec(E);
+        std::vector<int32_t> ids(E);
+        std::vector<double> vec(3 * E);
         std::vector<float> alpha(nc), hmax(nc, 0.f);
         std::vector<double> alpha64(nc);
-        float amin = INFINITY;
+        std::vector<std::pair<double, int64_t>> order;
+        float amin = INFINITY, amax = 0.f;
         for (int c = 0; c < nc; ++c) {
             alpha64[c] = d->cell_annulus[c];
             alpha[c] = (float)d->cell_annulus[c];
             amin = fminf(amin, alpha[c]);
+            amax = fmaxf(amax, alpha[c]);
+            order.clear();
             for (int64_t k = off[c]; k < off[c + 1]; ++k) {
                 const int nb = d->nb_ids[k];
                 if (nb < 0 || nb >= nc) {
@@ -263,14 +287,27 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
                     set_error("Poisson seed realization contains coincident seeds");
                     return MADI_ERR_CLASSIFIER;
                 }
-                rec[k] = make_float4((float)(ex / len), (float)(ey / len), (float)(ez / len), (float)(0.5 * len));
+                order.emplace_back(len, k);
+            }
+            std::sort(order.begin(), order.end());
+            int64_t dst = off[c];
+            for (const auto& pr : order) {
+                const int64_t k = pr.second;
+                const double ex = d->nb_vec[3 * k], ey = d->nb_vec[3 * k + 1], ez = d->nb_vec[3 * k + 2];
+                const double len = pr.first;
+                rec[dst] = make_float4((float)(ex / len), (float)(ey / len), (float)(ez / len), (float)(0.5 * len));
+                ids[dst] = d->nb_ids[k];
+                vec[3 * dst] = ex;
+                vec[3 * dst + 1] = ey;
+                vec[3 * dst + 2] = ez;
                 hmax[c] = fmaxf(hmax[c], (float)(0.5 * len));
+                ++dst;
             }
         }
         if ((rc = upload(g, off.data(), off.size(), &D.nb_off))) return rc;
         if ((rc = upload(g, rec.data(), rec.size(), &D.nb_rec))) return rc;
-        if ((rc = upload(g, d->nb_ids, (size_t)E, &D.nb_id))) return rc;
-        if ((rc = upload(g, d->nb_vec, (size_t)E * 3, &D.nb_e64))) return rc;
+        if ((rc = upload(g, ids.data(), ids.size(), &D.nb_id))) return rc;
+        if ((rc = upload(g, vec.data(), vec.size(), &D.nb_e64))) return rc;
         if ((rc = upload(g, alpha.data(), alpha.size(), &D.cell_alpha))) return rc;
         if ((rc = upload(g, alpha64.data(), alpha64.size(), &D.cell_alpha64))) return rc;
         if ((rc = upload(g, hmax.data(), hmax.size(), &D.cell_hmax))) return rc;
@@ -290,11 +327,16 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
         D.hint_n = d->hint_n;
         D.hint_h = d->L / d->hint_n;
         D.alpha_min = amin;
+        D.alpha_max = amax;
     }
     return MADI_OK;
 }
 
 static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != nullptr; }
+static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
+
+static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
+static int chunk_of(int v) { return v > 0 ? v : 8; }
 
 static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Events& ev)
 {
@@ -329,6 +371,7 @@ static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Ev
         P.dirs[k] = plan->dirs[k];
         P.dirs32[k] = (float)plan->dirs[k];
     }
+    ev.n_pairs = plan->n_pairs;
     // events sorted by grid index (stable in pair order)
     for (int j = 0; j < n_grid; ++j)
         for (int p = 0; p < plan->n_pairs; ++p) {
@@ -385,18 +428,20 @@ static int make_walk(const madi_walk_params* p, WalkDev& W)
     return MADI_OK;
 }
 
-__global__ void k_zero_i64(long long* a, int n)
+__global__ void k_fill_i64(long long* a, int64_t n, long long v)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] = 0;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
 }
 
-static int zero_i64(long long* a, int n, hipStream_t s)
+static int fill_i64(long long* a, int64_t n, long long v, hipStream_t s)
 {
     if (!a || n <= 0) return MADI_OK;
-    hipLaunchKernelGGL(k_zero_i64, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
-    return check_launch("k_zero_i64");
+    hipLaunchKernelGGL(k_fill_i64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, n, v);
+    return check_launch("k_fill_i64");
 }
+
+static int zero_i64(long long* a, int n, hipStream_t s) { return fill_i64(a, n, 0, s); }
 
 }  // namespace madi
 
@@ -480,6 +525,10 @@ int madi_classify(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d
         set_error("madi_classify: invalid arguments");
         return MADI_ERR_ARG;
     }
+    if (!kd_ready(g->dev)) {
+        set_error("madi_classify: geometry has no KD tree");
+        return MADI_ERR_ARG;
+    }
     return launch_classify_f64(g->dev, d_pts, n, d_cell, d_inside, d_ok, nullptr, (hipStream_t)stream);
 }
 
@@ -527,6 +576,10 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
         return MADI_ERR_ARG;
     }
     W.use_golden = golden;
+    if (!kd_ready(g->dev)) {
+        set_error("madi_walk_f64: geometry has no KD tree");
+        return MADI_ERR_ARG;
+    }
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
     return launch_walk_f64(g->dev, W, jp, d_gold_init, d_gold_incr, d_gold_u, d_Y, d_inside_trace, d_escaped, d_err,
@@ -585,6 +638,10 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         set_error("madi_walk_fused: the FP32 walk needs the geometry's Delaunay graph");
         return MADI_ERR_ARG;
     }
+    if (precision == 64 && !kd_ready(g->dev)) {
+        set_error("madi_walk_fused: the FP64 walk needs the geometry's KD tree");
+        return MADI_ERR_ARG;
+    }
     int64_t need = 0;
     madi_walk_fused_workspace(W.n_walkers, plan, &need);
     if (workspace_bytes < need) {
@@ -611,7 +668,8 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     fo.occupancy = (long long*)d_occupancy;
     fo.flags = (long long*)d_flags;
     if (precision == 32)
-        rc = launch_walk_f32_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
+        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), chunk_of(p->scan_chunk),
+                                   (unsigned long long*)d_counters, s);
     else
         rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
     if (rc) return rc;
@@ -624,6 +682,132 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         return MADI_OK;
     }
     return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, s);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- batching
+
+namespace {
+
+constexpr int64_t kAlign = 256;
+
+int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+int64_t batch_bytes(int32_t n_work)
+{
+    return align_up((int64_t)sizeof(PlanDev)) + kAlign + align_up((int64_t)n_work * (int64_t)sizeof(WorkDev));
+}
+
+}  // namespace
+
+extern "C" {
+
+int madi_batch_workspace(int32_t n_work, int64_t* bytes)
+{
+    if (!bytes || n_work < 0) {
+        set_error("madi_batch_workspace: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    *bytes = batch_bytes(n_work);
+    return MADI_OK;
+}
+
+int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, const madi_tune* tune,
+                      void* d_workspace, int64_t workspace_bytes, int64_t* d_counters, void* stream)
+{
+    if (n_work < 0 || (n_work > 0 && !work) || !d_workspace) {
+        set_error("madi_batch_launch: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (workspace_bytes < batch_bytes(n_work)) {
+        set_error("madi_batch_launch: workspace of %lld bytes is smaller than the %lld required",
+                  (long long)workspace_bytes, (long long)batch_bytes(n_work));
+        return MADI_ERR_ARG;
+    }
+    if (n_work == 0) return MADI_OK;
+    int max_grid = 1;
+    for (int i = 0; i < n_work; ++i) max_grid = std::max(max_grid, work[i].walk.n_grid);
+    PlanDev P;
+    Events ev;
+    int rc = make_plan(plan, max_grid, P, ev);
+    if (rc) return rc;
+    std::vector<WorkDev> table((size_t)n_work);
+    int64_t total = 0;
+    for (int i = 0; i < n_work; ++i) {
+        const madi_work& w = work[i];
+        WorkDev& D = table[(size_t)i];
+        if (!w.geom || !w.d_partial || !w.d_occupancy || !w.d_flags) {
+            set_error("madi_batch_launch: work item %d has a NULL geometry or output", i);
+            return MADI_ERR_ARG;
+        }
+        if ((rc = make_walk(&w.walk, D.w))) return rc;
+        for (int p = 0; p < plan->n_pairs; ++p)
+            if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
+                plan->j_delta[p] >= w.walk.n_grid) {
+                set_error("madi_batch_launch: work item %d has fewer Y-grid times than the plan needs", i);
+                return MADI_ERR_ARG;
+            }
+        if (!graph_ready(w.geom->dev)) {
+            set_error("madi_batch_launch: work item %d geometry has no Delaunay graph", i);
+            return MADI_ERR_ARG;
+        }
+        const int64_t tiles = (w.walk.n_walkers + kBlock - 1) / kBlock;
+        if (w.tile_lo < 0 || w.n_tiles < 0 || w.tile_lo + w.n_tiles > tiles) {
+            set_error("madi_batch_launch: work item %d tiles [%lld, %lld) outside the entry's %lld tiles", i,
+                      (long long)w.tile_lo, (long long)(w.tile_lo + w.n_tiles), (long long)tiles);
+            return MADI_ERR_ARG;
+        }
+        D.g = w.geom->dev;
+        D.tile_lo = w.tile_lo;
+        D.n_tiles = w.n_tiles;
+        D.work_begin = total;
+        D.partial = w.d_partial;
+        D.occupancy = (long long*)w.d_occupancy;
+        D.flags = (long long*)w.d_flags;
+        total += w.n_tiles;
+    }
+    const float cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : work[0].walk.tier_cap_um);
+    const int chunk = chunk_of(tune && tune->scan_chunk > 0 ? tune->scan_chunk : work[0].walk.scan_chunk);
+    hipStream_t s = (hipStream_t)stream;
+    char* base = (char*)d_workspace;
+    PlanDev* dplan = (PlanDev*)base;
+    unsigned long long* queue = (unsigned long long*)(base + align_up((int64_t)sizeof(PlanDev)));
+    WorkDev* dwork = (WorkDev*)(base + align_up((int64_t)sizeof(PlanDev)) + kAlign);
+    HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dwork, table.data(), table.size() * sizeof(WorkDev), hipMemcpyHostToDevice, s));
+    if ((rc = zero_i64((long long*)queue, 1, s))) return rc;
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) return rc;
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
+    int64_t grid = (int64_t)cus * per_cu;
+    if (grid > total) grid = total;
+    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, chunk, (int32_t)grid,
+                            (unsigned long long*)d_counters, s);
+}
+
+int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
+                        double* d_sin, void* stream)
+{
+    if (!plan || !d_partial || !d_cos || !d_sin || n_tiles < 0) {
+        set_error("madi_tiles_finalize: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const int32_t n_cols = plan->n_pairs * plan->n_b * plan->n_dirs;
+    return launch_finalize_tiles(d_partial, 0, n_tiles, n_cols, d_cos, d_sin, (hipStream_t)stream);
+}
+
+int madi_fill_i64(int64_t* d, int64_t n, int64_t value, void* stream)
+{
+    if (n < 0 || (n > 0 && !d)) {
+        set_error("madi_fill_i64: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (n == 0) return MADI_OK;
+    return fill_i64((long long*)d, n, (long long)value, (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -76,7 +76,7 @@ struct GeomDev {
     int32_t hint_n;
     double hint_h;
     float alpha_min;
-    float pad0;
+    float alpha_max;
     // domain
     double L;
     double source_lo, source_hi;

@@ -84,13 +84,12 @@ __device__ __forceinline__ void column_put(double cs, double sn, int col, int n_
 // Emit every column of the plan for this thread's walker.  dM is the
 // walker's moment per pair (unused pairs ignored); active=false contributes
 // zeros (threads past the end of the walker range, escaped walkers).
-template <typename T>
-__device__ void emit_columns(const PlanDev& plan, const T (&dM)[MADI_MAX_PAIRS][3], bool active, double* lds,
-                             double* tile_slot)
+template <typename T, int NP = MADI_MAX_PAIRS>
+__device__ void emit_columns(const PlanDev& plan, const T (&dM)[NP][3], bool active, double* lds, double* tile_slot)
 {
     const int n_b = plan.n_b, n_dirs = plan.n_dirs, n_cols = plan.n_cols;
 #pragma unroll
-    for (int p = 0; p < MADI_MAX_PAIRS; ++p) {
+    for (int p = 0; p < NP; ++p) {
         if (p < plan.n_pairs) {
             const T mx = dM[p][0], my = dM[p][1], mz = dM[p][2];
             for (int b = 0; b < n_b; ++b) {
@@ -125,6 +124,7 @@ __device__ void emit_columns(const PlanDev& plan, const T (&dM)[MADI_MAX_PAIRS][
 // -Y (Delta + delta).
 struct Events {
     int32_t n;
+    int32_t n_pairs;
     int32_t j[3 * MADI_MAX_PAIRS];
     int32_t pair[3 * MADI_MAX_PAIRS];
     int32_t sign[3 * MADI_MAX_PAIRS];

@@ -104,8 +104,11 @@ def spec_from_columns(j_delta, j_Delta, j_sum, pcs, dirs=AXIS_DIRECTIONS):
     return spec, col_pair, col_b
 
 
-def walk_params(n_walkers: int, cfg: SimConfig, pp: float, seed: int, first_walker: int = 0) -> _hip.WalkParams:
+def walk_params(n_walkers: int, cfg: SimConfig, pp: float, seed: int, first_walker: int = 0,
+                tier_cap_um: float = 0.0, scan_chunk: int = 0) -> _hip.WalkParams:
     p = _hip.WalkParams()
+    p.tier_cap_um = float(tier_cap_um)
+    p.scan_chunk = int(scan_chunk)
     p.n_walkers = int(n_walkers)
     p.n_steps = int(cfg.n_steps)
     p.steps_per_h = int(cfg.steps_per_h)

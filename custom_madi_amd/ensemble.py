@@ -506,7 +506,8 @@ def periodic_nearest_distances(seeds: np.ndarray, L: float) -> np.ndarray:
 
 
 def create_periodic_ensemble(n_cells: int, rho: float, vi: float, cfg: SimConfig | None = None,
-                             seed: int | None = 0, verify_vi: bool = False, n_vi_points: int = 200_000) -> Ensemble:
+                             seed: int | None = 0, verify_vi: bool = False, n_vi_points: int = 200_000,
+                             with_kd: bool = True) -> Ensemble:
     """Periodic cube of n_cells uniform seeds at density rho [cells/µL].
 
     Side L = (n_cells / rho)^(1/3); alpha* from Eq. S7a at the target v_i;
@@ -526,7 +527,8 @@ def create_periodic_ensemble(n_cells: int, rho: float, vi: float, cfg: SimConfig
     if np.any(nn <= 0.0):
         raise RuntimeError("Poisson seed realization contains coincident seeds")
     annulus = np.minimum(alpha_star, cfg.kappa * nn / 2.0).astype(np.float64)
-    nodes = build_kdtree(seeds)
+    empty = np.zeros(0, dtype=np.int32)
+    nodes = build_kdtree(seeds) if with_kd else (empty, empty.astype(np.int8), empty, empty, empty)
     cert = PopulationCertificate(0.0, 0.0, True, int(n_cells), float("inf"), True, 0.0, 0)
     stats = _geometry_stats(vi, float("nan"), float("nan"), rho, float("nan"), alpha_star, mean_av, annulus,
                             n_cells, n_cells, n_vi_points, L, L, cert, reference)
@@ -616,8 +618,13 @@ class GeometryArrays:
     pad_margin: float = 0.0
 
 
-def geometry_arrays(ens: Ensemble) -> GeometryArrays:
-    """Everything the kernels need, derived from an ensemble."""
+def geometry_arrays(ens: Ensemble, with_kd: bool = True) -> GeometryArrays:
+    """Everything the kernels need, derived from an ensemble.
+
+    with_kd=False skips the exact classifier's KD tree (and, for periodic
+    boxes, the KD padding margin): the FP32 production walk needs only the
+    Delaunay graph and the hint grid.
+    """
     L = float(ens.L)
     if ens.is_free_water:
         z = np.zeros(0)
@@ -633,9 +640,10 @@ def geometry_arrays(ens: Ensemble) -> GeometryArrays:
         nb_vec = seeds[indices] - np.repeat(seeds, np.diff(indptr), axis=0)
         tree = ens._tree()
         n_hint = int(min(64, max(4, round(1.5 * L / spacing))))
+        nodes = (ens.kd_node_seed, ens.kd_node_axis, ens.kd_node_left, ens.kd_node_right, ens.kd_node_parent)
         return GeometryArrays(
-            kd_xyz=seeds, kd_annulus=ann, kd_label=np.arange(n, dtype=np.int32),
-            nodes=(ens.kd_node_seed, ens.kd_node_axis, ens.kd_node_left, ens.kd_node_right, ens.kd_node_parent),
+            kd_xyz=seeds if with_kd else seeds[:0], kd_annulus=ann, kd_label=np.arange(n, dtype=np.int32),
+            nodes=nodes if with_kd else (),
             cell_xyz=seeds, cell_annulus=ann, nb_offsets=indptr, nb_ids=indices.astype(np.int32),
             nb_vec=np.ascontiguousarray(nb_vec), hint_n=n_hint, hint=_hint_grid(tree, L, n_hint), L=L,
             source_lo=float(ens.source_lo), source_hi=float(ens.source_hi), periodic=False, free_water=False,
@@ -644,9 +652,12 @@ def geometry_arrays(ens: Ensemble) -> GeometryArrays:
     # classifier's ball d1 + 2 alpha1 around any point of the box and (b)
     # the circumspheres of every Delaunay simplex that touches a cell.
     ptree = cKDTree(seeds, boxsize=L)
-    d1_upper, _ = _nearest_distance_upper_bound_periodic(ptree, L, min(spacing / 4.0, 2.0))
-    margin = min(L, d1_upper + 2.0 * float(np.max(ann)) + 1e-6)
-    margin = max(margin, 2.0 * spacing)
+    if with_kd:
+        d1_upper, _ = _nearest_distance_upper_bound_periodic(ptree, L, min(spacing / 4.0, 2.0))
+        margin = min(L, d1_upper + 2.0 * float(np.max(ann)) + 1e-6)
+        margin = max(margin, 2.0 * spacing)
+    else:
+        margin = min(L, 3.0 * spacing)
     for _ in range(8):
         pts, label, shift = periodic_images(seeds, L, margin)
         indptr, indices, tri = delaunay_graph(pts, n)
@@ -656,18 +667,18 @@ def geometry_arrays(ens: Ensemble) -> GeometryArrays:
         hi_ok = np.all(centre + radius[:, None] <= L + margin)
         if lo_ok and hi_ok:
             break
-        margin *= 1.5
+        margin *= 1.25
     else:
         raise RuntimeError("periodic Delaunay padding could not be certified")
     offs = indptr[: n + 1]
     ids = indices[: offs[-1]]
     owner = np.repeat(np.arange(n), np.diff(offs))
     nb_vec = pts[ids] - pts[owner]
-    nodes = build_kdtree(pts)
+    nodes = build_kdtree(pts) if with_kd else ()
     n_hint = int(min(64, max(4, round(1.5 * L / spacing))))
     return GeometryArrays(
-        kd_xyz=np.ascontiguousarray(pts), kd_annulus=np.ascontiguousarray(ann[label]), kd_label=label,
-        nodes=nodes, cell_xyz=seeds, cell_annulus=ann, nb_offsets=np.ascontiguousarray(offs),
+        kd_xyz=np.ascontiguousarray(pts) if with_kd else pts[:0], kd_annulus=np.ascontiguousarray(ann[label]),
+        kd_label=label, nodes=nodes, cell_xyz=seeds, cell_annulus=ann, nb_offsets=np.ascontiguousarray(offs),
         nb_ids=np.ascontiguousarray(label[ids]).astype(np.int32), nb_vec=np.ascontiguousarray(nb_vec),
         hint_n=n_hint, hint=_hint_grid(ptree, L, n_hint), L=L, source_lo=float(ens.source_lo),
         source_hi=float(ens.source_hi), periodic=True, free_water=False, pad_margin=float(margin),
@@ -720,13 +731,14 @@ class DeviceGeometry:
         d.periodic = int(arr.periodic)
         d.free_water = int(arr.free_water)
         if not arr.free_water:
-            seed, axis, left, right, parent = arr.nodes
-            d.n_kd = len(arr.kd_xyz)
-            d.kd_xyz = f64(arr.kd_xyz)
-            d.kd_annulus = f64(arr.kd_annulus)
-            d.kd_label = i32(arr.kd_label)
-            d.node_seed, d.node_axis = i32(seed), i8(axis)
-            d.node_left, d.node_right, d.node_parent = i32(left), i32(right), i32(parent)
+            if len(arr.kd_xyz):
+                seed, axis, left, right, parent = arr.nodes
+                d.n_kd = len(arr.kd_xyz)
+                d.kd_xyz = f64(arr.kd_xyz)
+                d.kd_annulus = f64(arr.kd_annulus)
+                d.kd_label = i32(arr.kd_label)
+                d.node_seed, d.node_axis = i32(seed), i8(axis)
+                d.node_left, d.node_right, d.node_parent = i32(left), i32(right), i32(parent)
             d.n_cells = len(arr.cell_xyz)
             d.cell_xyz = f64(arr.cell_xyz)
             d.cell_annulus = f64(arr.cell_annulus)

@@ -1,0 +1,164 @@
+"""Continuous-batching server for library-entry simulation requests.
+
+Requests (k_io, rho, V with their own cell and walker counts) arrive at any
+time.  Host geometry is prepared in worker processes; a prepared request is
+admitted to the GPU when free HBM allows (``DeviceScheduler.admit``), and its
+tiles join the next generation of the persistent kernel alongside whatever
+else is running.  Finished entries are appended to a library stream file
+(one JSON line per entry) and can be written as a v5 .npz library.
+Latency of an entry = finish time - submit time.
+"""
+
+from __future__ import annotations
+
+import json
+import multiprocessing as mp
+import time
+from collections import deque
+from pathlib import Path
+from typing import List, Optional
+
+import numpy as np
+
+from .batch import DeviceScheduler, EntryRequest, EntryResult, PreparedEntry, prepare_entry
+
+
+def make_pool(workers: int):
+    """Spawned worker pool for host geometry; create it before the GPU is touched."""
+    ctx = mp.get_context("spawn")
+    return ctx.Pool(processes=max(1, int(workers)))
+
+
+class LibraryStream:
+    """Append-only record of finished entries (one JSON object per line)."""
+
+    def __init__(self, path: Optional[str]):
+        self.path = Path(path) if path else None
+        self.fh = open(self.path, "a") if self.path else None
+        self.count = 0
+
+    def append(self, res: EntryResult, spec_shape) -> None:
+        self.count += 1
+        if self.fh is None:
+            return
+        rec = {
+            "key": list(res.key), "n_walkers": res.n_walkers, "pp": res.pp,
+            "signal": (res.cos / res.n_walkers).ravel().tolist(),
+            "signal_imag": (res.sin / res.n_walkers).ravel().tolist(), "shape": list(spec_shape),
+            "occupancy_fraction": (res.occupancy / res.n_walkers).tolist(),
+            "n_escaped": res.n_escaped, "latency_s": res.latency_s,
+        }
+        self.fh.write(json.dumps(rec) + "\n")
+        self.fh.flush()
+
+    def close(self):
+        if self.fh:
+            self.fh.close()
+
+
+class Server:
+    """Submit requests; ``step()`` advances preparation, admission, launches and finalisation."""
+
+    def __init__(self, scheduler: DeviceScheduler, pool=None, stream: Optional[LibraryStream] = None,
+                 max_prep_inflight: int = 64):
+        self.sch = scheduler
+        self.pool = pool
+        self.stream = stream or LibraryStream(None)
+        self.preparing: deque = deque()   # AsyncResult or PreparedEntry
+        self.ready: deque = deque()       # prepared, waiting for HBM
+        self.results: List[EntryResult] = []
+        self.max_prep = max_prep_inflight
+        self.rejected = 0
+
+    def submit(self, req: EntryRequest) -> None:
+        req.submitted = time.perf_counter()
+        if self.pool is None:
+            self.preparing.append(prepare_entry(req))
+        else:
+            self.preparing.append(self.pool.apply_async(prepare_entry, (req,)))
+
+    def _collect_prepared(self) -> None:
+        while self.preparing:
+            head = self.preparing[0]
+            if isinstance(head, PreparedEntry):
+                self.ready.append(self.preparing.popleft())
+                continue
+            if not head.ready():
+                break
+            prep = self.preparing.popleft().get()
+            # keep the submit time of the request object the caller holds
+            self.ready.append(prep)
+
+    def step(self, block: bool = False) -> List[EntryResult]:
+        self._collect_prepared()
+        while self.ready and self.sch.admit(self.ready[0]):
+            self.ready.popleft()
+        self.sch.launch_generation()
+        done = self.sch.poll(block=block)
+        for r in done:
+            self.stream.append(r, self.sch.spec.shape)
+        self.results += done
+        return done
+
+    def idle(self) -> bool:
+        return not (self.preparing or self.ready or self.sch.active or self.sch.inflight)
+
+    def drain(self) -> None:
+        while not self.idle():
+            self._collect_prepared()
+            if not self.preparing and self.ready and not self.sch.active and not self.sch.inflight:
+                if not self.sch.admit(self.ready[0]):
+                    raise MemoryError("a request does not fit in free HBM even with the device idle")
+                self.ready.popleft()
+            self.step(block=bool(self.sch.inflight))
+            if self.preparing and not self.sch.inflight:
+                time.sleep(0.002)
+
+
+def poisson_arrivals(rate_hz: float, duration_s: float, seed: int = 3) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    t, out = 0.0, []
+    while True:
+        t += rng.exponential(1.0 / rate_hz)
+        if t >= duration_s:
+            return np.asarray(out)
+        out.append(t)
+
+
+def serve_poisson(server: Server, requests: List[EntryRequest], arrivals: np.ndarray,
+                  progress=None) -> dict:
+    """Feed requests at the given arrival offsets (seconds) and run until all finish."""
+    t0 = time.perf_counter()
+    k = 0
+    hbm = []
+    last_report = t0
+    while k < len(requests) or not server.idle():
+        now = time.perf_counter() - t0
+        while k < len(requests) and arrivals[k] <= now:
+            server.submit(requests[k])
+            k += 1
+        server.step(block=False)
+        if server.sch.inflight and k < len(requests):
+            time.sleep(0.001)
+        elif not server.sch.inflight and k < len(requests):
+            time.sleep(min(0.005, max(0.0, arrivals[k] - (time.perf_counter() - t0))))
+        elif server.sch.inflight:
+            server.step(block=True)
+        else:
+            time.sleep(0.002)
+        if progress and time.perf_counter() - last_report > 10.0:
+            last_report = time.perf_counter()
+            progress(f"t={now:.0f}s submitted={k} done={len(server.results)} active={len(server.sch.active)} "
+                     f"ready={len(server.ready)} preparing={len(server.preparing)}")
+            free, total = server.sch.torch.cuda.mem_get_info()
+            hbm.append((now, total - free))
+    wall = time.perf_counter() - t0
+    lat = np.array([r.latency_s for r in server.results])
+    return {
+        "requests": len(requests), "finished": len(server.results), "wall_s": wall,
+        "entries_per_hour": len(server.results) / wall * 3600.0,
+        "latency_s_p50": float(np.percentile(lat, 50)) if len(lat) else None,
+        "latency_s_p99": float(np.percentile(lat, 99)) if len(lat) else None,
+        "latency_s_max": float(lat.max()) if len(lat) else None,
+        "hbm_used_bytes_samples": hbm,
+    }

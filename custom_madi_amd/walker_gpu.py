@@ -43,9 +43,16 @@ def _gpu_available() -> bool:
         return False
 
 
-# The reference's name for "the GPU path is available" (madi/walker_gpu.py:39-41).
-HAS_CUDA = _gpu_available()
-HAS_GPU = HAS_CUDA
+def __getattr__(name):
+    # HAS_CUDA / HAS_GPU: the reference's "GPU path is available" flag
+    # (madi/walker_gpu.py:39-41), evaluated on first use so that importing
+    # this module (for example in host-only worker processes) does not
+    # initialise the GPU runtime.
+    if name in ("HAS_CUDA", "HAS_GPU"):
+        value = _gpu_available()
+        globals()["HAS_CUDA"] = globals()["HAS_GPU"] = value
+        return value
+    raise AttributeError(name)
 
 
 def pp_to_kio_eq5(pp: float, mean_AV: float, cfg: SimConfig) -> float:

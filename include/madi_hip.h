@@ -40,7 +40,7 @@ extern "C" {
 
 /* Counters returned by the walk kernels (int64 each). */
 #define MADI_CNT_CLASSIFY 0      /* endpoint classifications run (graph or KD) */
-#define MADI_CNT_SAFE_HIT 1      /* endpoints answered by the certified safe ball */
+#define MADI_CNT_SAFE_HIT 1      /* endpoints answered by the certified plane tier */
 #define MADI_CNT_GRAPH_ITERS 2   /* Delaunay-graph anchor moves */
 #define MADI_CNT_FP64_RECHECK 3  /* FP32 decisions re-decided in FP64 */
 #define MADI_CNT_FACETS 4        /* facet (edge) evaluations */
@@ -81,7 +81,7 @@ int madi_get_device_info(madi_device_info* out);
  *
  * kd_*: the exact classifier's point set and its balanced KD tree in
  *   pre-order (node_seed/axis/left/right/parent as madi/ensemble.py builds
- *   them).  kd_label maps each KD point to its cell (identity for the
+ *   them); n_kd = 0 gives a graph-only geometry for the FP32 walk.  kd_label maps each KD point to its cell (identity for the
  *   reference's non-periodic sets; periodic images map to their cell).
  * cell_*, nb_*: the Delaunay neighbour graph of the cells (CSR over cells,
  *   nb_vec = c_k + shift - c_a in FP64), used by the FP32 production walk.
@@ -152,12 +152,14 @@ typedef struct {
     int32_t n_steps;
     int32_t steps_per_h;
     int32_t n_grid;
-    int32_t reserved;
+    float tier_cap_um;      /* FP32 walk: plane-tier cap beyond max alpha (0 = default 3 um) */
     double sigma;
     double ts;
     double pp;
     uint64_t seed;          /* walk seed (device RNG) */
     uint64_t first_walker;  /* RNG subsequence index of walker 0 */
+    int32_t scan_chunk;     /* FP32 walk: facet evaluations per lane per iteration (4, 8 or 16; 0 = 8) */
+    int32_t reserved;
 } madi_walk_params;
 
 /* Reference-semantics FP64 walk with the exact classifier at every endpoint.
@@ -211,6 +213,53 @@ int madi_walk_fused_workspace(int64_t n_walkers, const madi_phase_plan* plan, in
 int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
                     void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
                     int64_t* d_flags, int64_t* d_counters, void* stream);
+
+/* ------------------------------------------------------------- batching */
+
+#define MADI_TILE_WALKERS 256
+
+/* One work item of a batched launch: tiles [tile_lo, tile_lo + n_tiles) of
+ * one library entry (tile t = walkers [256 t, 256 t + 256)).  d_partial
+ * holds the entry's per-tile slots (ceil(n_walkers / 256) x n_cols x 2 FP64)
+ * and d_occupancy (n_grid) / d_flags (MADI_N_FLAGS) accumulate over the
+ * entry's launches; the caller zeroes them when it admits the entry. */
+typedef struct {
+    const madi_geom* geom;
+    madi_walk_params walk;
+    int64_t tile_lo;
+    int64_t n_tiles;
+    double* d_partial;
+    int64_t* d_occupancy;
+    int64_t* d_flags;
+} madi_work;
+
+typedef struct {
+    int32_t grid_per_cu;  /* persistent workgroups per CU (0 = 4) */
+    float tier_cap_um;    /* plane-tier cap (0 = first item's walk.tier_cap_um) */
+    int32_t scan_chunk;   /* facet evaluations per lane per iteration (0 = first item's walk.scan_chunk) */
+    int32_t reserved;
+} madi_tune;
+
+/* Device workspace bytes of a batched launch of n_work items. */
+int madi_batch_workspace(int32_t n_work, int64_t* bytes);
+
+/* One persistent, work-stealing launch of the FP32 production walk over many
+ * entries of uneven size: workgroups take tiles from an atomic counter until
+ * every tile of every item is done.  Each tile writes its own slot, so an
+ * entry's result is bitwise identical alone or in any mix.  Replaces the
+ * one-entry-at-a-time loop of build_library_from_triplets
+ * (madi/library.py:546-629) with one launch per batch or per serving
+ * generation.  Asynchronous on stream. */
+int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, const madi_tune* tune,
+                      void* d_workspace, int64_t workspace_bytes, int64_t* d_counters, void* stream);
+
+/* Sum an entry's tile slots [0, n_tiles) in tile order into d_cos / d_sin
+ * (n_cols FP64 each). */
+int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
+                        double* d_sin, void* stream);
+
+/* d[i] = value for i < n, by a fill kernel (usable where memset nodes are not). */
+int madi_fill_i64(int64_t* d, int64_t n, int64_t value, void* stream);
 
 #ifdef __cplusplus
 }

@@ -27,9 +27,13 @@
  * reference CPU walk on injected streams.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
+
+/* Thread count of the OpenMP loops below (benchmark control). */
+void og_set_threads(int n) { if (n > 0) omp_set_num_threads(n); }
 
 /* ---------------------------------------------------------------- RNG */
 

@@ -67,6 +67,8 @@ def lib() -> C.CDLL:
         if not LIB.is_file():
             build()
         L = C.CDLL(str(LIB))
+        L.og_set_threads.restype = None
+        L.og_set_threads.argtypes = [C.c_int]
         L.og_seed_state.restype = OState
         L.og_seed_state.argtypes = [C.c_uint64]
         L.og_next.restype = C.c_uint64
@@ -174,6 +176,10 @@ def periodic_brute(seeds, annulus, L, points):
     wrapped = np.mod(points, L)
     ids, inside = classify_brute(tiled, ann, wrapped)
     return (ids % len(seeds)).astype(np.int32), inside
+
+
+def set_threads(n):
+    lib().og_set_threads(int(n))
 
 
 def rng_states(seed, first, n):

@@ -1,0 +1,157 @@
+"""Batched / served walks and the drop-in API on an MI355X.
+
+An entry's result must be bitwise identical whether it runs alone (one
+launch), in a batch, or in a different mix of entries; the batched FP32
+walk must agree with the FP64 exact-classifier walk within |dS| <= 2e-3 on
+the same seeds; the drop-in library API must build, save and load.
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from custom_madi_amd import batch, engine
+from custom_madi_amd.config import SimConfig
+from custom_madi_amd.ensemble import DeviceGeometry, geometry_arrays
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def small_requests():
+    return batch.library_requests(5, seed=11, walkers=(3000, 9000), cells=(60, 400), T_max_ms=12.0)
+
+
+@pytest.fixture(scope="module")
+def spec():
+    return batch.acquisition_spec(n_dirs=6, delta=2.0, Deltas=(5.0, 8.0, 10.0), b_values=(0.0, 1000.0, 4000.0))
+
+
+def _same(a, b):
+    return (np.array_equal(a.cos, b.cos) and np.array_equal(a.sin, b.sin)
+            and np.array_equal(a.occupancy, b.occupancy))
+
+
+def test_batched_equals_plain_bitwise(small_requests, spec):
+    preps = [batch.prepare_entry(r) for r in small_requests]
+    plain = batch.run_plain(small_requests, spec)
+    mixed = batch.run_batched(preps, spec, max_tiles_per_generation=7)
+    by_key = {r.key: r for r in mixed}
+    for r in plain:
+        assert r.n_escaped == 0 and r.n_error == 0
+        assert _same(r, by_key[r.key]), r.key
+
+
+def test_entry_alone_equals_entry_in_mixes(small_requests, spec):
+    preps = [batch.prepare_entry(r) for r in small_requests]
+    alone = batch.run_batched(preps[2:3], spec)[0]
+    mix1 = {r.key: r for r in batch.run_batched(preps, spec)}
+    mix2 = {r.key: r for r in batch.run_batched(preps[::-1], spec, max_tiles_per_generation=3, grid_per_cu=1)}
+    assert _same(alone, mix1[alone.key])
+    assert _same(alone, mix2[alone.key])
+
+
+def test_batched_fp32_matches_fp64_exact_walk(spec):
+    req = batch.library_requests(1, seed=5, walkers=(65_536, 65_536), cells=(300, 300), T_max_ms=12.0)[0]
+    prep = batch.prepare_entry(req)
+    res32 = batch.run_batched([prep], spec)[0]
+    # the same geometry with the exact classifier's KD tree for the FP64 walk
+    from custom_madi_amd.ensemble import create_periodic_ensemble
+    ens = create_periodic_ensemble(req.n_cells, req.rho, req.rho * req.V * 1e-6, req.config(), seed=req.seed)
+    dg = DeviceGeometry(geometry_arrays(ens))
+    p = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed)
+    res64 = engine.walk_fused(dg, p, spec, precision=64)
+    dS = np.abs(res32.cos - res64.cos) / req.n_walkers
+    say(f"batched fp32 vs fp64: max |dS| = {dS.max():.3e}")
+    assert dS.max() <= 2e-3
+    assert res64.n_escaped == 0
+
+
+def test_scheduler_respects_hbm_reserve(small_requests, spec):
+    import torch
+    prep = batch.prepare_entry(small_requests[0])
+    free, _ = torch.cuda.mem_get_info()
+    sch = batch.DeviceScheduler(spec, hbm_reserve_bytes=free + (1 << 30))
+    assert not sch.admit(prep)
+    sch = batch.DeviceScheduler(spec)
+    assert sch.admit(prep)
+    out = sch.drain()
+    assert len(out) == 1 and out[0].n_escaped == 0
+
+
+def test_serving_finishes_every_request(spec):
+    from custom_madi_amd import serve
+    reqs = batch.library_requests(6, seed=21, walkers=(2000, 6000), cells=(60, 200), T_max_ms=12.0)
+    sch = batch.DeviceScheduler(spec, max_tiles_per_generation=8)
+    srv = serve.Server(sch)
+    rep = serve.serve_poisson(srv, reqs, np.linspace(0.0, 0.3, len(reqs)))
+    assert rep["finished"] == len(reqs)
+    assert rep["latency_s_p99"] >= rep["latency_s_p50"] > 0
+    ref = {r.key: r for r in batch.run_batched([batch.prepare_entry(r) for r in reqs], spec)}
+    for r in srv.results:
+        assert _same(r, ref[r.key])
+
+
+def test_dropin_library_build_v5_round_trip(tmp_path):
+    """build_library_from_triplets on reference-semantics geometry (S8 box, Omega_pop,
+    exact FP64 walk, v5 diagnostics on the declared subset grid), saved and loaded."""
+    from custom_madi_amd import library, signal
+    from custom_madi_amd.config import ENSEMBLE_MEAN_SUBSET_B_VALUES_S_MM2
+    cfg = SimConfig(n_walkers=256, n_ensembles=2, T_max_ms=110.0, D0=2.0, ts=0.05,
+                    small_deltas=[5.0, 7.0, 10.0, 12.0, 15.0, 20.0, 25.0, 30.0],
+                    big_deltas=[15.0, 25.0, 30.0, 36.0, 40.0, 50.0, 60.0, 80.0],
+                    b_values=list(ENSEMBLE_MEAN_SUBSET_B_VALUES_S_MM2), geometry_validation_points=20_000,
+                    geometry_vi_tolerance=0.03)
+    cols = signal.build_columns(cfg)
+    out = tmp_path / "lib.npz"
+    lib = library.build_library_from_triplets([(20.0, 1.0e5, 6.0), (0.0, 0.0, 0.0)], cfg=cfg, seed=7,
+                                              save_path=str(out), verbose=False)
+    assert len(lib) == 2
+    back = library.load_library(str(out))
+    meta = library.load_library_meta(str(out))
+    assert meta["library_schema"] == "madi-library-v5" and meta["has_v5_diagnostics"]
+    assert meta["ensemble_means_subset_shape"] == (2, 2, library.ENSEMBLE_MEAN_SUBSET_N_COLUMNS)
+    cell = [e for e in back if not e.is_free_water][0]
+    free = [e for e in back if e.is_free_water][0]
+    assert cell.vector.shape == (cols.n_pairs * cols.n_b,)
+    assert np.all(cell.vector[::cols.n_b] == 1.0)      # b = 0 columns
+    assert 0.55 < cell.vi < 0.65
+    assert cell.metadata["boundary"]["n_escaped"] == 0
+    # free water: each column is a mean of cos(phase) over 3 axes x walkers x
+    # ensembles, with phase ~ N(0, 2 b D0); its standard error follows from
+    # E[cos^2] = (1 + S^4) / 2 with S = exp(-b D0)
+    b = np.tile(np.asarray(cfg.b_values), cols.n_pairs)
+    expect = np.exp(-b * cfg.D0 * 1e-3)
+    n_eff = 3 * cfg.n_walkers * cfg.n_ensembles
+    se = np.sqrt(np.maximum((1.0 + expect ** 4) / 2.0 - expect ** 2, 0.0) / n_eff)
+    assert np.all(np.abs(free.vector - expect) <= 5.0 * se + 0.012)
+
+
+def test_dropin_fp64_fused_equals_y_reduce_path():
+    from custom_madi_amd import walker_gpu
+    from tests import _geom
+    ens = _geom.cellular(L=150.0, seed=9)
+    cfg = SimConfig(L=150.0, n_walkers=3000, T_max_ms=6.0, D0=2.0, ts=0.01, small_deltas=[1.0],
+                    big_deltas=[3.0, 4.0], b_values=[0.0, 1000.0, 4000.0])
+    from custom_madi_amd import signal
+    cols = signal.build_columns(cfg)
+    fused = walker_gpu._walk_and_reduce_one_ensemble(ens, 0.05, cfg, cols.j_delta, cols.j_Delta, cols.j_sum,
+                                                     cols.phase_coef, 1234)
+    # a column set too large to fuse takes the Y + reduce path; compare the shared columns
+    many = np.concatenate([cols.phase_coef, np.linspace(0.001, 0.02, 40)])
+    jd = np.concatenate([cols.j_delta, np.full(40, 1)])
+    jD = np.concatenate([cols.j_Delta, np.full(40, 3)])
+    js = np.concatenate([cols.j_sum, np.full(40, 4)])
+    assert engine.spec_from_columns(jd, jD, js, many) is None
+    unfused = walker_gpu._walk_and_reduce_one_ensemble(ens, 0.05, cfg, jd, jD, js, many, 1234)
+    n = len(cols.phase_coef)
+    assert np.allclose(fused.cos_sum, unfused.cos_sum[:n], rtol=1e-10, atol=1e-8)
+    assert np.array_equal(fused.occupancy_counts, unfused.occupancy_counts)
