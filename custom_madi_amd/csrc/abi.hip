@@ -259,6 +259,11 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
             return MADI_ERR_ARG;
         }
         std::vector<int32_t> off(nc + 1);
+        for (int c = 0; c < nc; ++c)
+            if (d->nb_offsets[c + 1] < d->nb_offsets[c]) {
+                set_error("geometry: nb_offsets must be non-decreasing");
+                return MADI_ERR_ARG;
+            }
         for (int c = 0; c <= nc; ++c) off[c] = (int32_t)d->nb_offsets[c];
         // each cell's edges sorted by half length (the FP32 walk stops its edge
         // loop once h_j - |p| passes the bound it needs)
@@ -336,7 +341,8 @@ static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != n
 static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
 
 static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
-static int chunk_of(int v) { return v > 0 ? v : 8; }
+// FP32 walk classification rounds per iteration (1-3); 0 picks the default.
+static int rounds_of(int r) { return r <= 0 ? 2 : (r > 3 ? 3 : r); }
 
 static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Events& ev)
 {
@@ -668,7 +674,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     fo.occupancy = (long long*)d_occupancy;
     fo.flags = (long long*)d_flags;
     if (precision == 32)
-        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), chunk_of(p->scan_chunk),
+        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds),
                                    (unsigned long long*)d_counters, s);
     else
         rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
@@ -768,7 +774,6 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
         total += w.n_tiles;
     }
     const float cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : work[0].walk.tier_cap_um);
-    const int chunk = chunk_of(tune && tune->scan_chunk > 0 ? tune->scan_chunk : work[0].walk.scan_chunk);
     hipStream_t s = (hipStream_t)stream;
     char* base = (char*)d_workspace;
     PlanDev* dplan = (PlanDev*)base;
@@ -785,7 +790,8 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > total) grid = total;
-    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, chunk, (int32_t)grid,
+    const int mode = tune && tune->scan_rounds > 0 ? tune->scan_rounds : work[0].walk.scan_rounds;
+    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, rounds_of(mode), (int32_t)grid,
                             (unsigned long long*)d_counters, s);
 }
 

@@ -18,7 +18,14 @@ struct Rng {
     uint64_t s0, s1;
 };
 
-__device__ __forceinline__ uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+// Rotate right by k (0 < k < 32) as two funnel shifts of the 32-bit halves.
+__device__ __forceinline__ uint64_t rotr64_lt32(uint64_t x, uint32_t k)
+{
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    const uint32_t nlo = __builtin_amdgcn_alignbit(hi, lo, k);
+    const uint32_t nhi = __builtin_amdgcn_alignbit(lo, hi, k);
+    return ((uint64_t)nhi << 32) | nlo;
+}
 
 __device__ __forceinline__ uint64_t rng_next(Rng& r)
 {
@@ -26,21 +33,22 @@ __device__ __forceinline__ uint64_t rng_next(Rng& r)
     uint64_t b = r.s1;
     const uint64_t out = a + b;
     b ^= a;
-    r.s0 = rotl64(a, 55) ^ b ^ (b << 14);
-    r.s1 = rotl64(b, 36);
+    r.s0 = rotr64_lt32(a, 9) ^ b ^ (b << 14);  // rotl 55
+    r.s1 = rotr64_lt32(b, 28);                 // rotl 36
     return out;
 }
 
-__device__ __forceinline__ double u53_to_f64(uint64_t x) { return (double)(x >> 11) * 0x1.0p-53; }
-
-// float32(uniform53): the 53-bit integer rounded to the nearest float, then
-// scaled by 2^-53 (exact).  Equals float((x>>11) * 2^-53) of the reference.
-__device__ __forceinline__ float u53_to_f32(uint64_t x)
+// (x >> 11) * 2^-53 as hi * 2^-32 + (lo >> 11) * 2^-53: both terms and the
+// sum are exact in FP64 (53 bits), so this is the same value.
+__device__ __forceinline__ double u53_to_f64(uint64_t x)
 {
-    const uint64_t k = x >> 11;
-    const double d = (double)k;  // exact: k < 2^53
-    return (float)d * 0x1.0p-53f;
+    const uint32_t lo = (uint32_t)x, hi = (uint32_t)(x >> 32);
+    return __builtin_fma((double)hi, 0x1.0p-32, (double)(lo >> 11) * 0x1.0p-53);
 }
+
+// float32(uniform53): the exact FP64 uniform rounded once to float, which is
+// float((x>>11) * 2^-53) of the reference.
+__device__ __forceinline__ float u53_to_f32(uint64_t x) { return (float)u53_to_f64(x); }
 
 // ------------------------------------------------------------- geometry
 

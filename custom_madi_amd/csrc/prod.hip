@@ -21,12 +21,12 @@
 //
 // Plane tier.  A full classification at x records its binding plane (the
 // edge a->k of smallest margin), the second-smallest margin m2 over a's
-// other edges and n2 over k's other edges.  For a later point q within
-// R_a = m2 - alpha_a of x on a's side of the plane (resp. R_k = n2 - alpha_k
-// on k's side) every other plane keeps its margin above alpha, so the label
-// follows from the one plane: s = h - q.u;  s >= 0 -> cell a, inside iff
-// s >= alpha_a;  s < 0 -> cell k, inside iff -s >= alpha_k.  Membrane
-// bounces and gap walkers crossing a bisector are answered by this tier.
+// other edges and, for an outside point, n2 over k's other edges.  Near x
+// the label follows from the one plane: s = h - q.u;  s >= 0 -> cell a,
+// inside iff s >= alpha_a;  s < 0 -> cell k, inside iff -s >= alpha_k, with
+// validity radii m2 (outside answers) and m2 - alpha_a (inside answers) on
+// a's side, n2 and n2 - alpha_k on k's side (tier_query).  Membrane bounces
+// and gap walkers crossing a bisector are answered by this tier.
 #include "common.hpp"
 #include "phase.hpp"
 #include "rng.hpp"
@@ -155,13 +155,15 @@ __device__ __forceinline__ Refresh locate_f32(const GeomDev& g, int& a, float& p
     return r;
 }
 
-// Plane-tier state of one walker (registers).
+// Plane-tier state of one walker (registers).  The centre is the point where
+// the tier was built, relative to the anchor.
 struct Tier {
-    float ux, uy, uz, h;  // plane of edge a->k, unit normal towards k
-    float cx, cy, cz;     // centre, relative to the anchor
-    float ra2, rk2;       // squared validity radii on a's and on k's side (0 = none)
+    float ux, uy, uz, h;           // plane of edge a->k, unit normal towards k
+    float cx, cy, cz;              // centre
+    float ra_in2, ra_out2;         // squared radii on a's side: inside / outside answers
+    float rk_in2, rk_out2;         // the same on k's side (0 = no k-side answers)
     float alpha_a, alpha_k;
-    float eps;            // rounding bound of s over the validity ball
+    float eps;                     // rounding bound of s over the validity ball
     int k;
 };
 
@@ -170,81 +172,49 @@ __device__ __forceinline__ void tier_clear(Tier& t)
     t.ux = t.uy = t.uz = 0.f;
     t.h = 0.f;
     t.cx = t.cy = t.cz = 0.f;
-    t.ra2 = t.rk2 = 0.f;
+    t.ra_in2 = t.ra_out2 = t.rk_in2 = t.rk_out2 = 0.f;
     t.alpha_a = t.alpha_k = 0.f;
     t.eps = 0.f;
     t.k = -1;
 }
 
-// Build the tier at p (anchor a, located by r); k-side radius from k's edges.
-__device__ __forceinline__ void tier_build(const GeomDev& g, int a, float px, float py, float pz, const Refresh& r,
-                                           float cap, Tier& t, long long& facets)
+// Free water: every point is outside.
+__device__ __forceinline__ void tier_free_water(Tier& t)
 {
-    const float a1 = g.cell_alpha[a];
-    t.cx = px;
-    t.cy = py;
-    t.cz = pz;
-    t.alpha_a = a1;
-    if (r.c < 0) {
-        // no edge below the cap: every margin is at least cap
-        t.ux = t.uy = t.uz = 0.f;
-        t.h = cap;
-        t.k = -1;
-        t.alpha_k = 0.f;
-        const float ra = cap - a1 - 2.f * r.eps;
-        t.ra2 = ra > 0.f ? ra * ra : 0.f;
-        t.rk2 = 0.f;
-        t.eps = r.eps + kUlp * cap;
-        return;
-    }
-    const float4 e = g.nb_rec[r.c];
-    t.ux = e.x;
-    t.uy = e.y;
-    t.uz = e.z;
-    t.h = e.w;
-    const int k = g.nb_id[r.c];
-    t.k = k;
-    const float ak = g.cell_alpha[k];
-    t.alpha_k = ak;
-    const float ra = r.m2 - fmaxf(a1, 0.f) - 2.f * r.eps;
-    // k-side: margins of p with respect to k's edges, excluding the shared plane
-    const float qx = (float)((double)px - g.nb_e64[3 * r.c]);
-    const float qy = (float)((double)py - g.nb_e64[3 * r.c + 1]);
-    const float qz = (float)((double)pz - g.nb_e64[3 * r.c + 2]);
-    const float qn = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
-    const float epsk = kUlp * (fabsf(qx) + fabsf(qy) + fabsf(qz) + g.cell_hmax[k]);
-    float n2 = cap;
-    const int lo = g.nb_off[k], hi = g.nb_off[k + 1];
-    for (int j = lo; j < hi; ++j) {
-        const float4 f = g.nb_rec[j];
-        if (f.w - qn >= n2) break;
-        if (f.w == e.w && f.x == -e.x && f.y == -e.y && f.z == -e.z && g.nb_id[j] == a) continue;
-        const float n = f.w - (qx * f.x + qy * f.y + qz * f.z);
-        ++facets;
-        n2 = fminf(n2, n);
-    }
-    const float rk = n2 - fmaxf(ak, 0.f) - 2.f * fmaxf(r.eps, epsk);
-    t.ra2 = ra > 0.f ? ra * ra : 0.f;
-    t.rk2 = rk > 0.f ? rk * rk : 0.f;
-    const float rmax = fmaxf(ra, rk);
-    t.eps = fmaxf(r.eps, epsk) + kUlp * (1.8f * fmaxf(rmax, 0.f) + e.w);
+    tier_clear(t);
+    t.h = 1.f;
+    t.alpha_a = 2.f;
+    t.ra_out2 = __builtin_inff();
 }
 
-// Tier answer for q (relative to the anchor); returns 1 when certified.
+// Tier answer for q (relative to the anchor a); returns 1 when certified.
+//
+// Let x be the centre, m2 the second-smallest margin of a's planes at x and
+// n2 the second-smallest of k's planes at x (both capped).  Margins change by
+// at most |q - x|, so for |q - x| < m2 every plane of a other than a->k keeps
+// a positive margin: q lies in a's Voronoi cell iff s = h - q.u >= 0, and is
+// then outside the contracted cell iff s < alpha_a; inside needs every other
+// plane at alpha_a or more as well, i.e. |q - x| < m2 - alpha_a.  The same
+// holds on k's side with n2 and alpha_k.  An outside answer therefore has the
+// larger radius m2 (n2): walkers in the extracellular gap, whose second plane
+// is often closer than alpha + a step, keep their tier much longer.
 __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float qy, float qz, int& cell, int& inside)
 {
     const float dx = qx - t.cx, dy = qy - t.cy, dz = qz - t.cz;
     const float d2 = dx * dx + dy * dy + dz * dz;
     const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
     if (s >= 0.f) {
-        if (!(d2 < t.ra2) || fabsf(s - t.alpha_a) <= t.eps) return 0;
+        const int ins = s >= t.alpha_a;
+        if (fabsf(s - t.alpha_a) <= t.eps || !(d2 < (ins ? t.ra_in2 : t.ra_out2))) return 0;
         cell = a;
-        inside = s >= t.alpha_a;
+        inside = ins;
         return 1;
     }
-    if (!(d2 < t.rk2) || fabsf(-s - t.alpha_k) <= t.eps || -s <= t.eps) return 0;
+    const float r = -s;
+    const int ins = r >= t.alpha_k;
+    if (r <= t.eps || fabsf(r - t.alpha_k) <= t.eps || !(d2 < (ins ? t.rk_in2 : t.rk_out2))) return 0;
     cell = t.k;
-    inside = -s >= t.alpha_k;
+    inside = ins;
     return 1;
 }
 
@@ -320,60 +290,78 @@ __global__ __launch_bounds__(kBlock) void k_classify_graph(GeomDev g, const doub
     }
 }
 
-// Box-Muller first output in FP32 from the same two float32 uniforms the
-// FP64 walk uses; cos(2 pi u2) via the hardware cosine in revolutions.
-__device__ __forceinline__ float normal_f32(Rng& r)
+// Box-Muller first output in FP32, scaled: sqrt(c * log2(u1)) * cos(2 pi u2)
+// with c = -2 sigma^2 ln 2, from the same two float32 uniforms the FP64 walk
+// uses; the hardware cosine takes its argument in revolutions.
+__device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 {
     const float u1 = u53_to_f32(rng_next(r));
     const float u2 = u53_to_f32(rng_next(r));
-    return __builtin_sqrtf(-2.0f * __logf(u1)) * __builtin_amdgcn_cosf(u2);
+    return __builtin_amdgcn_sqrtf(c * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
 }
 
-// Full classification of q from anchor a, certified, with a new tier.
-// Returns 0 on a classifier failure.
-__device__ __forceinline__ int classify_step(const GeomDev& g, int& a, float& qx, float& qy, float& qz, float& ox,
-                                             float& oy, float& oz, float cap, int& cell, int& inside, Tier& t,
-                                             long long& iters, long long& facets, long long& rechecks)
-{
-    Refresh r = locate_f32(g, a, qx, qy, qz, ox, oy, oz, cap, iters, facets);
-    if (!r.ok) return 0;
-    cell = a;
-    inside = r.inside;
-    if (r.uncertain) {
-        ++rechecks;
-        if (!locate_f64(g, a, qx, qy, qz, ox, oy, oz, inside, facets)) return 0;
-        cell = a;
-        tier_clear(t);
-        return 1;
-    }
-    tier_build(g, a, qx, qy, qz, r, cap, t, facets);
-    return 1;
-}
+// Margins of p (relative to seed `cell`) over the cell's edges, in order of
+// half length, stopping once h - |p| reaches min(m2, cap).  Records are
+// fetched kPrefetch at a time so one memory round trip covers several edges.
+constexpr int kPrefetch = 4;
 
-struct TileOut {
-    double* slot;          // n_cols x 2 doubles of this tile
-    long long* occupancy;  // n_grid
-    long long* flags;      // MADI_N_FLAGS
+struct Scan {
+    float m1, m2, eps;
+    int c;  // edge of m1 (-1: none below the cap)
 };
 
-constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
-
-enum LanePhase : int { kStep = 0, kScanA = 1, kScanK = 2, kDone = 3 };
-
-__device__ __forceinline__ void occ_add(bool in_lds, int* lds_occ, long long* occ, int j)
+__device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, float py, float pz, float cap,
+                                          long long& facets)
 {
-    if (in_lds) atomicAdd(&lds_occ[j], 1);
-    else atomicAdd((unsigned long long*)&occ[j], 1ull);
+    Scan o;
+    o.m1 = o.m2 = __builtin_inff();
+    o.c = -1;
+    o.eps = kUlp * (fabsf(px) + fabsf(py) + fabsf(pz) + g.cell_hmax[cell]);
+    const float pn = __builtin_sqrtf(px * px + py * py + pz * pz);
+    const int lo = g.nb_off[cell], hi = g.nb_off[cell + 1];
+    int nf = 0;
+    for (int k = lo; k < hi; k += kPrefetch) {
+        float4 r[kPrefetch];
+#pragma unroll
+        for (int u = 0; u < kPrefetch; ++u)
+            r[u] = (k + u < hi) ? g.nb_rec[k + u] : make_float4(0.f, 0.f, 0.f, __builtin_inff());
+        bool stop = false;
+#pragma unroll
+        for (int u = 0; u < kPrefetch; ++u) {
+            if (!stop) {
+                if (r[u].w - pn >= fminf(o.m2, cap)) {
+                    stop = true;
+                } else {
+                    const float m = r[u].w - (px * r[u].x + py * r[u].y + pz * r[u].z);
+                    ++nf;
+                    if (m < o.m1) {
+                        o.m2 = o.m1;
+                        o.m1 = m;
+                        o.c = k + u;
+                    } else {
+                        o.m2 = fminf(o.m2, m);
+                    }
+                }
+            }
+        }
+        if (stop) break;
+    }
+    facets += nf;
+    return o;
 }
 
-// Tier from a finished two-sided scan: plane c of anchor a, a-side bound m2a,
-// k-side bound n2 (both capped), rounding bounds epsa / epsk, centre q.
-__device__ __forceinline__ void tier_from_scans(const GeomDev& g, int a, int c, float qx, float qy, float qz,
-                                                float m2a, float epsa, float n2, float epsk, Tier& t)
+// Tier from the two scans at q: binding edge c of anchor a with a-side bound
+// m2a; k-side bound n2 when the k side was scanned.
+__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float qx, float qy, float qz, float m2a,
+                                          float epsa, bool kside, float n2, float epsk, Tier& t)
 {
     const float4 e = g.nb_rec[c];
     const int k = g.nb_id[c];
     const float aa = g.cell_alpha[a], ak = g.cell_alpha[k];
+    const float ep = kside ? fmaxf(epsa, epsk) : epsa;
+    const float ra_in = m2a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m2a - 2.f * ep;
+    const float rk_in = kside ? n2 - fmaxf(ak, 0.f) - 2.f * ep : 0.f;
+    const float rk_out = kside ? n2 - 2.f * ep : 0.f;
     t.ux = e.x;
     t.uy = e.y;
     t.uz = e.z;
@@ -384,31 +372,147 @@ __device__ __forceinline__ void tier_from_scans(const GeomDev& g, int a, int c, 
     t.k = k;
     t.alpha_a = aa;
     t.alpha_k = ak;
-    const float ep = fmaxf(epsa, epsk);
-    const float ra = m2a - fmaxf(aa, 0.f) - 2.f * epsa;
-    const float rk = n2 - fmaxf(ak, 0.f) - 2.f * ep;
-    t.ra2 = ra > 0.f ? ra * ra : 0.f;
-    t.rk2 = rk > 0.f ? rk * rk : 0.f;
-    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra, rk), 0.f) + e.w);
+    t.ra_in2 = ra_in > 0.f ? ra_in * ra_in : 0.f;
+    t.ra_out2 = ra_out > 0.f ? ra_out * ra_out : 0.f;
+    t.rk_in2 = rk_in > 0.f ? rk_in * rk_in : 0.f;
+    t.rk_out2 = rk_out > 0.f ? rk_out * rk_out : 0.f;
+    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + e.w);
 }
 
 // Tier when no edge lies below the cap: every margin is at least cap.
 __device__ __forceinline__ void tier_no_plane(const GeomDev& g, int a, float qx, float qy, float qz, float cap,
                                               float epsa, Tier& t)
 {
+    tier_clear(t);
     const float aa = g.cell_alpha[a];
-    t.ux = t.uy = t.uz = 0.f;
     t.h = cap;
     t.cx = qx;
     t.cy = qy;
     t.cz = qz;
-    t.k = -1;
     t.alpha_a = aa;
-    t.alpha_k = 0.f;
-    const float ra = cap - aa - 2.f * epsa;
-    t.ra2 = ra > 0.f ? ra * ra : 0.f;
-    t.rk2 = 0.f;
+    const float ra = cap - fmaxf(aa, 0.f) - 2.f * epsa;
+    t.ra_in2 = ra > 0.f ? ra * ra : 0.f;
     t.eps = epsa + kUlp * cap;
+}
+
+// Full classification of q from anchor a: greedy location, label, FP64
+// re-decision near the alpha threshold, and a new tier.  q and o move with
+// the anchor.  The k side is scanned only for outside points (an inside
+// walker reaches k's side only by crossing the whole gap).  Returns 0 on a
+// classifier failure.
+__device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, float& qz, float& ox, float& oy,
+                             float& oz, float cap, int& inside, Tier& t, long long& iters, long long& facets,
+                             long long& rechecks)
+{
+    for (int it = 0; it < kMaxGraphIters; ++it) {
+        const Scan s = scan_cell(g, a, qx, qy, qz, cap, facets);
+        if (s.m1 < -s.eps) {
+            // a neighbour is nearer: move the anchor and rescan
+            const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
+            qx = (float)((double)qx - ex);
+            qy = (float)((double)qy - ey);
+            qz = (float)((double)qz - ez);
+            ox = (float)((double)ox + ex);
+            oy = (float)((double)oy + ey);
+            oz = (float)((double)oz + ez);
+            a = g.nb_id[s.c];
+            ++iters;
+            continue;
+        }
+        const float a1 = g.cell_alpha[a];
+        inside = s.m1 >= a1;
+        if (fabsf(s.m1 - a1) <= s.eps) {
+            ++rechecks;
+            if (!locate_f64(g, a, qx, qy, qz, ox, oy, oz, inside, facets)) return 0;
+            tier_clear(t);
+            return 1;
+        }
+        if (s.c < 0) {
+            tier_no_plane(g, a, qx, qy, qz, cap, s.eps, t);
+            return 1;
+        }
+        const bool kside = !inside;
+        float n2 = 0.f, epsk = 0.f;
+        if (kside) {
+            // k's margins at q; the smallest is at most the shared plane's
+            // (-m1 <= eps), so the second smallest bounds every other k plane
+            // whenever it is positive
+            const float kx = (float)((double)qx - g.nb_e64[3 * s.c]);
+            const float ky = (float)((double)qy - g.nb_e64[3 * s.c + 1]);
+            const float kz = (float)((double)qz - g.nb_e64[3 * s.c + 2]);
+            const Scan sk = scan_cell(g, g.nb_id[s.c], kx, ky, kz, cap, facets);
+            n2 = fminf(sk.m2, cap);
+            epsk = sk.eps;
+        }
+        tier_make(g, a, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, kside, n2, epsk, t);
+        return 1;
+    }
+    return 0;
+}
+
+struct TileOut {
+    double* slot;          // n_cols x 2 doubles of this tile
+    long long* occupancy;  // n_grid
+    long long* flags;      // MADI_N_FLAGS
+};
+
+constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
+
+// LDS a tile walk needs.
+struct TileLds {
+    double cols[kWaves * kColChunk * 2];
+    int occ[kMaxLdsGrid];
+};
+
+// Records per scan chunk: all are fetched before any is used, so a chunk
+// costs one memory round trip.
+constexpr int kChunk = 8;
+
+// Running state of one facet scan (anchor cell `sc`, point s relative to it).
+struct ScanState {
+    int cur, end, c;
+    float m1, m2, pn, eps;
+    float sx, sy, sz;
+};
+
+__device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
+{
+    S.cur = g.nb_off[cell];
+    S.end = g.nb_off[cell + 1];
+    S.c = -1;
+    S.m1 = S.m2 = __builtin_inff();
+    S.sx = sx;
+    S.sy = sy;
+    S.sz = sz;
+    S.pn = __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
+    S.eps = kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]);
+}
+
+// One chunk of the scan (requires S.cur < S.end).  Margins of records past
+// the early-exit point are at least min(m2, cap), so evaluating them cannot
+// change m1, c or the capped m2: the chunk needs no per-record branch.
+// Records past the cell's end are masked.  The scan ends when the chunk's
+// last half length minus |s| reaches min(m2, cap) (edges are sorted by h).
+__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets)
+{
+    float4 r[kChunk];
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        const int k = S.cur + u;
+        r[u] = g.nb_rec[k < S.end ? k : S.end - 1];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunk; ++u) {
+        float m = r[u].w - (S.sx * r[u].x + S.sy * r[u].y + S.sz * r[u].z);
+        m = (S.cur + u < S.end) ? m : __builtin_inff();
+        const bool lt = m < S.m1;
+        S.m2 = lt ? S.m1 : fminf(S.m2, m);
+        S.c = lt ? S.cur + u : S.c;
+        S.m1 = fminf(S.m1, m);
+    }
+    facets += (uint32_t)min(kChunk, S.end - S.cur);
+    S.cur += kChunk;
+    if (r[kChunk - 1].w - S.pn >= fminf(S.m2, cap)) S.cur = S.end;
 }
 
 // One tile of kBlock walkers [w_lo, w_lo + kBlock) of an entry: the whole
@@ -416,27 +520,32 @@ __device__ __forceinline__ void tier_no_plane(const GeomDev& g, int a, float qx,
 // block must call it (it synchronises on LDS).  NP >= plan.n_pairs bounds
 // the moment registers.
 //
-// Each lane runs its walker as a small state machine, so lanes of a wave need
-// not be at the same step: per iteration a lane either takes its next step
-// (draws, proposal, plane tier) or advances its pending classification by E
-// edges (kScanA: greedy location and a-side margins; kScanK: k-side margins
-// for the new tier).  A step completes when its label is known.  The wave's
-// cost per iteration is bounded by one step plus E edge evaluations, instead
-// of the longest classification among its 64 lanes.  Every walker still
-// consumes its own random stream in order, so results do not depend on E.
-template <int NP, int E>
+// Each lane runs its walker as a small state machine, so lanes of a wave
+// need not be at the same step.  Per iteration a lane either takes its next
+// step (draws, proposal, plane tier) or, while its proposal is being
+// classified, advances the classification by up to R rounds of (one chunk
+// of kChunk facets, then the end-of-scan logic: greedy move, label, FP64
+// re-decision, start of the k-side scan).  A step completes when its label
+// is known.  A wave's iteration thus costs one step plus R chunks, instead
+// of the longest classification among its 64 lanes, and walkers with long
+// classifications fall behind by a few iterations rather than stalling the
+// wave.  Every walker consumes its own random stream in order, so results
+// do not depend on R.
+template <int NP, int R>
 __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
                                               int64_t w_lo, const PlanDev& plan, const Events& ev, const TileOut& out,
-                                              float cap_width, unsigned long long* counters, double* lds_cols,
-                                              int* lds_occ)
+                                              float cap_width, unsigned long long* counters, TileLds& L)
 {
+    enum : int { kStep = 0, kScanA = 1, kScanK = 2, kDone = 3 };
+    const int lane = threadIdx.x & 63;
     const int64_t w = w_lo + threadIdx.x;
     const bool live = w < p.n_walkers;
-    long long n_class = 0, n_tier = 0, iters = 0, facets = 0, rechecks = 0, crossings = 0, rejects = 0;
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
+    long long iters = 0, facets = 0, rechecks = 0;
     const float cap = g.alpha_max + cap_width;
     const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
     if (occ_in_lds)
-        for (int j = threadIdx.x; j < p.n_grid; j += blockDim.x) lds_occ[j] = 0;
+        for (int j = threadIdx.x; j < p.n_grid; j += kBlock) L.occ[j] = 0;
     __syncthreads();
 
     Rng rng{0, 0};
@@ -458,19 +567,16 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
         oy = -py;
         oz = -pz;
         ++n_class;
-        if (!classify_step(g, a, px, py, pz, ox, oy, oz, cap, cell, in, t, iters, facets, rechecks)) {
+        if (!classify_full(g, a, px, py, pz, ox, oy, oz, cap, in, t, iters, facets, rechecks)) {
             err = 1;
             escaped = 1;
         }
+        cell = a;
         ox = -px;
         oy = -py;
         oz = -pz;
     } else if (live) {
-        // free water: a tier that answers "outside" everywhere
-        t.h = 1.f;
-        t.alpha_a = 2.f;
-        t.ra2 = __builtin_inff();
-        t.eps = 0.f;
+        tier_free_water(t);
     }
     const float lox = (float)(-x0), loy = (float)(-y0), loz = (float)(-z0);
     const float hix = (float)(g.L - x0), hiy = (float)(g.L - y0), hiz = (float)(g.L - z0);
@@ -479,29 +585,32 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     float dM[NP][3];
 #pragma unroll
     for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
-    if (live && !err && in) occ_add(occ_in_lds, lds_occ, out.occupancy, 0);
+    if (live && !err && in) {
+        if (occ_in_lds) atomicAdd(&L.occ[0], 1);
+        else atomicAdd((unsigned long long*)&out.occupancy[0], 1ull);
+    }
 
-    const float sig = (float)p.sigma;
+    const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
     const float ts = (float)p.ts;
     int phase = (live && !escaped && p.n_steps > 0) ? kStep : kDone;
     int step = 0, until_sample = p.steps_per_h, j_sample = 0, next_ev = 0;
-    // pending step: proposal (relative to the scan anchor), offset, reject bits
+    // pending step: proposal relative to its anchor, offset, acceptance bits
     float qx = 0.f, qy = 0.f, qz = 0.f, nox = 0.f, noy = 0.f, noz = 0.f;
-    int na = 0, rj = 0;
-    int pin = 0, pcell = 0;  // label of the pending proposal, once decided
-    // scan state
-    int cur = 0, end = 0, c = -1;
-    float m1 = 0.f, m2 = 0.f, pn = 0.f, eps = 0.f;
-    float kx = 0.f, ky = 0.f, kz = 0.f;
+    int na = 0, rj = 0, pin = 0, pcell = 0, moves = 0;
+    // scan state and the a-side results kept for the tier
+    ScanState S;
+    S.cur = S.end = 0;
+    S.c = -1;
+    S.m1 = S.m2 = S.pn = S.eps = S.sx = S.sy = S.sz = 0.f;
     int sa_c = -1;
     float sa_m2 = 0.f, sa_eps = 0.f;
 
     while (__any(phase != kDone)) {
-        int complete = 0, tier_mode = 0;  // tier on accept: 0 keep, 1 two-sided, 2 clear, 3 no plane
+        int complete = 0, tmode = 0;  // tier on accept: 0 keep, 1 two-sided, 2 clear, 3 no plane, 4 a side only
         if (phase == kStep) {
-            const float dx = normal_f32(rng) * sig;
-            const float dy = normal_f32(rng) * sig;
-            const float dz = normal_f32(rng) * sig;
+            const float dx = normal_scaled_f32(rng, nscale);
+            const float dy = normal_scaled_f32(rng, nscale);
+            const float dz = normal_scaled_f32(rng, nscale);
             const uint64_t ku = rng_next(rng) >> 11;
             rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
             qx = px + dx;
@@ -515,106 +624,88 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
                 escaped = 1;
                 phase = kDone;
-                // samples after an escape keep the frozen label
-                for (int j = j_sample + 1; j < p.n_grid; ++j)
-                    if (in) occ_add(occ_in_lds, lds_occ, out.occupancy, j);
+                // samples after an escape keep the last label
+                if (in)
+                    for (int j = j_sample + 1; j < p.n_grid; ++j) {
+                        if (occ_in_lds) atomicAdd(&L.occ[j], 1);
+                        else atomicAdd((unsigned long long*)&out.occupancy[j], 1ull);
+                    }
             } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
                 ++n_tier;
                 complete = 1;
             } else {
                 ++n_class;
-                cur = g.nb_off[na];
-                end = g.nb_off[na + 1];
-                m1 = m2 = __builtin_inff();
-                c = -1;
-                pn = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
-                eps = kUlp * (fabsf(qx) + fabsf(qy) + fabsf(qz) + g.cell_hmax[na]);
+                moves = 0;
+                scan_start(g, na, qx, qy, qz, S);
                 phase = kScanA;
             }
         }
-        if (phase == kScanA || phase == kScanK) {
-            const bool sa = phase == kScanA;
-            const float sx = sa ? qx : kx, sy = sa ? qy : ky, sz = sa ? qz : kz;
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
-                if (cur < end) {
-                    const float4 r = g.nb_rec[cur];
-                    if (r.w - pn >= fminf(m2, cap)) {
-                        cur = end;
+        for (int r = 0; r < R; ++r) {
+            const bool scanning = phase == kScanA || phase == kScanK;
+            if (scanning && S.cur < S.end) scan_chunk(g, cap, S, nfac);
+            if (scanning && S.cur >= S.end) {
+                if (phase == kScanK) {
+                    // k's smallest margin is at most the shared plane's
+                    // (-m1 <= eps), so its second smallest bounds every
+                    // other k plane whenever it is positive
+                    complete = 1;
+                    tmode = 1;
+                } else if (S.m1 < -S.eps) {
+                    if (++moves > kMaxGraphIters) {
+                        // the greedy walk does not settle: flag the walker
+                        err = 1;
+                        escaped = 1;
+                        phase = kDone;
                     } else {
-                        const float m = r.w - (sx * r.x + sy * r.y + sz * r.z);
-                        ++facets;
-                        if (m < m1) {
-                            m2 = m1;
-                            m1 = m;
-                            c = cur;
-                        } else {
-                            m2 = fminf(m2, m);
-                        }
-                        ++cur;
-                    }
-                }
-            }
-            if (cur >= end) {
-                if (sa) {
-                    if (m1 < -eps) {
                         // a neighbour is nearer: move the anchor and rescan
-                        const double ex = g.nb_e64[3 * c], ey = g.nb_e64[3 * c + 1], ez = g.nb_e64[3 * c + 2];
+                        const double ex = g.nb_e64[3 * S.c], ey = g.nb_e64[3 * S.c + 1], ez = g.nb_e64[3 * S.c + 2];
                         qx = (float)((double)qx - ex);
                         qy = (float)((double)qy - ey);
                         qz = (float)((double)qz - ez);
                         nox = (float)((double)nox + ex);
                         noy = (float)((double)noy + ey);
                         noz = (float)((double)noz + ez);
-                        na = g.nb_id[c];
+                        na = g.nb_id[S.c];
                         ++iters;
-                        cur = g.nb_off[na];
-                        end = g.nb_off[na + 1];
-                        m1 = m2 = __builtin_inff();
-                        c = -1;
-                        pn = __builtin_sqrtf(qx * qx + qy * qy + qz * qz);
-                        eps = kUlp * (fabsf(qx) + fabsf(qy) + fabsf(qz) + g.cell_hmax[na]);
-                    } else {
-                        const float a1 = g.cell_alpha[na];
-                        pin = m1 >= a1;
-                        pcell = na;
-                        if (fabsf(m1 - a1) <= eps) {
-                            ++rechecks;
-                            if (!locate_f64(g, na, qx, qy, qz, nox, noy, noz, pin, facets)) {
-                                err = 1;
-                                escaped = 1;
-                                phase = kDone;
-                            } else {
-                                pcell = na;
-                                complete = 1;
-                                tier_mode = 2;
-                            }
-                        } else if (c < 0) {
-                            sa_eps = eps;
-                            complete = 1;
-                            tier_mode = 3;
-                        } else {
-                            sa_c = c;
-                            sa_m2 = fminf(m2, cap);
-                            sa_eps = eps;
-                            const int k = g.nb_id[c];
-                            kx = (float)((double)qx - g.nb_e64[3 * c]);
-                            ky = (float)((double)qy - g.nb_e64[3 * c + 1]);
-                            kz = (float)((double)qz - g.nb_e64[3 * c + 2]);
-                            cur = g.nb_off[k];
-                            end = g.nb_off[k + 1];
-                            m1 = m2 = __builtin_inff();
-                            pn = __builtin_sqrtf(kx * kx + ky * ky + kz * kz);
-                            eps = kUlp * (fabsf(kx) + fabsf(ky) + fabsf(kz) + g.cell_hmax[k]);
-                            phase = kScanK;
-                        }
+                        scan_start(g, na, qx, qy, qz, S);
                     }
                 } else {
-                    // k-side scan done: its smallest margin is at most the shared
-                    // plane's, so its second smallest bounds every other k plane
-                    complete = 1;
-                    tier_mode = 1;
+                    const float a1 = g.cell_alpha[na];
+                    pin = S.m1 >= a1;
+                    pcell = na;
+                    sa_c = S.c;
+                    sa_m2 = fminf(S.m2, cap);
+                    sa_eps = S.eps;
+                    if (fabsf(S.m1 - a1) <= S.eps) {
+                        ++rechecks;
+                        if (!locate_f64(g, na, qx, qy, qz, nox, noy, noz, pin, facets)) {
+                            err = 1;
+                            escaped = 1;
+                            phase = kDone;
+                        } else {
+                            pcell = na;
+                            complete = 1;
+                            tmode = 2;
+                        }
+                    } else if (S.c < 0) {
+                        complete = 1;
+                        tmode = 3;
+                    } else if (pin) {
+                        // an inside walker reaches k's side only across the
+                        // whole gap: no k-side scan
+                        complete = 1;
+                        tmode = 4;
+                    } else {
+                        const int k = g.nb_id[S.c];
+                        const float kx = (float)((double)qx - g.nb_e64[3 * S.c]);
+                        const float ky = (float)((double)qy - g.nb_e64[3 * S.c + 1]);
+                        const float kz = (float)((double)qz - g.nb_e64[3 * S.c + 2]);
+                        scan_start(g, k, kx, ky, kz, S);
+                        phase = kScanK;
+                    }
                 }
+                if (complete) phase = kStep;  // leaves the scan; completed below
             }
         }
         if (complete) {
@@ -630,9 +721,10 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                 }
             }
             if (accept) {
-                if (tier_mode == 1) tier_from_scans(g, na, sa_c, qx, qy, qz, sa_m2, sa_eps, fminf(m2, cap), eps, t);
-                else if (tier_mode == 2) tier_clear(t);
-                else if (tier_mode == 3) tier_no_plane(g, na, qx, qy, qz, cap, sa_eps, t);
+                if (tmode == 1) tier_make(g, na, sa_c, qx, qy, qz, sa_m2, sa_eps, true, fminf(S.m2, cap), S.eps, t);
+                else if (tmode == 4) tier_make(g, na, sa_c, qx, qy, qz, sa_m2, sa_eps, false, 0.f, 0.f, t);
+                else if (tmode == 2) tier_clear(t);
+                else if (tmode == 3) tier_no_plane(g, na, qx, qy, qz, cap, sa_eps, t);
                 a = na;
                 in = pin;
                 cell = pcell;
@@ -657,7 +749,10 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             if (--until_sample == 0) {
                 until_sample = p.steps_per_h;
                 const int j = ++j_sample;
-                if (in) occ_add(occ_in_lds, lds_occ, out.occupancy, j);
+                if (in) {
+                    if (occ_in_lds) atomicAdd(&L.occ[j], 1);
+                    else atomicAdd((unsigned long long*)&out.occupancy[j], 1ull);
+                }
                 if (next_ev < ev.n && ev.j[next_ev] == j) {
                     // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
                     const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
@@ -680,25 +775,26 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             phase = step >= p.n_steps ? kDone : kStep;
         }
     }
+    facets += nfac;
     {
         const long long ne = wave_popc(live && escaped);
         const long long nr = wave_popc(live && err);
-        if ((threadIdx.x & 63) == 0) {
+        if (lane == 0) {
             if (ne) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
             if (nr) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
         }
     }
     __syncthreads();
     if (occ_in_lds)
-        for (int j = threadIdx.x; j < p.n_grid; j += blockDim.x)
-            if (lds_occ[j]) atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)lds_occ[j]);
-    emit_columns<float, NP>(plan, dM, live && !escaped, lds_cols, out.slot);
+        for (int j = threadIdx.x; j < p.n_grid; j += kBlock)
+            if (L.occ[j]) atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)L.occ[j]);
+    emit_columns<float, NP>(plan, dM, live && !escaped, L.cols, out.slot);
     if (counters) {
         const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
                      v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
                      v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
                      v6 = wave_sum_f64((double)rejects);
-        if ((threadIdx.x & 63) == 0) {
+        if (lane == 0) {
             atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
             atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
             atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)v2);
@@ -711,32 +807,30 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
 }
 
 // One entry: block b runs tile b.
-template <int NP, int E>
-__global__ __launch_bounds__(kBlock) void k_walk_f32(GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow,
+template <int NP, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f32(GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow,
                                                     FusedOut fo, float cap_width, unsigned long long* counters)
 {
-    __shared__ double lds_cols[kWaves * kColChunk * 2];
-    __shared__ int lds_occ[kMaxLdsGrid];
+    __shared__ TileLds lds;
     TileOut out;
     out.slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
     out.occupancy = fo.occupancy;
     out.flags = fo.flags;
-    walk_tile_f32<NP, E>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters,
-                         lds_cols, lds_occ);
+    walk_tile_f32<NP, R>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters,
+                             lds);
 }
 
 // Persistent work-stealing launch over many entries: each workgroup takes
 // the next global tile index from an atomic counter, finds its work item by
 // binary search on work_begin, and runs that tile.  Every workgroup exits
 // once the counter passes the total, so the grid drains.
-template <int NP, int E>
-__global__ __launch_bounds__(kBlock) void k_batch_f32(const WorkDev* __restrict__ work, int32_t n_work, int64_t total,
+template <int NP, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_f32(const WorkDev* __restrict__ work, int32_t n_work, int64_t total,
                                                      unsigned long long* queue, const PlanDev* __restrict__ plan,
                                                      Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
                                                      unsigned long long* counters)
 {
-    __shared__ double lds_cols[kWaves * kColChunk * 2];
-    __shared__ int lds_occ[kMaxLdsGrid];
+    __shared__ TileLds lds;
     __shared__ long long s_tile;
     __shared__ int s_item;
     for (;;) {
@@ -764,8 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_f32(const WorkDev* __restrict_
         out.slot = W.partial + (size_t)tile * plan->n_cols * 2;
         out.occupancy = W.occupancy;
         out.flags = W.flags;
-        walk_tile_f32<NP, E>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds_cols,
-                             lds_occ);
+        walk_tile_f32<NP, R>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds);
     }
 }
 
@@ -796,46 +889,42 @@ int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_
     return check_launch("k_classify_graph");
 }
 
-#define MADI_DISPATCH_NP_E(NP_VAL, E_VAL, LAUNCH)                  \
-    do {                                                           \
-        const int np_ = (NP_VAL), e_ = (E_VAL);                    \
-        if (np_ <= 2) {                                            \
-            if (e_ <= 4) LAUNCH(2, 4);                             \
-            else if (e_ <= 8) LAUNCH(2, 8);                        \
-            else LAUNCH(2, 16);                                    \
-        } else if (np_ <= 4) {                                     \
-            if (e_ <= 4) LAUNCH(4, 4);                             \
-            else if (e_ <= 8) LAUNCH(4, 8);                        \
-            else LAUNCH(4, 16);                                    \
-        } else {                                                   \
-            if (e_ <= 4) LAUNCH(MADI_MAX_PAIRS, 4);                \
-            else if (e_ <= 8) LAUNCH(MADI_MAX_PAIRS, 8);           \
-            else LAUNCH(MADI_MAX_PAIRS, 16);                       \
-        }                                                          \
+#define MADI_DISPATCH_R(NPV, R_VAL, LAUNCH) \
+    do {                                     \
+        if ((R_VAL) <= 1) LAUNCH(NPV, 1);    \
+        else if ((R_VAL) == 2) LAUNCH(NPV, 2); \
+        else LAUNCH(NPV, 3);                 \
+    } while (0)
+#define MADI_DISPATCH_NP(NP_VAL, R_VAL, LAUNCH)                      \
+    do {                                                             \
+        const int np_ = (NP_VAL);                                    \
+        if (np_ <= 2) MADI_DISPATCH_R(2, R_VAL, LAUNCH);             \
+        else if (np_ <= 4) MADI_DISPATCH_R(4, R_VAL, LAUNCH);        \
+        else MADI_DISPATCH_R(MADI_MAX_PAIRS, R_VAL, LAUNCH);         \
     } while (0)
 
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
-                          float cap_width, int scan_chunk, unsigned long long* counters, hipStream_t s)
+                          float cap_width, int rounds, unsigned long long* counters, hipStream_t s)
 {
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
-#define MADI_LAUNCH_WALK(NPV, EV)                                                                                   \
-    hipLaunchKernelGGL((k_walk_f32<NPV, EV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, fo,     \
+#define MADI_LAUNCH_WALK(NPV, RV)                                                                                   \
+    hipLaunchKernelGGL((k_walk_f32<NPV, RV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, fo,      \
                        cap_width, counters)
-    MADI_DISPATCH_NP_E(fo.ev.n_pairs, scan_chunk, MADI_LAUNCH_WALK);
+    MADI_DISPATCH_NP(fo.ev.n_pairs, rounds, MADI_LAUNCH_WALK);
 #undef MADI_LAUNCH_WALK
     return check_launch("k_walk_f32");
 }
 
 int launch_batch_f32(const WorkDev* work, int32_t n_work, int64_t total, unsigned long long* queue, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int scan_chunk, int32_t grid,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
                      unsigned long long* counters, hipStream_t s)
 {
     if (total == 0) return MADI_OK;
-#define MADI_LAUNCH_BATCH(NPV, EV)                                                                                  \
-    hipLaunchKernelGGL((k_batch_f32<NPV, EV>), dim3(grid), dim3(kBlock), 0, s, work, n_work, total, queue, plan, ev, \
-                       jump_pow, cap_width, counters)
-    MADI_DISPATCH_NP_E(ev.n_pairs, scan_chunk, MADI_LAUNCH_BATCH);
+#define MADI_LAUNCH_BATCH(NPV, RV)                                                                                  \
+    hipLaunchKernelGGL((k_batch_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, n_work, total, queue, plan,  \
+                       ev, jump_pow, cap_width, counters)
+    MADI_DISPATCH_NP(ev.n_pairs, rounds, MADI_LAUNCH_BATCH);
 #undef MADI_LAUNCH_BATCH
     return check_launch("k_batch_f32");
 }

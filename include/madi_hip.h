@@ -158,7 +158,7 @@ typedef struct {
     double pp;
     uint64_t seed;          /* walk seed (device RNG) */
     uint64_t first_walker;  /* RNG subsequence index of walker 0 */
-    int32_t scan_chunk;     /* FP32 walk: facet evaluations per lane per iteration (4, 8 or 16; 0 = 8) */
+    int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-3 (0 = 2) */
     int32_t reserved;
 } madi_walk_params;
 
@@ -236,7 +236,7 @@ typedef struct {
 typedef struct {
     int32_t grid_per_cu;  /* persistent workgroups per CU (0 = 4) */
     float tier_cap_um;    /* plane-tier cap (0 = first item's walk.tier_cap_um) */
-    int32_t scan_chunk;   /* facet evaluations per lane per iteration (0 = first item's walk.scan_chunk) */
+    int32_t scan_rounds;  /* 1-3; 0 = first item's walk.scan_rounds */
     int32_t reserved;
 } madi_tune;
 
