@@ -264,12 +264,28 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
                 set_error("geometry: nb_offsets must be non-decreasing");
                 return MADI_ERR_ARG;
             }
-        for (int c = 0; c <= nc; ++c) off[c] = (int32_t)d->nb_offsets[c];
+        // padded offsets: each cell's list rounded up to a multiple of kEdgePad
+        int64_t padded = 0;
+        for (int c = 0; c < nc; ++c) {
+            off[c] = (int32_t)padded;
+            const int64_t deg = d->nb_offsets[c + 1] - d->nb_offsets[c];
+            padded += (deg + kEdgePad - 1) / kEdgePad * kEdgePad;
+            if (padded > INT32_MAX) {
+                set_error("geometry: too many edges");
+                return MADI_ERR_ARG;
+            }
+        }
+        off[nc] = (int32_t)padded;
         // each cell's edges sorted by half length (the FP32 walk stops its edge
-        // loop once h_j - |p| passes the bound it needs)
-        std::vector<float4> rec(E);
-        std::vector<int32_t> ids(E);
-        std::vector<double> vec(3 * E);
+        // loop once h_j - |p| passes the bound it needs), then sentinels
+        std::vector<float4> rec(padded, make_float4(0.f, 0.f, 0.f, INFINITY));
+        std::vector<int32_t> ids(padded);
+        std::vector<double> vec(3 * padded, 0.0);
+        for (int c = 0; c < nc; ++c)
+            for (int64_t k = off[c]; k < off[c + 1]; ++k) {
+                ids[k] = c;
+                vec[3 * k] = 1e30;
+            }
         std::vector<float> alpha(nc), hmax(nc, 0.f);
         std::vector<double> alpha64(nc);
         std::vector<std::pair<double, int64_t>> order;
@@ -280,7 +296,7 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
             amin = fminf(amin, alpha[c]);
             amax = fmaxf(amax, alpha[c]);
             order.clear();
-            for (int64_t k = off[c]; k < off[c + 1]; ++k) {
+            for (int64_t k = d->nb_offsets[c]; k < d->nb_offsets[c + 1]; ++k) {
                 const int nb = d->nb_ids[k];
                 if (nb < 0 || nb >= nc) {
                     set_error("geometry: neighbour id out of range");

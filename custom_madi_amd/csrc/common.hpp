@@ -94,4 +94,10 @@ struct GeomDev {
 
 constexpr int kMaxGraphIters = 256;
 
+// Each cell's edge list is padded to a multiple of kEdgePad with sentinel
+// edges (unit vector 0, half length +inf, neighbour = the cell itself, FP64
+// vector of length 1e30): their margin is +inf in FP32 and their bisector is
+// never the nearer one in FP64, so scans may read whole groups of kEdgePad.
+constexpr int kEdgePad = 8;
+
 }  // namespace madi

@@ -290,74 +290,108 @@ __global__ __launch_bounds__(kBlock) void k_classify_graph(GeomDev g, const doub
     }
 }
 
-// Box-Muller first output in FP32, scaled: sqrt(c * log2(u1)) * cos(2 pi u2)
-// with c = -2 sigma^2 ln 2, from the same two float32 uniforms the FP64 walk
-// uses; the hardware cosine takes its argument in revolutions.
-__device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
+// Box-Muller first output in FP32, scaled by sigma, from the top 32 bits of
+// the two draws the FP64 walk turns into its uniforms (u = hi * 2^-32, within
+// 2^-24 of them; u1 = 0 is replaced by 2^-33):
+// sqrt(c * log2(u1)) * cos(2 pi u2) with c = -2 sigma^2 ln 2, log2(u1) taken
+// as log2(hi) - 32, and the hardware cosine in revolutions.
+// Round-to-nearest u32 -> f32 as the one conversion instruction (the
+// compiler widens a conversion of the high half of a 64-bit value into a
+// 64-bit-to-float sequence).
+__device__ __forceinline__ float u32_to_f32(uint32_t v)
 {
-    const float u1 = u53_to_f32(rng_next(r));
-    const float u2 = u53_to_f32(rng_next(r));
-    return __builtin_amdgcn_sqrtf(c * __builtin_amdgcn_logf(u1)) * __builtin_amdgcn_cosf(u2);
+    float f;
+    asm("v_cvt_f32_u32 %0, %1" : "=v"(f) : "v"(v));
+    return f;
 }
 
-// Margins of p (relative to seed `cell`) over the cell's edges, in order of
-// half length, stopping once h - |p| reaches min(m2, cap).  Records are
-// fetched kPrefetch at a time so one memory round trip covers several edges.
-constexpr int kPrefetch = 4;
+__device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
+{
+    const float h1 = fmaxf(u32_to_f32((uint32_t)(rng_next(r) >> 32)), 0.5f);
+    const float u2 = u32_to_f32((uint32_t)(rng_next(r) >> 32)) * 0x1.0p-32f;
+    return __builtin_amdgcn_sqrtf(fmaf(c, __builtin_amdgcn_logf(h1), -32.f * c)) * __builtin_amdgcn_cosf(u2);
+}
+
+// Running state of one facet scan: margins of the point s (relative to seed
+// `cell`) over the cell's edges in order of half length.
+struct ScanState {
+    int cur, end, c;
+    float m1, m2, pn, eps;
+    float nx, ny, nz;  // -s
+};
+
+__device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
+{
+    S.cur = g.nb_off[cell];
+    S.end = g.nb_off[cell + 1];
+    S.c = -1;
+    S.m1 = S.m2 = __builtin_inff();
+    S.nx = -sx;
+    S.ny = -sy;
+    S.nz = -sz;
+    S.pn = __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
+    S.eps = kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]);
+}
+
+// One group of kEdgePad edges (requires S.cur < S.end; lists are padded with
+// sentinels whose margin is +inf).  All records are fetched before any is
+// used, so a group costs one memory round trip.  Margins of edges past the
+// early-exit point are at least min(m2, cap), so evaluating them cannot
+// change m1, c or the capped m2: the group needs no per-edge branch.  The
+// scan ends when the group's last half length minus |s| reaches min(m2, cap)
+// (edges are sorted by h, so every later margin is at least that).
+__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets)
+{
+    const float4* __restrict__ rec = g.nb_rec + S.cur;
+    float4 r[kEdgePad];
+#pragma unroll
+    for (int u = 0; u < kEdgePad; ++u) r[u] = rec[u];
+    int cl = -1;
+#pragma unroll
+    for (int u = 0; u < kEdgePad; ++u) {
+        const float m = fmaf(S.nx, r[u].x, fmaf(S.ny, r[u].y, fmaf(S.nz, r[u].z, r[u].w)));
+        cl = m < S.m1 ? u : cl;
+        S.m2 = fminf(S.m2, fmaxf(S.m1, m));
+        S.m1 = fminf(S.m1, m);
+    }
+    if (cl >= 0) S.c = S.cur + cl;
+    facets += kEdgePad;
+    S.cur += kEdgePad;
+    if (r[kEdgePad - 1].w - S.pn >= fminf(S.m2, cap)) S.cur = S.end;
+}
 
 struct Scan {
     float m1, m2, eps;
     int c;  // edge of m1 (-1: none below the cap)
 };
 
+// Whole scan of one cell.
 __device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, float py, float pz, float cap,
                                           long long& facets)
 {
-    Scan o;
-    o.m1 = o.m2 = __builtin_inff();
-    o.c = -1;
-    o.eps = kUlp * (fabsf(px) + fabsf(py) + fabsf(pz) + g.cell_hmax[cell]);
-    const float pn = __builtin_sqrtf(px * px + py * py + pz * pz);
-    const int lo = g.nb_off[cell], hi = g.nb_off[cell + 1];
-    int nf = 0;
-    for (int k = lo; k < hi; k += kPrefetch) {
-        float4 r[kPrefetch];
-#pragma unroll
-        for (int u = 0; u < kPrefetch; ++u)
-            r[u] = (k + u < hi) ? g.nb_rec[k + u] : make_float4(0.f, 0.f, 0.f, __builtin_inff());
-        bool stop = false;
-#pragma unroll
-        for (int u = 0; u < kPrefetch; ++u) {
-            if (!stop) {
-                if (r[u].w - pn >= fminf(o.m2, cap)) {
-                    stop = true;
-                } else {
-                    const float m = r[u].w - (px * r[u].x + py * r[u].y + pz * r[u].z);
-                    ++nf;
-                    if (m < o.m1) {
-                        o.m2 = o.m1;
-                        o.m1 = m;
-                        o.c = k + u;
-                    } else {
-                        o.m2 = fminf(o.m2, m);
-                    }
-                }
-            }
-        }
-        if (stop) break;
-    }
+    ScanState S;
+    scan_start(g, cell, px, py, pz, S);
+    uint32_t nf = 0;
+    while (S.cur < S.end) scan_chunk(g, cap, S, nf);
     facets += nf;
-    return o;
+    return Scan{S.m1, S.m2, S.eps, S.c};
 }
 
-// Tier from the two scans at q: binding edge c of anchor a with a-side bound
-// m2a; k-side bound n2 when the k side was scanned.
+// Tier from the scans at q: binding edge c of anchor a with a-side bound m2a;
+// k-side bound n2 when the k side was scanned.  c < 0 means no edge below the
+// cap (every margin is at least cap: pass m2a = cap; the tier then answers
+// "inside a" up to cap - alpha_a); m2a = -inf gives a tier that answers
+// nothing.  One code path for every case, so the walk's update is branch-free.
 __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float qx, float qy, float qz, float m2a,
                                           float epsa, bool kside, float n2, float epsk, Tier& t)
 {
-    const float4 e = g.nb_rec[c];
-    const int k = g.nb_id[c];
+    const bool plane = c >= 0;
+    const int ci = plane ? c : 0;
+    const float4 e0 = g.nb_rec[ci];
+    const int k = plane ? g.nb_id[ci] : a;
+    const float4 e = plane ? e0 : make_float4(0.f, 0.f, 0.f, m2a);
     const float aa = g.cell_alpha[a], ak = g.cell_alpha[k];
+    kside = kside && plane;
     const float ep = kside ? fmaxf(epsa, epsk) : epsa;
     const float ra_in = m2a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m2a - 2.f * ep;
     const float rk_in = kside ? n2 - fmaxf(ak, 0.f) - 2.f * ep : 0.f;
@@ -376,23 +410,7 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
     t.ra_out2 = ra_out > 0.f ? ra_out * ra_out : 0.f;
     t.rk_in2 = rk_in > 0.f ? rk_in * rk_in : 0.f;
     t.rk_out2 = rk_out > 0.f ? rk_out * rk_out : 0.f;
-    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + e.w);
-}
-
-// Tier when no edge lies below the cap: every margin is at least cap.
-__device__ __forceinline__ void tier_no_plane(const GeomDev& g, int a, float qx, float qy, float qz, float cap,
-                                              float epsa, Tier& t)
-{
-    tier_clear(t);
-    const float aa = g.cell_alpha[a];
-    t.h = cap;
-    t.cx = qx;
-    t.cy = qy;
-    t.cz = qz;
-    t.alpha_a = aa;
-    const float ra = cap - fmaxf(aa, 0.f) - 2.f * epsa;
-    t.ra_in2 = ra > 0.f ? ra * ra : 0.f;
-    t.eps = epsa + kUlp * cap;
+    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fabsf(e.w));
 }
 
 // Full classification of q from anchor a: greedy location, label, FP64
@@ -428,7 +446,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             return 1;
         }
         if (s.c < 0) {
-            tier_no_plane(g, a, qx, qy, qz, cap, s.eps, t);
+            tier_make(g, a, -1, qx, qy, qz, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         const bool kside = !inside;
@@ -459,61 +477,12 @@ struct TileOut {
 constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
 
 // LDS a tile walk needs.
+template <int NP>
 struct TileLds {
     double cols[kWaves * kColChunk * 2];
     int occ[kMaxLdsGrid];
+    float evc[kMaxLdsGrid][NP];  // per grid time: summed event signs per (delta, Delta) pair
 };
-
-// Records per scan chunk: all are fetched before any is used, so a chunk
-// costs one memory round trip.
-constexpr int kChunk = 8;
-
-// Running state of one facet scan (anchor cell `sc`, point s relative to it).
-struct ScanState {
-    int cur, end, c;
-    float m1, m2, pn, eps;
-    float sx, sy, sz;
-};
-
-__device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
-{
-    S.cur = g.nb_off[cell];
-    S.end = g.nb_off[cell + 1];
-    S.c = -1;
-    S.m1 = S.m2 = __builtin_inff();
-    S.sx = sx;
-    S.sy = sy;
-    S.sz = sz;
-    S.pn = __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
-    S.eps = kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]);
-}
-
-// One chunk of the scan (requires S.cur < S.end).  Margins of records past
-// the early-exit point are at least min(m2, cap), so evaluating them cannot
-// change m1, c or the capped m2: the chunk needs no per-record branch.
-// Records past the cell's end are masked.  The scan ends when the chunk's
-// last half length minus |s| reaches min(m2, cap) (edges are sorted by h).
-__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets)
-{
-    float4 r[kChunk];
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        const int k = S.cur + u;
-        r[u] = g.nb_rec[k < S.end ? k : S.end - 1];
-    }
-#pragma unroll
-    for (int u = 0; u < kChunk; ++u) {
-        float m = r[u].w - (S.sx * r[u].x + S.sy * r[u].y + S.sz * r[u].z);
-        m = (S.cur + u < S.end) ? m : __builtin_inff();
-        const bool lt = m < S.m1;
-        S.m2 = lt ? S.m1 : fminf(S.m2, m);
-        S.c = lt ? S.cur + u : S.c;
-        S.m1 = fminf(S.m1, m);
-    }
-    facets += (uint32_t)min(kChunk, S.end - S.cur);
-    S.cur += kChunk;
-    if (r[kChunk - 1].w - S.pn >= fminf(S.m2, cap)) S.cur = S.end;
-}
 
 // One tile of kBlock walkers [w_lo, w_lo + kBlock) of an entry: the whole
 // walk, then the phase columns into the tile's slot.  All threads of the
@@ -524,7 +493,7 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
 // need not be at the same step.  Per iteration a lane either takes its next
 // step (draws, proposal, plane tier) or, while its proposal is being
 // classified, advances the classification by up to R rounds of (one chunk
-// of kChunk facets, then the end-of-scan logic: greedy move, label, FP64
+// of kEdgePad facets, then the end-of-scan logic: greedy move, label, FP64
 // re-decision, start of the k-side scan).  A step completes when its label
 // is known.  A wave's iteration thus costs one step plus R chunks, instead
 // of the longest classification among its 64 lanes, and walkers with long
@@ -534,7 +503,7 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
 template <int NP, int R>
 __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
                                               int64_t w_lo, const PlanDev& plan, const Events& ev, const TileOut& out,
-                                              float cap_width, unsigned long long* counters, TileLds& L)
+                                              float cap_width, unsigned long long* counters, TileLds<NP>& L)
 {
     enum : int { kStep = 0, kScanA = 1, kScanK = 2, kDone = 3 };
     const int lane = threadIdx.x & 63;
@@ -543,9 +512,19 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
     long long iters = 0, facets = 0, rechecks = 0;
     const float cap = g.alpha_max + cap_width;
+    // grids up to kMaxLdsGrid keep occupancy counters and the event table in
+    // LDS; longer grids count in global memory and walk the event list
     const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
-    if (occ_in_lds)
-        for (int j = threadIdx.x; j < p.n_grid; j += kBlock) L.occ[j] = 0;
+    if (occ_in_lds) {
+        for (int j = threadIdx.x; j < p.n_grid; j += kBlock) {
+            L.occ[j] = 0;
+#pragma unroll
+            for (int u = 0; u < NP; ++u) L.evc[j][u] = 0.f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int e = 0; e < ev.n; ++e) L.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
+    }
     __syncthreads();
 
     Rng rng{0, 0};
@@ -601,7 +580,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     ScanState S;
     S.cur = S.end = 0;
     S.c = -1;
-    S.m1 = S.m2 = S.pn = S.eps = S.sx = S.sy = S.sz = 0.f;
+    S.m1 = S.m2 = S.pn = S.eps = S.nx = S.ny = S.nz = 0.f;
     int sa_c = -1;
     float sa_m2 = 0.f, sa_eps = 0.f;
 
@@ -645,6 +624,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             const bool scanning = phase == kScanA || phase == kScanK;
             if (scanning && S.cur < S.end) scan_chunk(g, cap, S, nfac);
             if (scanning && S.cur >= S.end) {
+                int next = 0;  // 1: greedy move to the nearer neighbour, 2: scan k's side
                 if (phase == kScanK) {
                     // k's smallest margin is at most the shared plane's
                     // (-m1 <= eps), so its second smallest bounds every
@@ -658,17 +638,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                         escaped = 1;
                         phase = kDone;
                     } else {
-                        // a neighbour is nearer: move the anchor and rescan
-                        const double ex = g.nb_e64[3 * S.c], ey = g.nb_e64[3 * S.c + 1], ez = g.nb_e64[3 * S.c + 2];
-                        qx = (float)((double)qx - ex);
-                        qy = (float)((double)qy - ey);
-                        qz = (float)((double)qz - ez);
-                        nox = (float)((double)nox + ex);
-                        noy = (float)((double)noy + ey);
-                        noz = (float)((double)noz + ez);
-                        na = g.nb_id[S.c];
-                        ++iters;
-                        scan_start(g, na, qx, qy, qz, S);
+                        next = 1;
                     }
                 } else {
                     const float a1 = g.cell_alpha[na];
@@ -697,13 +667,29 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                         complete = 1;
                         tmode = 4;
                     } else {
-                        const int k = g.nb_id[S.c];
-                        const float kx = (float)((double)qx - g.nb_e64[3 * S.c]);
-                        const float ky = (float)((double)qy - g.nb_e64[3 * S.c + 1]);
-                        const float kz = (float)((double)qz - g.nb_e64[3 * S.c + 2]);
-                        scan_start(g, k, kx, ky, kz, S);
-                        phase = kScanK;
+                        next = 2;
                     }
+                }
+                if (next) {
+                    // the point relative to neighbour k of the binding edge
+                    const int c = S.c;
+                    const double ex = g.nb_e64[3 * c], ey = g.nb_e64[3 * c + 1], ez = g.nb_e64[3 * c + 2];
+                    const int k = g.nb_id[c];
+                    const float kx = (float)((double)qx - ex);
+                    const float ky = (float)((double)qy - ey);
+                    const float kz = (float)((double)qz - ez);
+                    if (next == 1) {
+                        qx = kx;
+                        qy = ky;
+                        qz = kz;
+                        nox = (float)((double)nox + ex);
+                        noy = (float)((double)noy + ey);
+                        noz = (float)((double)noz + ez);
+                        na = k;
+                        ++iters;
+                    }
+                    scan_start(g, k, kx, ky, kz, S);
+                    phase = next == 1 ? kScanA : kScanK;
                 }
                 if (complete) phase = kStep;  // leaves the scan; completed below
             }
@@ -721,10 +707,12 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                 }
             }
             if (accept) {
-                if (tmode == 1) tier_make(g, na, sa_c, qx, qy, qz, sa_m2, sa_eps, true, fminf(S.m2, cap), S.eps, t);
-                else if (tmode == 4) tier_make(g, na, sa_c, qx, qy, qz, sa_m2, sa_eps, false, 0.f, 0.f, t);
-                else if (tmode == 2) tier_clear(t);
-                else if (tmode == 3) tier_no_plane(g, na, qx, qy, qz, cap, sa_eps, t);
+                if (tmode != 0) {
+                    // 1 two-sided, 4 a side only, 3 no plane below the cap, 2 none
+                    const float m2a = tmode == 2 ? -__builtin_inff() : (tmode == 3 ? cap : sa_m2);
+                    tier_make(g, na, tmode == 3 ? -1 : sa_c, qx, qy, qz, m2a, sa_eps, tmode == 1, fminf(S.m2, cap),
+                              S.eps, t);
+                }
                 a = na;
                 in = pin;
                 cell = pcell;
@@ -753,7 +741,19 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                     if (occ_in_lds) atomicAdd(&L.occ[j], 1);
                     else atomicAdd((unsigned long long*)&out.occupancy[j], 1ull);
                 }
-                if (next_ev < ev.n && ev.j[next_ev] == j) {
+                if (occ_in_lds) {
+                    // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
+                    const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
+                    const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
+                    const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+#pragma unroll
+                    for (int u = 0; u < NP; ++u) {
+                        const float sg = L.evc[j][u];
+                        dM[u][0] = fmaf(sg, Yx, dM[u][0]);
+                        dM[u][1] = fmaf(sg, Yy, dM[u][1]);
+                        dM[u][2] = fmaf(sg, Yz, dM[u][2]);
+                    }
+                } else if (next_ev < ev.n && ev.j[next_ev] == j) {
                     // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
                     const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
                     const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
@@ -811,7 +811,7 @@ template <int NP, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f32(GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow,
                                                     FusedOut fo, float cap_width, unsigned long long* counters)
 {
-    __shared__ TileLds lds;
+    __shared__ TileLds<NP> lds;
     TileOut out;
     out.slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
     out.occupancy = fo.occupancy;
@@ -830,7 +830,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
                                                      Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
                                                      unsigned long long* counters)
 {
-    __shared__ TileLds lds;
+    __shared__ TileLds<NP> lds;
     __shared__ long long s_tile;
     __shared__ int s_item;
     for (;;) {
