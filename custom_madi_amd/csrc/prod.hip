@@ -584,7 +584,9 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     int sa_c = -1;
     float sa_m2 = 0.f, sa_eps = 0.f;
 
+    uint32_t wave_iters = 0;
     while (__any(phase != kDone)) {
+        ++wave_iters;
         int complete = 0, tmode = 0;  // tier on accept: 0 keep, 1 two-sided, 2 clear, 3 no plane, 4 a side only
         if (phase == kStep) {
             const float dx = normal_scaled_f32(rng, nscale);
@@ -802,6 +804,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             atomicAdd(&counters[MADI_CNT_FACETS], (unsigned long long)v4);
             atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
             atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
+            atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
         }
     }
 }

@@ -47,7 +47,8 @@ extern "C" {
 #define MADI_CNT_KD_NODES 5      /* KD nodes visited (FP64 exact classifier) */
 #define MADI_CNT_CROSSINGS 6     /* proposals with m > 0 */
 #define MADI_CNT_REJECTS 7       /* rejected crossings */
-#define MADI_N_COUNTERS 8
+#define MADI_CNT_WAVE_ITERS 8    /* FP32 walk: state-machine iterations, summed over waves */
+#define MADI_N_COUNTERS 9
 
 /* Flags summed over walkers (int64 each). */
 #define MADI_FLAG_ESCAPED 0
