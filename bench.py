@@ -118,7 +118,8 @@ def run_c2(args):
     batch.run_plain(warm, spec)
 
     # batched / served form: all requests submitted at t0
-    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap, scan_rounds=args.scan_rounds,
+    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
+                                scan_rounds=args.scan_rounds, classify_mode=args.classify_mode,
                                 max_tiles_per_generation=args.gen_tiles)
     srv = serve.Server(sch, pool=pool)
     t0 = time.perf_counter()
@@ -189,7 +190,8 @@ def run_c3(args):
     log = lambda m: print(f"[bench c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
     warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
     batch.run_plain(warm, spec)
-    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap, scan_rounds=args.scan_rounds,
+    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
+                                scan_rounds=args.scan_rounds, classify_mode=args.classify_mode,
                                 max_tiles_per_generation=args.gen_tiles)
     stream = serve.LibraryStream(args.library_out)
     srv = serve.Server(sch, pool=pool, stream=stream)
@@ -226,7 +228,10 @@ def main(argv=None):
     ap.add_argument("--workers", type=int, default=16)
     ap.add_argument("--grid-per-cu", type=int, default=0)
     ap.add_argument("--tier-cap", type=float, default=0.0)
-    ap.add_argument("--scan-rounds", type=int, default=0, help="FP32 walk: classification rounds per iteration, 1-3 (0 default)")
+    ap.add_argument("--scan-rounds", type=int, default=0,
+                    help="FP32 walk: classification rounds per iteration, 1-2 (0 default)")
+    ap.add_argument("--classify-mode", type=int, default=0,
+                    help="FP32 walk: 1 per-lane state machine, 2 wave-cooperative lock step (0 default)")
     ap.add_argument("--gen-tiles", type=int, default=65536)
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
@@ -257,8 +262,8 @@ def main(argv=None):
     dg = ens.device_geometry()
     N = args.walkers
     runner = engine.FusedRunner(N, spec, cfg.n_grid)
-    params = engine.walk_params(N, cfg, pp, seed=0, tier_cap_um=args.tier_cap,
-                                scan_rounds=args.scan_rounds)
+    params = engine.walk_params(N, cfg, pp, seed=0, tier_cap_um=args.tier_cap, scan_rounds=args.scan_rounds,
+                                classify_mode=args.classify_mode)
     print(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}; workload c1 walkers={N} "
           f"steps={cfg.n_steps} pp={pp:.6g}", file=sys.stderr, flush=True)
 

@@ -59,7 +59,8 @@ class WalkParams(C.Structure):
     _fields_ = [
         ("n_walkers", C.c_int64), ("n_steps", C.c_int32), ("steps_per_h", C.c_int32), ("n_grid", C.c_int32),
         ("tier_cap_um", C.c_float), ("sigma", C.c_double), ("ts", C.c_double), ("pp", C.c_double),
-        ("seed", C.c_uint64), ("first_walker", C.c_uint64), ("scan_rounds", C.c_int32), ("reserved", C.c_int32),
+        ("seed", C.c_uint64), ("first_walker", C.c_uint64), ("scan_rounds", C.c_int32),
+        ("classify_mode", C.c_int32),
     ]
 
 
@@ -72,7 +73,7 @@ class Work(C.Structure):
 
 class Tune(C.Structure):
     _fields_ = [("grid_per_cu", C.c_int32), ("tier_cap_um", C.c_float), ("scan_rounds", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("classify_mode", C.c_int32)]
 
 
 TILE_WALKERS = 256

@@ -123,6 +123,7 @@ class DeviceScheduler:
     """Admission, generations and finalisation of entries on the current GPU."""
 
     def __init__(self, spec: engine.PhaseSpec, grid_per_cu: int = 0, tier_cap_um: float = 0.0, scan_rounds: int = 0,
+                 classify_mode: int = 0,
                  max_tiles_per_generation: int = 65536, hbm_reserve_bytes: int = 8 << 30,
                  max_inflight: int = 2):
         torch = _hip.require_gpu()
@@ -135,6 +136,7 @@ class DeviceScheduler:
         self.tune.grid_per_cu = int(grid_per_cu)
         self.tune.tier_cap_um = float(tier_cap_um)
         self.tune.scan_rounds = int(scan_rounds)
+        self.tune.classify_mode = int(classify_mode)
         self.max_tiles = int(max_tiles_per_generation)
         self.reserve = int(hbm_reserve_bytes)
         self.max_inflight = int(max_inflight)

@@ -357,8 +357,11 @@ static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != n
 static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
 
 static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
-// FP32 walk classification rounds per iteration (1-3); 0 picks the default.
-static int rounds_of(int r) { return r <= 0 ? 2 : (r > 3 ? 3 : r); }
+// FP32 walk classification rounds per iteration (1-2); 0 picks the default.
+static int rounds_of(int r) { return r <= 0 ? 2 : (r > 2 ? 2 : r); }
+// FP32 walk classification mode -> kernel variant (0 per-lane state machine,
+// 1 wave-cooperative lock step); 0 picks the default.
+static int variant_of(int mode) { return mode == 2 ? 1 : 0; }
 
 static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Events& ev)
 {
@@ -691,7 +694,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     fo.flags = (long long*)d_flags;
     if (precision == 32)
         rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds),
-                                   (unsigned long long*)d_counters, s);
+                                   variant_of(p->classify_mode), (unsigned long long*)d_counters, s);
     else
         rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
     if (rc) return rc;
@@ -807,8 +810,9 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     int64_t grid = (int64_t)cus * per_cu;
     if (grid > total) grid = total;
     const int mode = tune && tune->scan_rounds > 0 ? tune->scan_rounds : work[0].walk.scan_rounds;
-    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, rounds_of(mode), (int32_t)grid,
-                            (unsigned long long*)d_counters, s);
+    const int cm = tune && tune->classify_mode > 0 ? tune->classify_mode : work[0].walk.classify_mode;
+    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, rounds_of(mode), variant_of(cm),
+                            (int32_t)grid, (unsigned long long*)d_counters, s);
 }
 
 int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,

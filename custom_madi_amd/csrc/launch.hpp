@@ -68,9 +68,9 @@ int launch_reduce_f64(const double* Y, int64_t n, int32_t G, const int32_t* jd, 
 int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_t* cell, int8_t* inside, int8_t* ok,
                           unsigned long long* counters, hipStream_t s);
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
-                          float cap_width, int rounds, unsigned long long* counters, hipStream_t s);
+                          float cap_width, int rounds, int variant, unsigned long long* counters, hipStream_t s);
 int launch_batch_f32(const WorkDev* work, int32_t n_work, int64_t total, unsigned long long* queue, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int variant, int32_t grid,
                      unsigned long long* counters, hipStream_t s);
 int launch_finalize_tiles(const double* partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols, double* cos_out,
                           double* sin_out, hipStream_t s);

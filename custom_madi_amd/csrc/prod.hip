@@ -445,7 +445,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             tier_clear(t);
             return 1;
         }
-        if (s.c < 0) {
+        if (s.c < 0 || s.m1 >= cap) {
             tier_make(g, a, -1, qx, qy, qz, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
@@ -476,13 +476,117 @@ struct TileOut {
 
 constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
 
+// One slot of a wave's classification queue: a request (anchor, point,
+// offset) replaced in place by its answer (anchor, moved point and offset,
+// status, and the new tier; the tier's centre is the point).
+struct SvcSlot {
+    float4 v[5];
+};
+
 // LDS a tile walk needs.
 template <int NP>
 struct TileLds {
     double cols[kWaves * kColChunk * 2];
     int occ[kMaxLdsGrid];
     float evc[kMaxLdsGrid][NP];  // per grid time: summed event signs per (delta, Delta) pair
+    SvcSlot svc[kBlock];  // classification requests and answers (wave-cooperative variant)
 };
+
+// A proposal under classification: point relative to anchor na, the
+// anchor's offset, and the a-side results kept for the tier.
+struct Pending {
+    float qx, qy, qz, ox, oy, oz;
+    int na, pin, moves, sa_c;
+    float sa_m2, sa_eps;
+};
+
+constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classification
+
+__device__ __forceinline__ void pending_start(const GeomDev& g, Pending& P, ScanState& S, int& phase)
+{
+    P.moves = 0;
+    P.sa_c = -1;
+    P.sa_m2 = P.sa_eps = 0.f;
+    scan_start(g, P.na, P.qx, P.qy, P.qz, S);
+    phase = kScanA;
+}
+
+// Advances a pending classification by one group of edges and, when a scan
+// ends, by its end-of-scan step: greedy move to a nearer neighbour, the
+// label with FP64 re-decision near alpha, the start of k's side.  Returns 1
+// when the label is known (tmode: 1 two-sided tier, 2 none, 3 no plane
+// below the cap, 4 a side only), -1 on a classifier failure, 0 otherwise.
+__device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& phase, ScanState& S, Pending& P,
+                                              int& tmode, uint32_t& nfac, long long& iters, long long& facets,
+                                              long long& rechecks)
+{
+    if (S.cur < S.end) scan_chunk(g, cap, S, nfac);
+    if (S.cur < S.end) return 0;
+    int next;  // 1: greedy move to the nearer neighbour, 2: scan k's side
+    if (phase == kScanK) {
+        // k's smallest margin is at most the shared plane's (-m1 <= eps),
+        // so its second smallest bounds every other k plane when positive
+        tmode = 1;
+        return 1;
+    }
+    if (S.m1 < -S.eps) {
+        if (++P.moves > kMaxGraphIters) return -1;  // the greedy walk does not settle
+        next = 1;
+    } else {
+        const float a1 = g.cell_alpha[P.na];
+        P.pin = S.m1 >= a1;
+        P.sa_c = S.c;
+        P.sa_m2 = fminf(S.m2, cap);
+        P.sa_eps = S.eps;
+        if (fabsf(S.m1 - a1) <= S.eps) {
+            ++rechecks;
+            if (!locate_f64(g, P.na, P.qx, P.qy, P.qz, P.ox, P.oy, P.oz, P.pin, facets)) return -1;
+            tmode = 2;
+            return 1;
+        }
+        if (S.c < 0 || S.m1 >= cap) {
+            // every margin is at least cap (which edge attains the smallest
+            // is then immaterial)
+            tmode = 3;
+            return 1;
+        }
+        if (P.pin) {
+            // an inside walker reaches k's side only across the whole gap
+            tmode = 4;
+            return 1;
+        }
+        next = 2;
+    }
+    // the point relative to neighbour k of the binding edge
+    const int c = S.c;
+    const double ex = g.nb_e64[3 * c], ey = g.nb_e64[3 * c + 1], ez = g.nb_e64[3 * c + 2];
+    const int k = g.nb_id[c];
+    const float kx = (float)((double)P.qx - ex);
+    const float ky = (float)((double)P.qy - ey);
+    const float kz = (float)((double)P.qz - ez);
+    if (next == 1) {
+        P.qx = kx;
+        P.qy = ky;
+        P.qz = kz;
+        P.ox = (float)((double)P.ox + ex);
+        P.oy = (float)((double)P.oy + ey);
+        P.oz = (float)((double)P.oz + ez);
+        P.na = k;
+        ++iters;
+    }
+    scan_start(g, k, kx, ky, kz, S);
+    phase = next == 1 ? kScanA : kScanK;
+    return 0;
+}
+
+// The tier a completed classification leaves (see tier_make).
+__device__ __forceinline__ void tier_of(const GeomDev& g, const Pending& P, int tmode, const ScanState& S, float cap,
+                                        Tier& t)
+{
+    const float m2a = tmode == 2 ? -__builtin_inff() : (tmode == 3 ? cap : P.sa_m2);
+    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, P.qx, P.qy, P.qz, m2a, P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps,
+              t);
+}
 
 // One tile of kBlock walkers [w_lo, w_lo + kBlock) of an entry: the whole
 // walk, then the phase columns into the tile's slot.  All threads of the
@@ -492,20 +596,20 @@ struct TileLds {
 // Each lane runs its walker as a small state machine, so lanes of a wave
 // need not be at the same step.  Per iteration a lane either takes its next
 // step (draws, proposal, plane tier) or, while its proposal is being
-// classified, advances the classification by up to R rounds of (one chunk
-// of kEdgePad facets, then the end-of-scan logic: greedy move, label, FP64
-// re-decision, start of the k-side scan).  A step completes when its label
-// is known.  A wave's iteration thus costs one step plus R chunks, instead
-// of the longest classification among its 64 lanes, and walkers with long
+// classified, advances the classification by up to R rounds of (one group of
+// kEdgePad edges, then the end-of-scan step: greedy move, label, FP64
+// re-decision, start of k's side).  A step completes when its label is
+// known.  A wave's iteration thus costs one step plus R groups instead of the
+// longest classification among its 64 lanes, and walkers with long
 // classifications fall behind by a few iterations rather than stalling the
-// wave.  Every walker consumes its own random stream in order, so results
-// do not depend on R.
+// wave.  Every walker consumes its own random stream in order, so results do
+// not depend on R.
 template <int NP, int R>
 __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
                                               int64_t w_lo, const PlanDev& plan, const Events& ev, const TileOut& out,
                                               float cap_width, unsigned long long* counters, TileLds<NP>& L)
 {
-    enum : int { kStep = 0, kScanA = 1, kScanK = 2, kDone = 3 };
+    enum : int { kStep = 0, kDone = 3 };
     const int lane = threadIdx.x & 63;
     const int64_t w = w_lo + threadIdx.x;
     const bool live = w < p.n_walkers;
@@ -573,35 +677,35 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     const float ts = (float)p.ts;
     int phase = (live && !escaped && p.n_steps > 0) ? kStep : kDone;
     int step = 0, until_sample = p.steps_per_h, j_sample = 0, next_ev = 0;
-    // pending step: proposal relative to its anchor, offset, acceptance bits
-    float qx = 0.f, qy = 0.f, qz = 0.f, nox = 0.f, noy = 0.f, noz = 0.f;
-    int na = 0, rj = 0, pin = 0, pcell = 0, moves = 0;
-    // scan state and the a-side results kept for the tier
+    // the pending step: proposal, label once known, acceptance bits
+    Pending P;
+    P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
+    P.na = P.pin = P.moves = 0;
+    P.sa_c = -1;
+    P.sa_m2 = P.sa_eps = 0.f;
+    int pcell = 0, rj = 0;
     ScanState S;
     S.cur = S.end = 0;
     S.c = -1;
     S.m1 = S.m2 = S.pn = S.eps = S.nx = S.ny = S.nz = 0.f;
-    int sa_c = -1;
-    float sa_m2 = 0.f, sa_eps = 0.f;
-
     uint32_t wave_iters = 0;
     while (__any(phase != kDone)) {
         ++wave_iters;
-        int complete = 0, tmode = 0;  // tier on accept: 0 keep, 1 two-sided, 2 clear, 3 no plane, 4 a side only
+        int complete = 0, tmode = 0;  // tmode 0: keep the tier; 1-4 see classify_round
         if (phase == kStep) {
             const float dx = normal_scaled_f32(rng, nscale);
             const float dy = normal_scaled_f32(rng, nscale);
             const float dz = normal_scaled_f32(rng, nscale);
             const uint64_t ku = rng_next(rng) >> 11;
             rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
-            qx = px + dx;
-            qy = py + dy;
-            qz = pz + dz;
-            nox = ox;
-            noy = oy;
-            noz = oz;
-            na = a;
-            const float nDx = ox + qx, nDy = oy + qy, nDz = oz + qz;
+            P.qx = px + dx;
+            P.qy = py + dy;
+            P.qz = pz + dz;
+            P.ox = ox;
+            P.oy = oy;
+            P.oz = oz;
+            P.na = a;
+            const float nDx = ox + P.qx, nDy = oy + P.qy, nDz = oz + P.qz;
             if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
                 escaped = 1;
                 phase = kDone;
@@ -611,95 +715,33 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                         if (occ_in_lds) atomicAdd(&L.occ[j], 1);
                         else atomicAdd((unsigned long long*)&out.occupancy[j], 1ull);
                     }
-            } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
+            } else if (tier_query(t, a, P.qx, P.qy, P.qz, pcell, P.pin)) {
                 ++n_tier;
                 complete = 1;
             } else {
                 ++n_class;
-                moves = 0;
-                scan_start(g, na, qx, qy, qz, S);
-                phase = kScanA;
+                pending_start(g, P, S, phase);
             }
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-            const bool scanning = phase == kScanA || phase == kScanK;
-            if (scanning && S.cur < S.end) scan_chunk(g, cap, S, nfac);
-            if (scanning && S.cur >= S.end) {
-                int next = 0;  // 1: greedy move to the nearer neighbour, 2: scan k's side
-                if (phase == kScanK) {
-                    // k's smallest margin is at most the shared plane's
-                    // (-m1 <= eps), so its second smallest bounds every
-                    // other k plane whenever it is positive
+            if (phase == kScanA || phase == kScanK) {
+                const int res = classify_round(g, cap, phase, S, P, tmode, nfac, iters, facets, rechecks);
+                if (res > 0) {
                     complete = 1;
-                    tmode = 1;
-                } else if (S.m1 < -S.eps) {
-                    if (++moves > kMaxGraphIters) {
-                        // the greedy walk does not settle: flag the walker
-                        err = 1;
-                        escaped = 1;
-                        phase = kDone;
-                    } else {
-                        next = 1;
-                    }
-                } else {
-                    const float a1 = g.cell_alpha[na];
-                    pin = S.m1 >= a1;
-                    pcell = na;
-                    sa_c = S.c;
-                    sa_m2 = fminf(S.m2, cap);
-                    sa_eps = S.eps;
-                    if (fabsf(S.m1 - a1) <= S.eps) {
-                        ++rechecks;
-                        if (!locate_f64(g, na, qx, qy, qz, nox, noy, noz, pin, facets)) {
-                            err = 1;
-                            escaped = 1;
-                            phase = kDone;
-                        } else {
-                            pcell = na;
-                            complete = 1;
-                            tmode = 2;
-                        }
-                    } else if (S.c < 0) {
-                        complete = 1;
-                        tmode = 3;
-                    } else if (pin) {
-                        // an inside walker reaches k's side only across the
-                        // whole gap: no k-side scan
-                        complete = 1;
-                        tmode = 4;
-                    } else {
-                        next = 2;
-                    }
+                    pcell = P.na;
+                    phase = kStep;  // completed below
+                } else if (res < 0) {
+                    err = 1;
+                    escaped = 1;
+                    phase = kDone;
                 }
-                if (next) {
-                    // the point relative to neighbour k of the binding edge
-                    const int c = S.c;
-                    const double ex = g.nb_e64[3 * c], ey = g.nb_e64[3 * c + 1], ez = g.nb_e64[3 * c + 2];
-                    const int k = g.nb_id[c];
-                    const float kx = (float)((double)qx - ex);
-                    const float ky = (float)((double)qy - ey);
-                    const float kz = (float)((double)qz - ez);
-                    if (next == 1) {
-                        qx = kx;
-                        qy = ky;
-                        qz = kz;
-                        nox = (float)((double)nox + ex);
-                        noy = (float)((double)noy + ey);
-                        noz = (float)((double)noz + ez);
-                        na = k;
-                        ++iters;
-                    }
-                    scan_start(g, k, kx, ky, kz, S);
-                    phase = next == 1 ? kScanA : kScanK;
-                }
-                if (complete) phase = kStep;  // leaves the scan; completed below
             }
         }
         if (complete) {
             int m = 0;
-            if (in != pin) m = 1;
-            else if (in && pin && cell != pcell) m = 2;
+            if (in != P.pin) m = 1;
+            else if (in && P.pin && cell != pcell) m = 2;
             bool accept = true;
             if (m > 0) {
                 ++crossings;
@@ -709,21 +751,16 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                 }
             }
             if (accept) {
-                if (tmode != 0) {
-                    // 1 two-sided, 4 a side only, 3 no plane below the cap, 2 none
-                    const float m2a = tmode == 2 ? -__builtin_inff() : (tmode == 3 ? cap : sa_m2);
-                    tier_make(g, na, tmode == 3 ? -1 : sa_c, qx, qy, qz, m2a, sa_eps, tmode == 1, fminf(S.m2, cap),
-                              S.eps, t);
-                }
-                a = na;
-                in = pin;
+                if (tmode != 0) tier_of(g, P, tmode, S, cap, t);
+                a = P.na;
+                in = P.pin;
                 cell = pcell;
-                px = qx;
-                py = qy;
-                pz = qz;
-                ox = nox;
-                oy = noy;
-                oz = noz;
+                px = P.qx;
+                py = P.qy;
+                pz = P.qz;
+                ox = P.ox;
+                oy = P.oy;
+                oz = P.oz;
             }
             const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
             const float yx = Dx - Cx, tx = Sx + yx;
@@ -756,7 +793,6 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                         dM[u][2] = fmaf(sg, Yz, dM[u][2]);
                     }
                 } else if (next_ev < ev.n && ev.j[next_ev] == j) {
-                    // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
                     const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
                     const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
                     const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
@@ -809,29 +845,407 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     }
 }
 
+// Scan of `cell` at point s by a group of G lanes (G a power of two, the
+// group an aligned block of G lanes, j the lane's index in it): lane j takes
+// edges j, j + G, ... of the padded list, then (m1, c, m2) are merged across
+// the group (ties to the lower edge index).  Every lane of the group ends
+// with the same result, equal to scan_cell's m1, c and min(m2, cap) (the
+// sequential scan's early exit skips only edges whose margins are at least
+// min(m2, cap); edges are visited in index order in both).
+__device__ __forceinline__ Scan group_scan(const GeomDev& g, int cell, float sx, float sy, float sz, int G, int j,
+                                           uint32_t& facets)
+{
+    const int lo = g.nb_off[cell], hi = g.nb_off[cell + 1];
+    const float nx = -sx, ny = -sy, nz = -sz;
+    float m1 = __builtin_inff(), m2 = __builtin_inff();
+    int c = -1;
+    for (int k = lo + j; k < hi; k += G) {
+        const float4 r = g.nb_rec[k];
+        const float m = fmaf(nx, r.x, fmaf(ny, r.y, fmaf(nz, r.z, r.w)));
+        const bool lt = m < m1;
+        m2 = fminf(m2, fmaxf(m1, m));
+        c = lt ? k : c;
+        m1 = fminf(m1, m);
+    }
+    facets += (uint32_t)((hi - lo) / G);
+    for (int off = G >> 1; off >= 1; off >>= 1) {
+        const float pm1 = __shfl_xor(m1, off, 64), pm2 = __shfl_xor(m2, off, 64);
+        const int pc = __shfl_xor(c, off, 64);
+        const bool take = pm1 < m1 || (pm1 == m1 && (uint32_t)pc < (uint32_t)c);
+        m2 = fminf(fmaxf(m1, pm1), fminf(m2, pm2));
+        c = take ? pc : c;
+        m1 = fminf(m1, pm1);
+    }
+    return Scan{m1, m2, kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]), c};
+}
+
+// Full classification of one request by its group (see classify_round for
+// the steps): greedy location, label with FP64 re-decision, k side for an
+// outside point, and the new tier.  All lanes of the group compute the same
+// values.  Returns 0 on a classifier failure.
+__device__ int classify_group(const GeomDev& g, float cap, int G, int j, int& na, float& qx, float& qy, float& qz,
+                              float& ox, float& oy, float& oz, int& pin, Tier& t, uint32_t& facets, long long& iters,
+                              long long& rechecks, long long& facets64)
+{
+    for (int moves = 0; moves <= kMaxGraphIters; ++moves) {
+        const Scan s = group_scan(g, na, qx, qy, qz, G, j, facets);
+        if (s.m1 < -s.eps) {
+            const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
+            qx = (float)((double)qx - ex);
+            qy = (float)((double)qy - ey);
+            qz = (float)((double)qz - ez);
+            ox = (float)((double)ox + ex);
+            oy = (float)((double)oy + ey);
+            oz = (float)((double)oz + ez);
+            na = g.nb_id[s.c];
+            if (j == 0) ++iters;
+            continue;
+        }
+        const float a1 = g.cell_alpha[na];
+        pin = s.m1 >= a1;
+        if (fabsf(s.m1 - a1) <= s.eps) {
+            if (j == 0) ++rechecks;
+            if (!locate_f64(g, na, qx, qy, qz, ox, oy, oz, pin, facets64)) return 0;
+            tier_clear(t);
+            return 1;
+        }
+        if (s.c < 0 || s.m1 >= cap) {
+            tier_make(g, na, -1, qx, qy, qz, cap, s.eps, false, 0.f, 0.f, t);
+            return 1;
+        }
+        if (pin) {
+            tier_make(g, na, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, false, 0.f, 0.f, t);
+            return 1;
+        }
+        const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
+        const Scan sk = group_scan(g, g.nb_id[s.c], (float)((double)qx - ex), (float)((double)qy - ey),
+                                   (float)((double)qz - ez), G, j, facets);
+        tier_make(g, na, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, true, fminf(sk.m2, cap), sk.eps, t);
+        return 1;
+    }
+    return 0;  // the greedy walk does not settle
+}
+
+// Orders LDS accesses of different lanes of one wave (a wave's LDS
+// instructions complete in order; this keeps the compiler from moving them).
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// One tile, lock-step variant: the block's walkers take every step together.
+// The proposals of a wave that the plane tier cannot answer (n of them) are
+// classified cooperatively by the wave: request r goes to the aligned group
+// of G = 8, 4, 2 or 1 lanes (the largest with 64 / G >= n), which scans each
+// cell's edges G at a time and merges the results with shuffles.  A step
+// then costs the step code plus about (edges / G) per classification instead
+// of the longest classification of the wave, and samples are uniform.
+// Results equal the state-machine variant's bit for bit.
+template <int NP>
+__device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
+                                               int64_t w_lo, const PlanDev& plan, const Events& ev,
+                                               const TileOut& out, float cap_width, unsigned long long* counters,
+                                               TileLds<NP>& L)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t w = w_lo + threadIdx.x;
+    const bool live = w < p.n_walkers;
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
+    long long iters = 0, facets = 0, rechecks = 0;
+    const float cap = g.alpha_max + cap_width;
+    const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
+    if (occ_in_lds) {
+        for (int j = threadIdx.x; j < p.n_grid; j += kBlock) {
+            L.occ[j] = 0;
+#pragma unroll
+            for (int u = 0; u < NP; ++u) L.evc[j][u] = 0.f;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int e = 0; e < ev.n; ++e) L.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
+    }
+    __syncthreads();
+
+    Rng rng{0, 0};
+    double x0 = 0.0, y0 = 0.0, z0 = 0.0;
+    if (live) {
+        rng = rng_for_walker(p.seed, p.first_walker + (uint64_t)w, jump_pow);
+        const double span = g.source_hi - g.source_lo;
+        x0 = g.source_lo + uniform_f64(rng) * span;
+        y0 = g.source_lo + uniform_f64(rng) * span;
+        z0 = g.source_lo + uniform_f64(rng) * span;
+    }
+    int a = 0, in = 0, cell = 0, err = 0, escaped = 0;
+    float px = 0.f, py = 0.f, pz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
+    Tier t;
+    tier_clear(t);
+    if (live && !g.free_water) {
+        anchor_world(g, x0, y0, z0, a, px, py, pz);
+        ox = -px;
+        oy = -py;
+        oz = -pz;
+        ++n_class;
+        if (!classify_full(g, a, px, py, pz, ox, oy, oz, cap, in, t, iters, facets, rechecks)) {
+            err = 1;
+            escaped = 1;
+        }
+        cell = a;
+        ox = -px;
+        oy = -py;
+        oz = -pz;
+    } else if (live) {
+        tier_free_water(t);
+    }
+    const float lox = (float)(-x0), loy = (float)(-y0), loz = (float)(-z0);
+    const float hix = (float)(g.L - x0), hiy = (float)(g.L - y0), hiz = (float)(g.L - z0);
+
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+    float dM[NP][3];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
+    {
+        const int c0 = (int)__popcll(__ballot(live && !err && in));
+        if (lane == 0 && c0) {
+            if (occ_in_lds) atomicAdd(&L.occ[0], c0);
+            else atomicAdd((unsigned long long*)&out.occupancy[0], (unsigned long long)c0);
+        }
+    }
+
+    const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
+    const float ts = (float)p.ts;
+    SvcSlot* const slots = L.svc + wave * 64;
+    int until_sample = p.steps_per_h, j_sample = 0, next_ev = 0;
+    for (int32_t step = 0; step < p.n_steps; ++step) {
+        const bool act = live && !escaped;
+        int need = 0, pin = in, pcell = cell, na = a, rj = 0;
+        float qx = px, qy = py, qz = pz, nox = ox, noy = oy, noz = oz;
+        if (act) {
+            const float dx = normal_scaled_f32(rng, nscale);
+            const float dy = normal_scaled_f32(rng, nscale);
+            const float dz = normal_scaled_f32(rng, nscale);
+            const uint64_t ku = rng_next(rng) >> 11;
+            rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
+            qx = px + dx;
+            qy = py + dy;
+            qz = pz + dz;
+            const float nDx = ox + qx, nDy = oy + qy, nDz = oz + qz;
+            if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
+                escaped = 1;  // the walker stops; later samples keep its last label
+            } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
+                ++n_tier;
+            } else {
+                need = 1;
+            }
+        }
+        const uint64_t mask = __ballot(need);
+        Tier nt;
+        if (mask) {
+            const int n = (int)__popcll(mask);
+            const int lg = n <= 8 ? 3 : (n <= 16 ? 2 : (n <= 32 ? 1 : 0));
+            const int G = 1 << lg;
+            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+            if (need) {
+                ++n_class;
+                slots[rank].v[0] = make_float4(__int_as_float(a), qx, qy, qz);
+                slots[rank].v[1] = make_float4(ox, oy, oz, 0.f);
+            }
+            wave_lds_fence();
+            const int r = lane >> lg, j = lane & (G - 1);
+            if (r < n) {
+                const float4 r0 = slots[r].v[0], r1 = slots[r].v[1];
+                int sa = __float_as_int(r0.x), spin = 0;
+                float sx = r0.y, sy = r0.z, sz = r0.w, sox = r1.x, soy = r1.y, soz = r1.z;
+                Tier st;
+                const int ok = classify_group(g, cap, G, j, sa, sx, sy, sz, sox, soy, soz, spin, st, nfac, iters,
+                                              rechecks, facets);
+                wave_lds_fence();
+                if (j == 0) {
+                    slots[r].v[0] = make_float4(__int_as_float(sa), sx, sy, sz);
+                    slots[r].v[1] = make_float4(sox, soy, soz, __int_as_float(ok | (spin << 1)));
+                    slots[r].v[2] = make_float4(st.ux, st.uy, st.uz, st.h);
+                    slots[r].v[3] = make_float4(st.ra_in2, st.ra_out2, st.rk_in2, st.rk_out2);
+                    slots[r].v[4] = make_float4(st.alpha_a, st.alpha_k, st.eps, __int_as_float(st.k));
+                }
+            }
+            wave_lds_fence();
+            if (need) {
+                const SvcSlot& s = slots[rank];
+                const float4 r0 = s.v[0], r1 = s.v[1], r2 = s.v[2], r3 = s.v[3], r4 = s.v[4];
+                const int st = __float_as_int(r1.w);
+                if (!(st & 1)) {
+                    err = 1;
+                    escaped = 1;
+                } else {
+                    na = __float_as_int(r0.x);
+                    qx = r0.y;
+                    qy = r0.z;
+                    qz = r0.w;
+                    nox = r1.x;
+                    noy = r1.y;
+                    noz = r1.z;
+                    pin = (st >> 1) & 1;
+                    pcell = na;
+                    nt.ux = r2.x;
+                    nt.uy = r2.y;
+                    nt.uz = r2.z;
+                    nt.h = r2.w;
+                    nt.cx = qx;
+                    nt.cy = qy;
+                    nt.cz = qz;
+                    nt.ra_in2 = r3.x;
+                    nt.ra_out2 = r3.y;
+                    nt.rk_in2 = r3.z;
+                    nt.rk_out2 = r3.w;
+                    nt.alpha_a = r4.x;
+                    nt.alpha_k = r4.y;
+                    nt.eps = r4.z;
+                    nt.k = __float_as_int(r4.w);
+                }
+            }
+            wave_lds_fence();
+        }
+        if (act && !escaped) {
+            int m = 0;
+            if (in != pin) m = 1;
+            else if (in && pin && cell != pcell) m = 2;
+            bool accept = true;
+            if (m > 0) {
+                ++crossings;
+                if ((rj >> (m - 1)) & 1) {
+                    accept = false;
+                    ++rejects;
+                }
+            }
+            if (accept) {
+                if (need) t = nt;
+                a = na;
+                in = pin;
+                cell = pcell;
+                px = qx;
+                py = qy;
+                pz = qz;
+                ox = nox;
+                oy = noy;
+                oz = noz;
+            }
+            const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
+            const float yx = Dx - Cx, tx = Sx + yx;
+            Cx = (tx - Sx) - yx;
+            Sx = tx;
+            const float yy = Dy - Cy, ty = Sy + yy;
+            Cy = (ty - Sy) - yy;
+            Sy = ty;
+            const float yz = Dz - Cz, tz = Sz + yz;
+            Cz = (tz - Sz) - yz;
+            Sz = tz;
+        }
+        if (--until_sample == 0) {
+            until_sample = p.steps_per_h;
+            const int j = ++j_sample;
+            const int cj = (int)__popcll(__ballot(live && in));
+            if (lane == 0 && cj) {
+                if (occ_in_lds) atomicAdd(&L.occ[j], cj);
+                else atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)cj);
+            }
+            const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
+            // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
+            const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
+            const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
+            const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+            if (occ_in_lds) {
+#pragma unroll
+                for (int u = 0; u < NP; ++u) {
+                    const float sg = L.evc[j][u];
+                    dM[u][0] = fmaf(sg, Yx, dM[u][0]);
+                    dM[u][1] = fmaf(sg, Yy, dM[u][1]);
+                    dM[u][2] = fmaf(sg, Yz, dM[u][2]);
+                }
+            } else {
+                while (next_ev < ev.n && ev.j[next_ev] == j) {
+                    const int q = ev.pair[next_ev];
+                    const float sg = (float)ev.sign[next_ev];
+#pragma unroll
+                    for (int u = 0; u < NP; ++u)
+                        if (u == q) {
+                            dM[u][0] += sg * Yx;
+                            dM[u][1] += sg * Yy;
+                            dM[u][2] += sg * Yz;
+                        }
+                    ++next_ev;
+                }
+            }
+        }
+    }
+    facets += nfac;
+    {
+        const long long ne = wave_popc(live && escaped);
+        const long long nr = wave_popc(live && err);
+        if (lane == 0) {
+            if (ne) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
+            if (nr) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
+        }
+    }
+    __syncthreads();
+    if (occ_in_lds)
+        for (int j = threadIdx.x; j < p.n_grid; j += kBlock)
+            if (L.occ[j]) atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)L.occ[j]);
+    emit_columns<float, NP>(plan, dM, live && !escaped, L.cols, out.slot);
+    if (counters) {
+        const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
+                     v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
+                     v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
+                     v6 = wave_sum_f64((double)rejects);
+        if (lane == 0) {
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)v2);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)v3);
+            atomicAdd(&counters[MADI_CNT_FACETS], (unsigned long long)v4);
+            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
+            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
+            atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)p.n_steps);
+        }
+    }
+}
+
 // One entry: block b runs tile b.
-template <int NP, int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f32(GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow,
-                                                    FusedOut fo, float cap_width, unsigned long long* counters)
+// Classification variants (M): 0 per-lane state machine with R rounds per
+// iteration, 1 wave-cooperative lock step.
+template <int NP, int R, int M>
+__device__ __forceinline__ void walk_tile(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
+                                          int64_t w_lo, const PlanDev& plan, const Events& ev, const TileOut& out,
+                                          float cap_width, unsigned long long* counters, TileLds<NP>& L)
+{
+    if constexpr (M == 1)
+        walk_tile_coop<NP>(g, p, jump_pow, w_lo, plan, ev, out, cap_width, counters, L);
+    else
+        walk_tile_f32<NP, R>(g, p, jump_pow, w_lo, plan, ev, out, cap_width, counters, L);
+}
+
+template <int NP, int R, int M>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f32(
+    GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow, FusedOut fo, float cap_width,
+    unsigned long long* counters)
 {
     __shared__ TileLds<NP> lds;
     TileOut out;
     out.slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
     out.occupancy = fo.occupancy;
     out.flags = fo.flags;
-    walk_tile_f32<NP, R>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters,
-                             lds);
+    walk_tile<NP, R, M>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters, lds);
 }
 
 // Persistent work-stealing launch over many entries: each workgroup takes
 // the next global tile index from an atomic counter, finds its work item by
 // binary search on work_begin, and runs that tile.  Every workgroup exits
 // once the counter passes the total, so the grid drains.
-template <int NP, int R>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_f32(const WorkDev* __restrict__ work, int32_t n_work, int64_t total,
-                                                     unsigned long long* queue, const PlanDev* __restrict__ plan,
-                                                     Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
-                                                     unsigned long long* counters)
+template <int NP, int R, int M>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_f32(
+    const WorkDev* __restrict__ work, int32_t n_work, int64_t total, unsigned long long* queue,
+    const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
+    unsigned long long* counters)
 {
     __shared__ TileLds<NP> lds;
     __shared__ long long s_tile;
@@ -861,7 +1275,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         out.slot = W.partial + (size_t)tile * plan->n_cols * 2;
         out.occupancy = W.occupancy;
         out.flags = W.flags;
-        walk_tile_f32<NP, R>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds);
+        walk_tile<NP, R, M>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds);
     }
 }
 
@@ -892,42 +1306,46 @@ int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_
     return check_launch("k_classify_graph");
 }
 
-#define MADI_DISPATCH_R(NPV, R_VAL, LAUNCH) \
-    do {                                     \
-        if ((R_VAL) <= 1) LAUNCH(NPV, 1);    \
-        else if ((R_VAL) == 2) LAUNCH(NPV, 2); \
-        else LAUNCH(NPV, 3);                 \
+#define MADI_DISPATCH_C(NPV, RV, M_VAL, LAUNCH) \
+    do {                                        \
+        if ((M_VAL) == 1) LAUNCH(NPV, 1, 1);    \
+        else LAUNCH(NPV, RV, 0);                \
     } while (0)
-#define MADI_DISPATCH_NP(NP_VAL, R_VAL, LAUNCH)                      \
-    do {                                                             \
-        const int np_ = (NP_VAL);                                    \
-        if (np_ <= 2) MADI_DISPATCH_R(2, R_VAL, LAUNCH);             \
-        else if (np_ <= 4) MADI_DISPATCH_R(4, R_VAL, LAUNCH);        \
-        else MADI_DISPATCH_R(MADI_MAX_PAIRS, R_VAL, LAUNCH);         \
+#define MADI_DISPATCH_R(NPV, R_VAL, C_VAL, LAUNCH)              \
+    do {                                                        \
+        if ((R_VAL) <= 1) MADI_DISPATCH_C(NPV, 1, C_VAL, LAUNCH); \
+        else MADI_DISPATCH_C(NPV, 2, C_VAL, LAUNCH);            \
+    } while (0)
+#define MADI_DISPATCH_NP(NP_VAL, R_VAL, C_VAL, LAUNCH)                      \
+    do {                                                                    \
+        const int np_ = (NP_VAL);                                           \
+        if (np_ <= 2) MADI_DISPATCH_R(2, R_VAL, C_VAL, LAUNCH);             \
+        else if (np_ <= 4) MADI_DISPATCH_R(4, R_VAL, C_VAL, LAUNCH);        \
+        else MADI_DISPATCH_R(MADI_MAX_PAIRS, R_VAL, C_VAL, LAUNCH);         \
     } while (0)
 
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
-                          float cap_width, int rounds, unsigned long long* counters, hipStream_t s)
+                          float cap_width, int rounds, int variant, unsigned long long* counters, hipStream_t s)
 {
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
-#define MADI_LAUNCH_WALK(NPV, RV)                                                                                   \
-    hipLaunchKernelGGL((k_walk_f32<NPV, RV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, fo,      \
-                       cap_width, counters)
-    MADI_DISPATCH_NP(fo.ev.n_pairs, rounds, MADI_LAUNCH_WALK);
+#define MADI_LAUNCH_WALK(NPV, RV, CV)                                                                               \
+    hipLaunchKernelGGL((k_walk_f32<NPV, RV, CV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, \
+                       jump_pow, fo, cap_width, counters)
+    MADI_DISPATCH_NP(fo.ev.n_pairs, rounds, variant, MADI_LAUNCH_WALK);
 #undef MADI_LAUNCH_WALK
     return check_launch("k_walk_f32");
 }
 
 int launch_batch_f32(const WorkDev* work, int32_t n_work, int64_t total, unsigned long long* queue, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int variant, int32_t grid,
                      unsigned long long* counters, hipStream_t s)
 {
     if (total == 0) return MADI_OK;
-#define MADI_LAUNCH_BATCH(NPV, RV)                                                                                  \
-    hipLaunchKernelGGL((k_batch_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, n_work, total, queue, plan,  \
-                       ev, jump_pow, cap_width, counters)
-    MADI_DISPATCH_NP(ev.n_pairs, rounds, MADI_LAUNCH_BATCH);
+#define MADI_LAUNCH_BATCH(NPV, RV, CV)                                                                              \
+    hipLaunchKernelGGL((k_batch_f32<NPV, RV, CV>), dim3(grid), dim3(kBlock), 0, s, work, n_work,    \
+                       total, queue, plan, ev, jump_pow, cap_width, counters)
+    MADI_DISPATCH_NP(ev.n_pairs, rounds, variant, MADI_LAUNCH_BATCH);
 #undef MADI_LAUNCH_BATCH
     return check_launch("k_batch_f32");
 }

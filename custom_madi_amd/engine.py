@@ -105,10 +105,11 @@ def spec_from_columns(j_delta, j_Delta, j_sum, pcs, dirs=AXIS_DIRECTIONS):
 
 
 def walk_params(n_walkers: int, cfg: SimConfig, pp: float, seed: int, first_walker: int = 0,
-                tier_cap_um: float = 0.0, scan_rounds: int = 0) -> _hip.WalkParams:
+                tier_cap_um: float = 0.0, scan_rounds: int = 0, classify_mode: int = 0) -> _hip.WalkParams:
     p = _hip.WalkParams()
     p.tier_cap_um = float(tier_cap_um)
     p.scan_rounds = int(scan_rounds)
+    p.classify_mode = int(classify_mode)
     p.n_walkers = int(n_walkers)
     p.n_steps = int(cfg.n_steps)
     p.steps_per_h = int(cfg.steps_per_h)

@@ -159,8 +159,8 @@ typedef struct {
     double pp;
     uint64_t seed;          /* walk seed (device RNG) */
     uint64_t first_walker;  /* RNG subsequence index of walker 0 */
-    int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-3 (0 = 2) */
-    int32_t reserved;
+    int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-2 (0 = 2) */
+    int32_t classify_mode;  /* FP32 walk: 0 default, 1 per-lane state machine, 2 wave-cooperative lock step */
 } madi_walk_params;
 
 /* Reference-semantics FP64 walk with the exact classifier at every endpoint.
@@ -237,8 +237,8 @@ typedef struct {
 typedef struct {
     int32_t grid_per_cu;  /* persistent workgroups per CU (0 = 4) */
     float tier_cap_um;    /* plane-tier cap (0 = first item's walk.tier_cap_um) */
-    int32_t scan_rounds;  /* 1-3; 0 = first item's walk.scan_rounds */
-    int32_t reserved;
+    int32_t scan_rounds;  /* 1-2; 0 = first item's walk.scan_rounds */
+    int32_t classify_mode;  /* as madi_walk_params.classify_mode; 0 = first item's */
 } madi_tune;
 
 /* Device workspace bytes of a batched launch of n_work items. */

@@ -155,3 +155,31 @@ def test_dropin_fp64_fused_equals_y_reduce_path():
     n = len(cols.phase_coef)
     assert np.allclose(fused.cos_sum, unfused.cos_sum[:n], rtol=1e-10, atol=1e-8)
     assert np.array_equal(fused.occupancy_counts, unfused.occupancy_counts)
+
+
+def test_tuner_choices_give_identical_results(spec):
+    """The classification mode (per-lane state machine with 1 or 2 rounds per
+    iteration, wave-cooperative lock step) changes only
+    where and when a proposal is classified, never the result: every choice
+    must give the same sums bit for bit, in the single-entry and the
+    persistent batch kernels."""
+    req = batch.library_requests(1, seed=31, walkers=(20_000, 20_000), cells=(200, 200), T_max_ms=12.0)[0]
+    prep = batch.prepare_entry(req)
+    geom = DeviceGeometry(prep.arrays)
+    ref = None
+    for rounds, cw in ((1, 1), (2, 1), (1, 2)):
+        if True:
+            say(f"single entry: scan_rounds={rounds} classify_mode={cw}")
+            p = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed, scan_rounds=rounds,
+                                   classify_mode=cw)
+            res = engine.walk_fused(geom, p, spec, precision=32)
+            assert res.n_escaped == 0 and res.n_error == 0
+            if ref is None:
+                ref = res
+            else:
+                assert np.array_equal(res.cos, ref.cos) and np.array_equal(res.sin, ref.sin)
+                assert np.array_equal(res.occupancy, ref.occupancy)
+    for cw in (1, 2):
+        say(f"batch: classify_mode={cw}")
+        out = batch.run_batched([prep], spec, classify_mode=cw)[0]
+        assert np.array_equal(out.cos, ref.cos) and np.array_equal(out.occupancy, ref.occupancy)
