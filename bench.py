@@ -99,6 +99,18 @@ def _noop(x):
     return x
 
 
+def _tuning(args, spec, log):
+    """Tuner knobs for the batched runs: explicit flags, else the autotuner's choice."""
+    if args.classify_mode or args.scan_rounds or args.grid_per_cu or args.no_tune:
+        return {"classify_mode": args.classify_mode, "scan_rounds": args.scan_rounds,
+                "grid_per_cu": args.grid_per_cu}, "flags"
+    from custom_madi_amd import tune
+    t0 = time.perf_counter()
+    choice = tune.autotune(spec, log=log)
+    log(f"tuned in {time.perf_counter() - t0:.1f} s: {choice}")
+    return choice.kwargs(), f"autotuner ({tune.cache_path()})"
+
+
 def run_c2(args):
     """256 uneven library entries: plain one-entry-per-launch form vs the batched form."""
     from custom_madi_amd import batch, serve
@@ -117,10 +129,11 @@ def run_c2(args):
     warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
     batch.run_plain(warm, spec)
 
+    knobs, knob_src = _tuning(args, spec, log)
+    walk_knobs = {k: knobs[k] for k in ("classify_mode", "scan_rounds")}
+
     # batched / served form: all requests submitted at t0
-    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
-                                scan_rounds=args.scan_rounds, classify_mode=args.classify_mode,
-                                max_tiles_per_generation=args.gen_tiles)
+    sch = batch.DeviceScheduler(spec, tier_cap_um=args.tier_cap, max_tiles_per_generation=args.gen_tiles, **knobs)
     srv = serve.Server(sch, pool=pool)
     t0 = time.perf_counter()
     for r in reqs:
@@ -133,14 +146,14 @@ def run_c2(args):
     # plain form on the first n_plain entries (sequential host prep, one launch each)
     n_plain = min(args.plain_entries, len(reqs))
     t0 = time.perf_counter()
-    res_p = batch.run_plain(reqs[:n_plain], spec)
+    res_p = batch.run_plain(reqs[:n_plain], spec, **walk_knobs)
     t_plain = time.perf_counter() - t0
     log(f"plain: {n_plain} entries in {t_plain:.1f} s")
 
     # plain form with host geometry prepared beforehand (device-only comparison)
     preps = pool.map(batch.prepare_entry, reqs[:n_plain])
     t0 = time.perf_counter()
-    res_pg = batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps))
+    res_pg = batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps), **walk_knobs)
     t_plain_dev = time.perf_counter() - t0
     pool.close()
 
@@ -166,6 +179,7 @@ def run_c2(args):
                                     "walker_steps_per_s": ws_plain / t_plain_dev},
         "batched_over_plain": eph_b / eph_p, "batched_over_plain_prebuilt": eph_b / eph_pd,
         "bitwise_equal_plain_vs_batched": f"{same}/{n_plain}",
+        "tuning": {**knobs, "source": knob_src},
         "classifier": sch.counter_dict(),
     }
     print(json.dumps(out), flush=True)
@@ -190,9 +204,8 @@ def run_c3(args):
     log = lambda m: print(f"[bench c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
     warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
     batch.run_plain(warm, spec)
-    sch = batch.DeviceScheduler(spec, grid_per_cu=args.grid_per_cu, tier_cap_um=args.tier_cap,
-                                scan_rounds=args.scan_rounds, classify_mode=args.classify_mode,
-                                max_tiles_per_generation=args.gen_tiles)
+    knobs, knob_src = _tuning(args, spec, log)
+    sch = batch.DeviceScheduler(spec, tier_cap_um=args.tier_cap, max_tiles_per_generation=args.gen_tiles, **knobs)
     stream = serve.LibraryStream(args.library_out)
     srv = serve.Server(sch, pool=pool, stream=stream)
     offered = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs) / args.duration
@@ -206,7 +219,67 @@ def run_c3(args):
                        "480 signals", "offered_walker_steps_per_s": offered,
            "hbm_used_gib_max": max((b for _, b in hbm), default=0) / 2 ** 30,
            "hbm_used_gib_samples": [[round(t, 1), round(b / 2 ** 30, 2)] for t, b in hbm], **rep,
-           "classifier": sch.counter_dict()}
+           "tuning": {**knobs, "source": knob_src}, "classifier": sch.counter_dict()}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def run_c4(args):
+    """Full-range library sweep (14,105 entries): a seeded sample batched, FP64-oracle error on a sub-sample."""
+    from custom_madi_amd import batch, engine, serve
+    from custom_madi_amd.ensemble import DeviceGeometry, create_periodic_ensemble, geometry_arrays
+    allreqs = batch.sweep_requests()
+    rng = np.random.default_rng(4)
+    n = len(allreqs) if args.entries <= 0 else min(args.entries, len(allreqs))
+    pick = np.sort(rng.choice(len(allreqs), size=n, replace=False))
+    reqs = [allreqs[i] for i in pick]
+    check = [reqs[i] for i in np.sort(rng.choice(len(reqs), size=min(args.fp64_entries, len(reqs)), replace=False))]
+    spec = batch.acquisition_spec(24)
+    pool = serve.make_pool(args.workers)
+    pool.map(_noop, range(args.workers))
+    from custom_madi_amd import _hip
+    _hip.require_gpu()
+    log = lambda m: print(f"[bench c4] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    walker_steps = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs)
+    log(f"{len(reqs)} of {len(allreqs)} entries, {walker_steps:.3e} walker-steps")
+    knobs, knob_src = _tuning(args, spec, log)
+    sch = batch.DeviceScheduler(spec, tier_cap_um=args.tier_cap, max_tiles_per_generation=args.gen_tiles, **knobs)
+    srv = serve.Server(sch, pool=pool)
+    t0 = time.perf_counter()
+    for r in reqs:
+        srv.submit(r)
+    srv.drain()
+    wall = time.perf_counter() - t0
+    pool.close()
+    res = {r.key: r for r in srv.results}
+    log(f"batched: {wall:.1f} s")
+    errs = []
+    for r in check:
+        ens = create_periodic_ensemble(r.n_cells, r.rho, r.rho * r.V * 1e-6, r.config(), seed=r.seed)
+        dg = DeviceGeometry(geometry_arrays(ens))
+        p32 = res[r.key]
+        p = engine.walk_params(r.n_walkers, r.config(), p32.pp, r.walk_seed)
+        f64 = engine.walk_fused(dg, p, spec, precision=64)
+        S32, S64 = p32.cos / r.n_walkers, f64.cos / r.n_walkers
+        se = np.sqrt(np.maximum(1.0 - S64 ** 2, 0.0) / r.n_walkers)
+        dS = np.abs(S32 - S64)
+        errs.append({"key": list(r.key), "cells": r.n_cells, "ts_ms": r.ts, "max_abs_dS": float(dS.max()),
+                     "max_abs_dS_over_3se": float((dS / (3 * se + 1e-300)).max()),
+                     "per_Delta_b_max_abs_dS": dS.max(axis=2).tolist(), "fp64_escaped": f64.n_escaped})
+        log(f"fp64 check {r.key}: max |dS| {dS.max():.2e}")
+    esc = sum(r.n_escaped + r.n_error for r in srv.results)
+    lat = [r.latency_s for r in srv.results]
+    out = {"metric": "library entries/h", "value": len(reqs) / wall * 3600.0, "unit": "entries/h", "n_gpus": 1,
+           "higher_is_better": True,
+           "workload": f"c4: full-range sweep, {len(reqs)} of {len(allreqs)} entries (seeded sample; V 0.01-206 pL, "
+                       "v_i 0.5/0.747/0.994, rho 4.4e3-5.8e7, k_io 0-130 x 65, 32-8000 cells, 5e5 walkers, "
+                       "dt 0.01 ms below 0.5 pL else 0.02 ms, 480 signals), FP32 walk, FP64 sums",
+           "wall_s": wall, "walker_steps_per_s": walker_steps / wall, "escapes_or_errors": esc,
+           "latency_s_p50": float(np.percentile(lat, 50)), "latency_s_p99": float(np.percentile(lat, 99)),
+           "full_sweep_walker_steps": sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in allreqs),
+           "fp64_check": {"entries": len(errs), "max_abs_dS": max((e["max_abs_dS"] for e in errs), default=None),
+                          "per_entry": errs},
+           "tuning": {**knobs, "source": knob_src}, "classifier": sch.counter_dict()}
     print(json.dumps(out), flush=True)
     return 0
 
@@ -222,7 +295,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1")
     ap.add_argument("--entries", type=int, default=256)
     ap.add_argument("--plain-entries", type=int, default=256)
     ap.add_argument("--workers", type=int, default=16)
@@ -233,6 +306,8 @@ def main(argv=None):
     ap.add_argument("--classify-mode", type=int, default=0,
                     help="FP32 walk: 1 per-lane state machine, 2 wave-cooperative lock step (0 default)")
     ap.add_argument("--gen-tiles", type=int, default=65536)
+    ap.add_argument("--no-tune", action="store_true", help="c2/c3/c4: default knobs instead of the autotuner")
+    ap.add_argument("--fp64-entries", type=int, default=16, help="c4: entries checked against the FP64 walk")
     ap.add_argument("--rate", type=float, default=20.0)
     ap.add_argument("--duration", type=float, default=60.0)
     ap.add_argument("--library-out", default=None)
@@ -246,6 +321,8 @@ def main(argv=None):
         return run_c2(args)
     if args.workload == "c3":
         return run_c3(args)
+    if args.workload == "c4":
+        return run_c4(args)
 
     import torch
     from custom_madi_amd import _hip, engine

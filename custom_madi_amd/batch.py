@@ -309,14 +309,16 @@ def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, **kw) ->
 
 
 def run_plain(requests: Iterable[EntryRequest], spec: engine.PhaseSpec,
-              prepare: Callable[[EntryRequest], PreparedEntry] = prepare_entry) -> List[EntryResult]:
+              prepare: Callable[[EntryRequest], PreparedEntry] = prepare_entry, classify_mode: int = 0,
+              scan_rounds: int = 0) -> List[EntryResult]:
     """One entry per launch: prepare, upload, launch, synchronise, in sequence."""
     results = []
     for req in requests:
         t_admit = time.perf_counter()
         prep = prepare(req)
         geom = DeviceGeometry(prep.arrays)
-        params = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed)
+        params = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed, scan_rounds=scan_rounds,
+                                    classify_mode=classify_mode)
         res = engine.walk_fused(geom, params, spec, precision=32)
         results.append(EntryResult(
             key=req.key, cos=res.cos, sin=res.sin, occupancy=res.occupancy, n_walkers=req.n_walkers,
@@ -343,6 +345,34 @@ def library_requests(n: int, seed: int = 2, walkers=(200_000, 1_000_000), cells=
         out.append(EntryRequest(key=(i, round(k, 4), round(rho, 1), round(v, 6)), kio=k, rho=rho, V=v, n_cells=c,
                                 n_walkers=w, seed=1000 + i, walk_seed=104_729 * (i + 1) & 0x7FFFFFFF, ts=ts, D0=D0,
                                 T_max_ms=T_max_ms))
+    return out
+
+
+def sweep_requests(n_V: int = 77, vi_values=(0.5, 0.747, 0.994), rho_range=(4.4e3, 5.8e7), V_range=(0.01, 206.0),
+                   n_kio: int = 65, kio_max: float = 130.0, walkers: int = 500_000, cells=(32, 8000),
+                   seed: int = 4, D0: float = 2.0, T_max_ms: float = 46.0) -> List[EntryRequest]:
+    """Full-range library sweep: V log-spaced over V_range, v_i from vi_values
+    (rho = v_i / V, kept inside rho_range), k_io evenly spaced over [0, kio_max]
+    (k_io = 0 gives p_p = 0), one geometry per (rho, V) group with a cell count
+    log-uniform over `cells`, dt = 0.01 ms below V = 0.5 pL and 0.02 ms above."""
+    rng = np.random.default_rng(seed)
+    out = []
+    kios = np.linspace(0.0, kio_max, n_kio)
+    group = 0
+    for v in np.exp(np.linspace(np.log(V_range[0]), np.log(V_range[1]), n_V)):
+        for f in vi_values:
+            rho = f / (v * 1e-6)
+            if not (rho_range[0] <= rho <= rho_range[1]):
+                continue
+            c = int(round(np.exp(rng.uniform(np.log(cells[0]), np.log(cells[1])))))
+            ts = 0.01 if v < 0.5 else 0.02
+            for k in kios:
+                i = len(out)
+                out.append(EntryRequest(key=(i, round(float(k), 4), round(rho, 1), round(float(v), 6)), kio=float(k),
+                                        rho=float(rho), V=float(v), n_cells=c, n_walkers=walkers,
+                                        seed=50_000 + group, walk_seed=104_729 * (i + 1) & 0x7FFFFFFF, ts=ts,
+                                        D0=D0, T_max_ms=T_max_ms))
+            group += 1
     return out
 
 

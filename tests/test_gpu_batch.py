@@ -183,3 +183,16 @@ def test_tuner_choices_give_identical_results(spec):
         say(f"batch: classify_mode={cw}")
         out = batch.run_batched([prep], spec, classify_mode=cw)[0]
         assert np.array_equal(out.cos, ref.cos) and np.array_equal(out.occupancy, ref.occupancy)
+
+
+def test_autotuner_picks_a_candidate_and_caches_it(spec, tmp_path, monkeypatch):
+    from custom_madi_amd import tune
+    monkeypatch.setenv("MADI_TUNE_CACHE", str(tmp_path / "tune.json"))
+    preps = tune.sample_entries(n=2, walkers=(5_000, 9_000))
+    cands = [(1, 2, 3), (2, 1, 4)]
+    choice = tune.autotune(spec, preps=preps, candidates=cands, use_cache=False, log=say)
+    assert (choice.classify_mode, choice.scan_rounds, choice.grid_per_cu) in cands
+    assert choice.trial_s > 0
+    again = tune.autotune(spec, preps=preps, candidates=[(1, 1, 2)])   # served from the cache
+    assert (again.classify_mode, again.scan_rounds, again.grid_per_cu) == \
+        (choice.classify_mode, choice.scan_rounds, choice.grid_per_cu)
