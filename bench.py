@@ -140,6 +140,7 @@ def run_c2(args):
         srv.submit(r)
     srv.drain()
     t_batch = time.perf_counter() - t0
+    sch.close()
     res_b = {r.key: r for r in srv.results}
     log(f"batched: {t_batch:.1f} s, generations {sch.gen}")
 
@@ -171,6 +172,7 @@ def run_c2(args):
                     "2e5-1e6 walkers, dt 0.02 ms x 2300 steps, 480 signals), FP32 walk, FP64 sums",
         "batched": {"entries": len(reqs), "wall_s": t_batch, "entries_per_hour": eph_b,
                     "walker_steps_per_s": walker_steps / t_batch, "generations": sch.gen,
+                    "graph_launches": sch.graph_launches,
                     "latency_s_p50": float(np.percentile(lat, 50)), "latency_s_p99": float(np.percentile(lat, 99)),
                     "host_workers": args.workers, "escapes_or_errors": esc},
         "plain": {"entries": n_plain, "wall_s": t_plain, "entries_per_hour": eph_p,
@@ -211,6 +213,7 @@ def run_c3(args):
     offered = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs) / args.duration
     log(f"{len(reqs)} requests over {args.duration:.0f} s, offered {offered:.3e} walker-steps/s")
     rep = serve.serve_poisson(srv, reqs, arrivals, progress=log)
+    sch.close()
     stream.close()
     pool.close()
     hbm = rep.pop("hbm_used_bytes_samples")
@@ -250,6 +253,7 @@ def run_c4(args):
         srv.submit(r)
     srv.drain()
     wall = time.perf_counter() - t0
+    sch.close()
     pool.close()
     res = {r.key: r for r in srv.results}
     log(f"batched: {wall:.1f} s")

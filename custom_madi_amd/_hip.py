@@ -115,6 +115,9 @@ SIGNATURES = {
     "madi_batch_workspace": (C.c_int, [C.c_int32, _i64p]),
     "madi_batch_launch": (C.c_int, [C.POINTER(Work), C.c_int32, C.POINTER(PhasePlan), C.POINTER(Tune), _p, C.c_int64,
                                     _p, _p]),
+    "madi_batch_graph_create": (C.c_int, [C.POINTER(PhasePlan), C.c_int32, C.POINTER(Tune), _p, C.POINTER(_p)]),
+    "madi_batch_graph_launch": (C.c_int, [_p, C.POINTER(Work), C.c_int32, _p]),
+    "madi_batch_graph_destroy": (C.c_int, [_p]),
     "madi_tiles_finalize": (C.c_int, [C.POINTER(PhasePlan), _p, C.c_int64, _p, _p, _p]),
     "madi_fill_i64": (C.c_int, [_p, C.c_int64, C.c_int64, _p]),
 }

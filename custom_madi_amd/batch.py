@@ -125,7 +125,7 @@ class DeviceScheduler:
     def __init__(self, spec: engine.PhaseSpec, grid_per_cu: int = 0, tier_cap_um: float = 0.0, scan_rounds: int = 0,
                  classify_mode: int = 0,
                  max_tiles_per_generation: int = 65536, hbm_reserve_bytes: int = 8 << 30,
-                 max_inflight: int = 2):
+                 max_inflight: int = 2, use_graphs: bool = True, graph_capacity: int = 1024):
         torch = _hip.require_gpu()
         self.torch = torch
         self.lib = _hip.load_library()
@@ -148,6 +148,32 @@ class DeviceScheduler:
         # finalisation runs on its own stream (default priority) so it does not
         # wait behind generations queued after the one that completed
         self.side = torch.cuda.Stream()
+        # one captured batch launch per in-flight generation slot
+        self.graphs: List[C.c_void_p] = []
+        self.graph_capacity = int(graph_capacity)
+        if use_graphs:
+            for _ in range(self.max_inflight):
+                h = C.c_void_p()
+                _hip.check(self.lib.madi_batch_graph_create(C.byref(self.plan), self.graph_capacity,
+                                                            C.byref(self.tune), _hip.dptr(self.counters),
+                                                            C.byref(h)), "madi_batch_graph_create")
+                self.graphs.append(h)
+        self.graph_launches = 0
+
+    def close(self) -> None:
+        """Release the captured graphs (waits for their last replays)."""
+        for h in self.graphs:
+            self.lib.madi_batch_graph_destroy(h)
+        self.graphs = []
+
+    def __del__(self):
+        import sys
+        if sys.is_finalizing():
+            return  # the HIP runtime may already be gone
+        try:
+            self.close()
+        except Exception:
+            pass
 
     # ------------------------------------------------------------ admission
     def entry_bytes(self, prep: PreparedEntry) -> int:
@@ -214,13 +240,22 @@ class DeviceScheduler:
         if not items:
             return False
         arr = (_hip.Work * len(items))(*items)
-        need = C.c_int64()
-        _hip.check(self.lib.madi_batch_workspace(len(items), C.byref(need)), "madi_batch_workspace")
-        ws = self.torch.empty(int(need.value), dtype=torch_uint8(self.torch), device="cuda")
         s = _hip.stream_ptr()
-        rc = self.lib.madi_batch_launch(arr, len(items), C.byref(self.plan), C.byref(self.tune), _hip.dptr(ws),
-                                        int(need.value), _hip.dptr(self.counters), s)
-        _hip.check(rc, "madi_batch_launch")
+        if self.graphs and len(items) <= self.graph_capacity:
+            # generation g replays graph g mod max_inflight; its previous
+            # replay (generation g - max_inflight) has completed, since at
+            # most max_inflight generations are in flight
+            ws = None
+            g = self.graphs[self.gen % len(self.graphs)]
+            _hip.check(self.lib.madi_batch_graph_launch(g, arr, len(items), s), "madi_batch_graph_launch")
+            self.graph_launches += 1
+        else:
+            need = C.c_int64()
+            _hip.check(self.lib.madi_batch_workspace(len(items), C.byref(need)), "madi_batch_workspace")
+            ws = self.torch.empty(int(need.value), dtype=torch_uint8(self.torch), device="cuda")
+            rc = self.lib.madi_batch_launch(arr, len(items), C.byref(self.plan), C.byref(self.tune), _hip.dptr(ws),
+                                            int(need.value), _hip.dptr(self.counters), s)
+            _hip.check(rc, "madi_batch_launch")
         ev = self.torch.cuda.Event()
         ev.record()
         self.inflight.append((self.gen, ev, ws, owners))
@@ -303,8 +338,10 @@ def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, **kw) ->
         while waiting and sch.admit(waiting[0]):
             waiting.popleft()
         if not sch.launch_generation() and not sch.inflight and waiting:
+            sch.close()
             raise MemoryError("an entry does not fit in free HBM even with the device idle")
         results += sch.poll(block=True)
+    sch.close()
     return results
 
 

@@ -719,9 +719,117 @@ constexpr int64_t kAlign = 256;
 
 int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-int64_t batch_bytes(int32_t n_work)
+// Batch workspace: [PlanDev | BatchHdr | WorkDev table], each part aligned.
+static int64_t batch_off_hdr() { return align_up((int64_t)sizeof(PlanDev)); }
+static int64_t batch_off_table() { return batch_off_hdr() + kAlign; }
+int64_t batch_bytes(int32_t n_work) { return batch_off_table() + align_up((int64_t)n_work * (int64_t)sizeof(WorkDev)); }
+
+// Grid-index span of a plan's events (a plan is valid for entries with at
+// least this many Y-grid times).
+static int32_t plan_grid_span(const madi_phase_plan* plan)
 {
-    return align_up((int64_t)sizeof(PlanDev)) + kAlign + align_up((int64_t)n_work * (int64_t)sizeof(WorkDev));
+    int32_t m = 0;
+    for (int p = 0; plan && p < plan->n_pairs && p < MADI_MAX_PAIRS; ++p)
+        m = std::max(m, std::max(plan->j_sum[p], std::max(plan->j_Delta[p], plan->j_delta[p])));
+    return m + 1;
+}
+
+// Device table of a batch (validates every item); total = sum of tiles.
+static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, WorkDev* table,
+                       int64_t& total, const char* who)
+{
+    int rc;
+    total = 0;
+    for (int i = 0; i < n_work; ++i) {
+        const madi_work& w = work[i];
+        WorkDev& D = table[i];
+        memset(&D, 0, sizeof D);
+        if (!w.geom || !w.d_partial || !w.d_occupancy || !w.d_flags) {
+            set_error("%s: work item %d has a NULL geometry or output", who, i);
+            return MADI_ERR_ARG;
+        }
+        if ((rc = make_walk(&w.walk, D.w))) return rc;
+        for (int p = 0; p < plan->n_pairs; ++p)
+            if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
+                plan->j_delta[p] >= w.walk.n_grid) {
+                set_error("%s: work item %d has fewer Y-grid times than the plan needs", who, i);
+                return MADI_ERR_ARG;
+            }
+        if (!graph_ready(w.geom->dev)) {
+            set_error("%s: work item %d geometry has no Delaunay graph", who, i);
+            return MADI_ERR_ARG;
+        }
+        const int64_t tiles = (w.walk.n_walkers + kBlock - 1) / kBlock;
+        if (w.tile_lo < 0 || w.n_tiles < 0 || w.tile_lo + w.n_tiles > tiles) {
+            set_error("%s: work item %d tiles [%lld, %lld) outside the entry's %lld tiles", who, i,
+                      (long long)w.tile_lo, (long long)(w.tile_lo + w.n_tiles), (long long)tiles);
+            return MADI_ERR_ARG;
+        }
+        D.g = w.geom->dev;
+        D.tile_lo = w.tile_lo;
+        D.n_tiles = w.n_tiles;
+        D.work_begin = total;
+        D.partial = w.d_partial;
+        D.occupancy = (long long*)w.d_occupancy;
+        D.flags = (long long*)w.d_flags;
+        total += w.n_tiles;
+    }
+    return MADI_OK;
+}
+
+// Launch knobs of a batch: tune's values, else the first item's, else defaults.
+struct BatchKnobs {
+    float cap_width;
+    int rounds, variant, grid;
+};
+
+static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, BatchKnobs& k)
+{
+    k.cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : (first ? first->tier_cap_um : 0.f));
+    k.rounds = rounds_of(tune && tune->scan_rounds > 0 ? tune->scan_rounds : (first ? first->scan_rounds : 0));
+    k.variant = variant_of(tune && tune->classify_mode > 0 ? tune->classify_mode : (first ? first->classify_mode : 0));
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
+    k.grid = cus * per_cu;
+    return MADI_OK;
+}
+
+}  // namespace
+
+// A captured batch launch: [copy of the staged header and table to the
+// device, persistent batch kernel] as one hipGraph, replayed per batch.
+struct madi_batch_graph {
+    int device = -1;
+    int32_t capacity = 0;
+    madi_phase_plan plan;
+    Events ev;
+    BatchKnobs knobs;
+    float tune_cap = 0.f;  // > 0 when the tune fixed the cap
+    char* d_buf = nullptr;
+    char* h_buf = nullptr;  // pinned staging, same layout as d_buf
+    int64_t bytes = 0;
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipEvent_t done = nullptr;
+    bool launched = false;
+};
+
+namespace {
+
+static void graph_release(madi_batch_graph* gr)
+{
+    if (!gr) return;
+    if (gr->launched && gr->done) (void)hipEventSynchronize(gr->done);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    if (gr->cs) (void)hipStreamDestroy(gr->cs);
+    if (gr->done) (void)hipEventDestroy(gr->done);
+    if (gr->d_buf) (void)hipFree(gr->d_buf);
+    if (gr->h_buf) (void)hipHostFree(gr->h_buf);
+    delete gr;
 }
 
 }  // namespace
@@ -757,62 +865,139 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     Events ev;
     int rc = make_plan(plan, max_grid, P, ev);
     if (rc) return rc;
-    std::vector<WorkDev> table((size_t)n_work);
+    // one host image of the workspace, one copy
+    std::vector<char> img((size_t)batch_bytes(n_work), 0);
+    memcpy(img.data(), &P, sizeof P);
     int64_t total = 0;
-    for (int i = 0; i < n_work; ++i) {
-        const madi_work& w = work[i];
-        WorkDev& D = table[(size_t)i];
-        if (!w.geom || !w.d_partial || !w.d_occupancy || !w.d_flags) {
-            set_error("madi_batch_launch: work item %d has a NULL geometry or output", i);
-            return MADI_ERR_ARG;
-        }
-        if ((rc = make_walk(&w.walk, D.w))) return rc;
-        for (int p = 0; p < plan->n_pairs; ++p)
-            if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
-                plan->j_delta[p] >= w.walk.n_grid) {
-                set_error("madi_batch_launch: work item %d has fewer Y-grid times than the plan needs", i);
-                return MADI_ERR_ARG;
-            }
-        if (!graph_ready(w.geom->dev)) {
-            set_error("madi_batch_launch: work item %d geometry has no Delaunay graph", i);
-            return MADI_ERR_ARG;
-        }
-        const int64_t tiles = (w.walk.n_walkers + kBlock - 1) / kBlock;
-        if (w.tile_lo < 0 || w.n_tiles < 0 || w.tile_lo + w.n_tiles > tiles) {
-            set_error("madi_batch_launch: work item %d tiles [%lld, %lld) outside the entry's %lld tiles", i,
-                      (long long)w.tile_lo, (long long)(w.tile_lo + w.n_tiles), (long long)tiles);
-            return MADI_ERR_ARG;
-        }
-        D.g = w.geom->dev;
-        D.tile_lo = w.tile_lo;
-        D.n_tiles = w.n_tiles;
-        D.work_begin = total;
-        D.partial = w.d_partial;
-        D.occupancy = (long long*)w.d_occupancy;
-        D.flags = (long long*)w.d_flags;
-        total += w.n_tiles;
-    }
-    const float cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : work[0].walk.tier_cap_um);
-    hipStream_t s = (hipStream_t)stream;
-    char* base = (char*)d_workspace;
-    PlanDev* dplan = (PlanDev*)base;
-    unsigned long long* queue = (unsigned long long*)(base + align_up((int64_t)sizeof(PlanDev)));
-    WorkDev* dwork = (WorkDev*)(base + align_up((int64_t)sizeof(PlanDev)) + kAlign);
-    HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(dwork, table.data(), table.size() * sizeof(WorkDev), hipMemcpyHostToDevice, s));
-    if ((rc = zero_i64((long long*)queue, 1, s))) return rc;
+    if ((rc = batch_table(work, n_work, plan, (WorkDev*)(img.data() + batch_off_table()), total,
+                          "madi_batch_launch")))
+        return rc;
+    BatchHdr* h = (BatchHdr*)(img.data() + batch_off_hdr());
+    h->queue = 0;
+    h->total = total;
+    h->n_work = n_work;
+    if (total == 0) return MADI_OK;
+    BatchKnobs k;
+    if ((rc = batch_knobs(tune, &work[0].walk, k))) return rc;
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
-    int dev = 0, cus = 0;
+    hipStream_t s = (hipStream_t)stream;
+    char* base = (char*)d_workspace;
+    HIP_TRY(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, s));
+    const int grid = (int)std::min<int64_t>(k.grid, total);
+    return launch_batch_f32((WorkDev*)(base + batch_off_table()), (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base,
+                            ev, jp, k.cap_width, k.rounds, k.variant, grid, (unsigned long long*)d_counters, s);
+}
+
+int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const madi_tune* tune, int64_t* d_counters,
+                            madi_batch_graph** out)
+{
+    if (!plan || !out || capacity < 1) {
+        set_error("madi_batch_graph_create: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    *out = nullptr;
+    PlanDev P;
+    madi_batch_graph* gr = new (std::nothrow) madi_batch_graph();
+    if (!gr) {
+        set_error("madi_batch_graph_create: out of host memory");
+        return MADI_ERR_OOM;
+    }
+    int rc = make_plan(plan, plan_grid_span(plan), P, gr->ev);
+    if (rc) {
+        delete gr;
+        return rc;
+    }
+    gr->plan = *plan;
+    gr->capacity = capacity;
+    gr->tune_cap = tune ? tune->tier_cap_um : 0.f;
+    if ((rc = batch_knobs(tune, nullptr, gr->knobs))) {
+        delete gr;
+        return rc;
+    }
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) {
+        delete gr;
+        return rc;
+    }
+    gr->bytes = batch_bytes(capacity);
+    hipError_t e = hipGetDevice(&gr->device);
+    if (e == hipSuccess) e = hipMalloc((void**)&gr->d_buf, (size_t)gr->bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&gr->h_buf, (size_t)gr->bytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&gr->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&gr->cs, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        set_error("madi_batch_graph_create: %s", hipGetErrorString(e));
+        graph_release(gr);
+        return e == hipErrorOutOfMemory ? MADI_ERR_OOM : MADI_ERR_HIP;
+    }
+    memset(gr->h_buf, 0, (size_t)gr->bytes);
+    memcpy(gr->h_buf, &P, sizeof P);
+    e = hipStreamBeginCapture(gr->cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) e = hipMemcpyAsync(gr->d_buf, gr->h_buf, (size_t)gr->bytes, hipMemcpyHostToDevice, gr->cs);
+    if (e == hipSuccess) {
+        rc = launch_batch_f32((WorkDev*)(gr->d_buf + batch_off_table()), (BatchHdr*)(gr->d_buf + batch_off_hdr()),
+                              (PlanDev*)gr->d_buf, gr->ev, jp, gr->knobs.cap_width, gr->knobs.rounds,
+                              gr->knobs.variant, gr->knobs.grid, (unsigned long long*)d_counters, gr->cs);
+        const hipError_t e2 = hipStreamEndCapture(gr->cs, &gr->graph);
+        if (rc) {
+            graph_release(gr);
+            return rc;
+        }
+        e = e2;
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&gr->exec, gr->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        set_error("madi_batch_graph_create: capture failed: %s", hipGetErrorString(e));
+        graph_release(gr);
+        return MADI_ERR_HIP;
+    }
+    *out = gr;
+    return MADI_OK;
+}
+
+int madi_batch_graph_launch(madi_batch_graph* gr, const madi_work* work, int32_t n_work, void* stream)
+{
+    if (!gr || n_work < 0 || (n_work > 0 && !work)) {
+        set_error("madi_batch_graph_launch: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    if (n_work > gr->capacity) {
+        set_error("madi_batch_graph_launch: %d work items exceed the graph's capacity of %d", n_work, gr->capacity);
+        return MADI_ERR_ARG;
+    }
+    int dev = -1;
     HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
-    int64_t grid = (int64_t)cus * per_cu;
-    if (grid > total) grid = total;
-    const int mode = tune && tune->scan_rounds > 0 ? tune->scan_rounds : work[0].walk.scan_rounds;
-    const int cm = tune && tune->classify_mode > 0 ? tune->classify_mode : work[0].walk.classify_mode;
-    return launch_batch_f32(dwork, n_work, total, queue, dplan, ev, jp, cap_width, rounds_of(mode), variant_of(cm),
-                            (int32_t)grid, (unsigned long long*)d_counters, s);
+    if (dev != gr->device) {
+        set_error("madi_batch_graph_launch: graph belongs to device %d, current device is %d", gr->device, dev);
+        return MADI_ERR_ARG;
+    }
+    if (n_work == 0) return MADI_OK;
+    if (!(gr->tune_cap > 0.f) && cap_width_of(work[0].walk.tier_cap_um) != gr->knobs.cap_width) {
+        set_error("madi_batch_graph_launch: the graph was captured for a tier cap of %g um", (double)gr->knobs.cap_width);
+        return MADI_ERR_ARG;
+    }
+    // the previous replay has read the staging buffer once it completed
+    if (gr->launched) HIP_TRY(hipEventSynchronize(gr->done));
+    int64_t total = 0;
+    int rc = batch_table(work, n_work, &gr->plan, (WorkDev*)(gr->h_buf + batch_off_table()), total,
+                         "madi_batch_graph_launch");
+    if (rc) return rc;
+    BatchHdr* h = (BatchHdr*)(gr->h_buf + batch_off_hdr());
+    h->queue = 0;
+    h->total = total;
+    h->n_work = n_work;
+    if (total == 0) return MADI_OK;
+    HIP_TRY(hipGraphLaunch(gr->exec, (hipStream_t)stream));
+    HIP_TRY(hipEventRecord(gr->done, (hipStream_t)stream));
+    gr->launched = true;
+    return MADI_OK;
+}
+
+int madi_batch_graph_destroy(madi_batch_graph* gr)
+{
+    graph_release(gr);
+    return MADI_OK;
 }
 
 int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,

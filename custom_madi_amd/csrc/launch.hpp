@@ -39,6 +39,17 @@ struct FusedOut {
 
 // One work item of the persistent batch kernel: tiles [tile_lo, tile_lo +
 // n_tiles) of one entry; work_begin is the global index of its first tile.
+// Header of a batch launch in device memory: the tile counter the
+// workgroups take tiles from (zero before the launch), the total number of
+// tiles and of work items.  Kept in memory rather than in kernel arguments so
+// a captured graph can run any batch.
+struct BatchHdr {
+    unsigned long long queue;
+    int64_t total;
+    int32_t n_work;
+    int32_t pad[3];
+};
+
 struct WorkDev {
     GeomDev g;
     WalkDev w;
@@ -69,9 +80,9 @@ int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_
                           unsigned long long* counters, hipStream_t s);
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
                           float cap_width, int rounds, int variant, unsigned long long* counters, hipStream_t s);
-int launch_batch_f32(const WorkDev* work, int32_t n_work, int64_t total, unsigned long long* queue, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int variant, int32_t grid,
-                     unsigned long long* counters, hipStream_t s);
+int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
+                     float cap_width, int rounds, int variant, int32_t grid, unsigned long long* counters,
+                     hipStream_t s);
 int launch_finalize_tiles(const double* partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols, double* cos_out,
                           double* sin_out, hipStream_t s);
 

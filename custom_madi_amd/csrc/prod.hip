@@ -1238,21 +1238,23 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 
 // Persistent work-stealing launch over many entries: each workgroup takes
-// the next global tile index from an atomic counter, finds its work item by
-// binary search on work_begin, and runs that tile.  Every workgroup exits
-// once the counter passes the total, so the grid drains.
+// the next global tile index from the header's counter, finds its work item
+// by binary search on work_begin, and runs that tile.  Every workgroup exits
+// once the counter passes the total, so the grid drains (workgroups beyond
+// the total exit at once).
 template <int NP, int R, int M>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_f32(
-    const WorkDev* __restrict__ work, int32_t n_work, int64_t total, unsigned long long* queue,
-    const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
-    unsigned long long* counters)
+    const WorkDev* __restrict__ work, BatchHdr* hdr, const PlanDev* __restrict__ plan, Events ev,
+    const Gf2Col* __restrict__ jump_pow, float cap_width, unsigned long long* counters)
 {
     __shared__ TileLds<NP> lds;
     __shared__ long long s_tile;
     __shared__ int s_item;
+    const int64_t total = hdr->total;
+    const int32_t n_work = hdr->n_work;
     for (;;) {
         if (threadIdx.x == 0) {
-            const long long t = (long long)atomicAdd(queue, 1ull);
+            const long long t = (long long)atomicAdd(&hdr->queue, 1ull);
             int lo = 0, hi = n_work - 1;
             if (t < total) {
                 while (lo < hi) {
@@ -1337,14 +1339,13 @@ int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
     return check_launch("k_walk_f32");
 }
 
-int launch_batch_f32(const WorkDev* work, int32_t n_work, int64_t total, unsigned long long* queue, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int variant, int32_t grid,
-                     unsigned long long* counters, hipStream_t s)
+int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
+                     float cap_width, int rounds, int variant, int32_t grid, unsigned long long* counters,
+                     hipStream_t s)
 {
-    if (total == 0) return MADI_OK;
 #define MADI_LAUNCH_BATCH(NPV, RV, CV)                                                                              \
-    hipLaunchKernelGGL((k_batch_f32<NPV, RV, CV>), dim3(grid), dim3(kBlock), 0, s, work, n_work,    \
-                       total, queue, plan, ev, jump_pow, cap_width, counters)
+    hipLaunchKernelGGL((k_batch_f32<NPV, RV, CV>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan, ev, jump_pow,  \
+                       cap_width, counters)
     MADI_DISPATCH_NP(ev.n_pairs, rounds, variant, MADI_LAUNCH_BATCH);
 #undef MADI_LAUNCH_BATCH
     return check_launch("k_batch_f32");

@@ -254,6 +254,22 @@ int madi_batch_workspace(int32_t n_work, int64_t* bytes);
 int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, const madi_tune* tune,
                       void* d_workspace, int64_t workspace_bytes, int64_t* d_counters, void* stream);
 
+/* A batched launch captured once as a hipGraph (copy of a pinned staging
+ * buffer holding the batch header and work table, then the persistent
+ * kernel) and replayed for every batch with madi_batch_graph_launch: one
+ * graph launch per serving generation instead of separate copies and
+ * kernel launches.  The plan, the tuning (0 = defaults) and the counters
+ * buffer are fixed at capture; a batch may hold up to `capacity` items.
+ * Replaces the same loop as madi_batch_launch (madi/library.py:546-629). */
+typedef struct madi_batch_graph madi_batch_graph;
+int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const madi_tune* tune, int64_t* d_counters,
+                            madi_batch_graph** out);
+/* Stage a batch and replay the graph on stream.  Waits for this graph's
+ * previous replay to finish before overwriting its staging buffer (callers
+ * alternate several graphs to keep batches in flight). */
+int madi_batch_graph_launch(madi_batch_graph* graph, const madi_work* work, int32_t n_work, void* stream);
+int madi_batch_graph_destroy(madi_batch_graph* graph);
+
 /* Sum an entry's tile slots [0, n_tiles) in tile order into d_cos / d_sin
  * (n_cols FP64 each). */
 int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,

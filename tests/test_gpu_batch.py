@@ -196,3 +196,22 @@ def test_autotuner_picks_a_candidate_and_caches_it(spec, tmp_path, monkeypatch):
     again = tune.autotune(spec, preps=preps, candidates=[(1, 1, 2)])   # served from the cache
     assert (again.classify_mode, again.scan_rounds, again.grid_per_cu) == \
         (choice.classify_mode, choice.scan_rounds, choice.grid_per_cu)
+
+
+def test_graph_replays_equal_eager_launches(small_requests, spec):
+    """Generations replayed from captured graphs (several replays of each,
+    with eager finalisation work in between) give the eager launches'
+    results bit for bit."""
+    preps = [batch.prepare_entry(r) for r in small_requests]
+    eager = {r.key: r for r in batch.run_batched(preps, spec, max_tiles_per_generation=5, use_graphs=False)}
+    sch = batch.DeviceScheduler(spec, max_tiles_per_generation=5)
+    assert len(sch.graphs) == sch.max_inflight
+    for p in preps:
+        assert sch.admit(p)
+    out = sch.drain()
+    say(f"graph launches: {sch.graph_launches} over {sch.gen} generations")
+    assert sch.graph_launches == sch.gen and sch.gen >= 2 * sch.max_inflight
+    sch.close()
+    assert len(out) == len(preps)
+    for r in out:
+        assert _same(r, eager[r.key]), r.key
