@@ -162,6 +162,7 @@ struct Tier {
     float cx, cy, cz;              // centre
     float ra_in2, ra_out2;         // squared radii on a's side: inside / outside answers
     float rk_in2, rk_out2;         // the same on k's side (0 = no k-side answers)
+    float rout2;                   // squared radius of a ball that is outside every contracted cell
     float alpha_a, alpha_k;
     float eps;                     // rounding bound of s over the validity ball
     int k;
@@ -172,7 +173,7 @@ __device__ __forceinline__ void tier_clear(Tier& t)
     t.ux = t.uy = t.uz = 0.f;
     t.h = 0.f;
     t.cx = t.cy = t.cz = 0.f;
-    t.ra_in2 = t.ra_out2 = t.rk_in2 = t.rk_out2 = 0.f;
+    t.ra_in2 = t.ra_out2 = t.rk_in2 = t.rk_out2 = t.rout2 = 0.f;
     t.alpha_a = t.alpha_k = 0.f;
     t.eps = 0.f;
     t.k = -1;
@@ -198,10 +199,22 @@ __device__ __forceinline__ void tier_free_water(Tier& t)
 // holds on k's side with n2 and alpha_k.  An outside answer therefore has the
 // larger radius m2 (n2): walkers in the extracellular gap, whose second plane
 // is often closer than alpha + a step, keep their tier much longer.
+//
+// The outside ball (rout2): every contracted cell j is the Voronoi cell of j
+// eroded by alpha_j, so it lies farther than alpha_j from any point outside
+// Vor(j); the neighbour k's is at least alpha_k + m1 from the centre x, and
+// a's own at least alpha_a - m1 (tier_make).  A point within that distance
+// of x is outside every cell, whichever seed is nearest; near Voronoi edges,
+// where m2 is small, this ball is the larger certificate.
 __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float qy, float qz, int& cell, int& inside)
 {
     const float dx = qx - t.cx, dy = qy - t.cy, dz = qz - t.cz;
     const float d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 < t.rout2) {
+        cell = a;  // the cell of an outside label is not used
+        inside = 0;
+        return 1;
+    }
     const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
     if (s >= 0.f) {
         const int ins = s >= t.alpha_a;
@@ -377,13 +390,14 @@ __device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, 
     return Scan{S.m1, S.m2, S.eps, S.c};
 }
 
-// Tier from the scans at q: binding edge c of anchor a with a-side bound m2a;
-// k-side bound n2 when the k side was scanned.  c < 0 means no edge below the
+// Tier from the scans at q: binding edge c of anchor a with smallest margin
+// m1 and a-side bound m2a; k-side bound n2 when the k side was scanned (an
+// outside point).  c < 0 means no edge below the
 // cap (every margin is at least cap: pass m2a = cap; the tier then answers
 // "inside a" up to cap - alpha_a); m2a = -inf gives a tier that answers
 // nothing.  One code path for every case, so the walk's update is branch-free.
-__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float qx, float qy, float qz, float m2a,
-                                          float epsa, bool kside, float n2, float epsk, Tier& t)
+__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float qx, float qy, float qz, float m1,
+                                          float m2a, float epsa, bool kside, float n2, float epsk, Tier& t)
 {
     const bool plane = c >= 0;
     const int ci = plane ? c : 0;
@@ -410,6 +424,10 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
     t.ra_out2 = ra_out > 0.f ? ra_out * ra_out : 0.f;
     t.rk_in2 = rk_in > 0.f ? rk_in * rk_in : 0.f;
     t.rk_out2 = rk_out > 0.f ? rk_out * rk_out : 0.f;
+    // outside point (kside): distance to the nearest contracted cell is at
+    // least min(alpha_a - m1, alpha_k + m1, alpha_min)
+    const float rout = kside ? fminf(fminf(aa - m1, ak + m1), g.alpha_min) - 2.f * ep : 0.f;
+    t.rout2 = rout > 0.f ? rout * rout : 0.f;
     t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fabsf(e.w));
 }
 
@@ -446,7 +464,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             return 1;
         }
         if (s.c < 0 || s.m1 >= cap) {
-            tier_make(g, a, -1, qx, qy, qz, cap, s.eps, false, 0.f, 0.f, t);
+            tier_make(g, a, -1, qx, qy, qz, 0.f, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         const bool kside = !inside;
@@ -462,7 +480,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             n2 = fminf(sk.m2, cap);
             epsk = sk.eps;
         }
-        tier_make(g, a, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, kside, n2, epsk, t);
+        tier_make(g, a, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, kside, n2, epsk, t);
         return 1;
     }
     return 0;
@@ -480,7 +498,7 @@ constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this m
 // offset) replaced in place by its answer (anchor, moved point and offset,
 // status, and the new tier; the tier's centre is the point).
 struct SvcSlot {
-    float4 v[5];
+    float4 v[6];
 };
 
 // LDS a tile walk needs.
@@ -497,7 +515,7 @@ struct TileLds {
 struct Pending {
     float qx, qy, qz, ox, oy, oz;
     int na, pin, moves, sa_c;
-    float sa_m2, sa_eps;
+    float sa_m1, sa_m2, sa_eps;
 };
 
 constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classification
@@ -506,7 +524,7 @@ __device__ __forceinline__ void pending_start(const GeomDev& g, Pending& P, Scan
 {
     P.moves = 0;
     P.sa_c = -1;
-    P.sa_m2 = P.sa_eps = 0.f;
+    P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
     scan_start(g, P.na, P.qx, P.qy, P.qz, S);
     phase = kScanA;
 }
@@ -536,6 +554,7 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         const float a1 = g.cell_alpha[P.na];
         P.pin = S.m1 >= a1;
         P.sa_c = S.c;
+        P.sa_m1 = S.m1;
         P.sa_m2 = fminf(S.m2, cap);
         P.sa_eps = S.eps;
         if (fabsf(S.m1 - a1) <= S.eps) {
@@ -584,7 +603,7 @@ __device__ __forceinline__ void tier_of(const GeomDev& g, const Pending& P, int 
                                         Tier& t)
 {
     const float m2a = tmode == 2 ? -__builtin_inff() : (tmode == 3 ? cap : P.sa_m2);
-    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, P.qx, P.qy, P.qz, m2a, P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps,
+    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, P.qx, P.qy, P.qz, P.sa_m1, m2a, P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps,
               t);
 }
 
@@ -682,7 +701,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
     P.na = P.pin = P.moves = 0;
     P.sa_c = -1;
-    P.sa_m2 = P.sa_eps = 0.f;
+    P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
     int pcell = 0, rj = 0;
     ScanState S;
     S.cur = S.end = 0;
@@ -910,17 +929,17 @@ __device__ int classify_group(const GeomDev& g, float cap, int G, int j, int& na
             return 1;
         }
         if (s.c < 0 || s.m1 >= cap) {
-            tier_make(g, na, -1, qx, qy, qz, cap, s.eps, false, 0.f, 0.f, t);
+            tier_make(g, na, -1, qx, qy, qz, 0.f, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         if (pin) {
-            tier_make(g, na, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, false, 0.f, 0.f, t);
+            tier_make(g, na, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
         const Scan sk = group_scan(g, g.nb_id[s.c], (float)((double)qx - ex), (float)((double)qy - ey),
                                    (float)((double)qz - ez), G, j, facets);
-        tier_make(g, na, s.c, qx, qy, qz, fminf(s.m2, cap), s.eps, true, fminf(sk.m2, cap), sk.eps, t);
+        tier_make(g, na, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, true, fminf(sk.m2, cap), sk.eps, t);
         return 1;
     }
     return 0;  // the greedy walk does not settle
@@ -1067,17 +1086,19 @@ __device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& 
                     slots[r].v[2] = make_float4(st.ux, st.uy, st.uz, st.h);
                     slots[r].v[3] = make_float4(st.ra_in2, st.ra_out2, st.rk_in2, st.rk_out2);
                     slots[r].v[4] = make_float4(st.alpha_a, st.alpha_k, st.eps, __int_as_float(st.k));
+                    slots[r].v[5] = make_float4(st.rout2, 0.f, 0.f, 0.f);
                 }
             }
             wave_lds_fence();
             if (need) {
                 const SvcSlot& s = slots[rank];
-                const float4 r0 = s.v[0], r1 = s.v[1], r2 = s.v[2], r3 = s.v[3], r4 = s.v[4];
+                const float4 r0 = s.v[0], r1 = s.v[1], r2 = s.v[2], r3 = s.v[3], r4 = s.v[4], r5 = s.v[5];
                 const int st = __float_as_int(r1.w);
                 if (!(st & 1)) {
                     err = 1;
                     escaped = 1;
                 } else {
+                    nt.rout2 = r5.x;
                     na = __float_as_int(r0.x);
                     qx = r0.y;
                     qy = r0.z;

@@ -1,0 +1,143 @@
+"""The plane tier of the FP32 walk, checked against brute force on the CPU.
+
+A tier built by a full classification at x answers later proposals q near x
+without classifying them (csrc/prod.hip: tier_make / tier_query).  This test
+restates both in FP64 numpy (exact arithmetic, so without the FP32 rounding
+slack) and checks every answer the tier gives against the all-seed SI Eq. S2
+labels of the 27-image tiled periodic box (oracle.periodic_brute).  It pins
+the radii: an inside answer trusted up to m2 instead of m2 - alpha, an outside
+ball not bounded by alpha_min, or a k-side answer without the k scan would
+each give wrong labels here.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from custom_madi_amd.ensemble import geometry_arrays
+from oracle import oracle as O
+from tests import _geom
+
+CAP_WIDTH = 3.0
+
+
+def _margins(arr, a, p):
+    lo, hi = arr.nb_offsets[a], arr.nb_offsets[a + 1]
+    e = arr.nb_vec[lo:hi]
+    ln = np.linalg.norm(e, axis=1)
+    u = e / ln[:, None]
+    return 0.5 * ln - u @ p, u, 0.5 * ln, lo
+
+
+def _second(m):
+    s = np.sort(m)
+    return s[0], (s[1] if len(s) > 1 else np.inf), int(np.argmin(m))
+
+
+def _locate(arr, a, p):
+    """Greedy walk to the nearest seed (exact), returning the anchor and p relative to it."""
+    for _ in range(512):
+        m, _, _, lo = _margins(arr, a, p)
+        k = int(np.argmin(m))
+        if m[k] >= 0:
+            return a, p
+        p = p - arr.nb_vec[lo + k]
+        a = int(arr.nb_ids[lo + k])
+    raise AssertionError("greedy walk did not settle")
+
+
+def tier_make(arr, a, x, cap):
+    """The tier built at x (relative to anchor a, a the nearest seed)."""
+    m, u, h, lo = _margins(arr, a, x)
+    m1, m2, c = _second(m)
+    m2 = min(m2, cap)
+    aa = arr.cell_annulus[a]
+    amin = float(arr.cell_annulus.min())
+    inside = m1 >= aa
+    t = dict(c=x.copy(), a=a, alpha_a=aa)
+    if m1 >= cap:
+        t.update(u=np.zeros(3), h=cap, k=a, alpha_k=0.0, ra_in=cap - max(aa, 0), ra_out=cap - 0.0,
+                 rk_in=0.0, rk_out=0.0, rout=0.0)
+        return t, inside
+    k = int(arr.nb_ids[lo + c])
+    ak = arr.cell_annulus[k]
+    t.update(u=u[c], h=h[c], k=k, alpha_k=ak, ra_in=m2 - max(aa, 0.0), ra_out=m2)
+    if inside:
+        t.update(rk_in=0.0, rk_out=0.0, rout=0.0)
+    else:
+        xk = x - arr.nb_vec[lo + c]
+        n = _margins(arr, k, xk)[0]
+        n2 = min(_second(n)[1], cap)
+        t.update(rk_in=n2 - max(ak, 0.0), rk_out=n2, rout=min(aa - m1, ak + m1, amin))
+    return t, inside
+
+
+def tier_query(t, q):
+    """(certified, cell, inside) as csrc/prod.hip:tier_query, without rounding slack."""
+    d = np.linalg.norm(q - t["c"])
+    if d < max(t["rout"], 0.0):
+        return True, None, False
+    s = t["h"] - q @ t["u"]
+    if s >= 0:
+        ins = s >= t["alpha_a"]
+        ok = d < max(t["ra_in"] if ins else t["ra_out"], 0.0)
+        return ok, t["a"], ins
+    r = -s
+    ins = r >= t["alpha_k"]
+    ok = d < max(t["rk_in"] if ins else t["rk_out"], 0.0)
+    return ok, t["k"], ins
+
+
+@pytest.mark.parametrize("spread", [False, True])
+def test_every_tier_answer_equals_the_brute_force_label(spread):
+    ens = _geom.periodic(n_cells=80, seed=3)
+    if spread:
+        # annuli differing from cell to cell, so alpha_min bounds the outside ball
+        ens.annulus = ens.annulus * np.random.default_rng(4).uniform(0.15, 1.0, size=len(ens.annulus))
+    arr = geometry_arrays(ens, with_kd=False)
+    cap = float(arr.cell_annulus.max()) + CAP_WIDTH
+    rng = np.random.default_rng(17)
+
+    def anchored(x_world):
+        a0 = int(np.argmin(np.linalg.norm(((arr.cell_xyz - x_world + ens.L / 2) % ens.L) - ens.L / 2, axis=1)))
+        return _locate(arr, a0, ((x_world - arr.cell_xyz[a0] + ens.L / 2) % ens.L) - ens.L / 2)
+
+    # half the centres anywhere, half in the gap near a Voronoi edge (second
+    # margin below 1 um), where third cells and the alpha_min bound matter
+    pool = rng.uniform(0, ens.L, size=(3000, 3))
+    near_edge = []
+    for x_world in pool:
+        a, x = anchored(x_world)
+        m = np.sort(_margins(arr, a, x)[0])
+        if m[0] < arr.cell_annulus[a] and m[1] < 1.0:
+            near_edge.append(x_world)
+        if len(near_edge) == 300:
+            break
+    centres = np.concatenate([pool[-300:], np.asarray(near_edge)])
+    c_lab, c_in = O.periodic_brute(ens.seeds, ens.annulus, ens.L, centres)
+    queries, expect, kinds = [], [], []
+    n_out = 0
+    for x_world, cl, ci in zip(centres, c_lab, c_in):
+        a, x = anchored(x_world)
+        t, inside = tier_make(arr, a, x, cap)
+        assert inside == bool(ci) and (not inside or a == int(cl))
+        n_out += not inside
+        rmax = max(t["ra_out"], t["rk_out"], t["rout"], 0.5) * 1.3
+        dirs = rng.normal(size=(60, 3))
+        dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+        for q in x + dirs * rng.uniform(0, rmax, size=(60, 1)) ** 1.0:
+            ok, cell, ins = tier_query(t, q)
+            if ok:
+                queries.append((arr.cell_xyz[a] + q) % ens.L)
+                expect.append((cell, ins))
+                kinds.append("ball" if cell is None else ("k" if cell == t["k"] and cell != a else "a"))
+    assert n_out > 60  # centres in the gap exercise the k side and the outside ball
+    lab, ins = O.periodic_brute(ens.seeds, ens.annulus, ens.L, np.asarray(queries))
+    kinds = np.asarray(kinds)
+    for kind in ("a", "k", "ball"):
+        assert (kinds == kind).sum() > 200, kind
+    for (cell, e_in), l, i in zip(expect, lab, ins):
+        assert bool(i) == e_in
+        if e_in:
+            assert int(l) == cell
