@@ -201,11 +201,14 @@ __device__ __forceinline__ void tier_free_water(Tier& t)
 // is often closer than alpha + a step, keep their tier much longer.
 //
 // The outside ball (rout2): every contracted cell j is the Voronoi cell of j
-// eroded by alpha_j, so it lies farther than alpha_j from any point outside
-// Vor(j); the neighbour k's is at least alpha_k + m1 from the centre x, and
-// a's own at least alpha_a - m1 (tier_make).  A point within that distance
-// of x is outside every cell, whichever seed is nearest; near Voronoi edges,
-// where m2 is small, this ball is the larger certificate.
+// eroded by alpha_j, so it lies at least alpha_j + dist(x, Vor(j)) from the
+// centre x.  The neighbour k's is thus at least alpha_k + m1 away and a's
+// own at least alpha_a - m1; the ball of radius min(m2, n2) around x lies in
+// Vor(a) u Vor(k) (no other plane of a or k is crossed within it), so every
+// third cell is at least alpha_min + min(m2, n2) away (tier_make).  A point
+// within the smallest of these distances of x is outside every cell,
+// whichever seed is nearest; near Voronoi edges, where m2 is small, this
+// ball is the larger certificate.
 __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float qy, float qz, int& cell, int& inside)
 {
     const float dx = qx - t.cx, dy = qy - t.cy, dz = qz - t.cz;
@@ -326,24 +329,40 @@ __device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 }
 
 // Running state of one facet scan: margins of the point s (relative to seed
-// `cell`) over the cell's edges in order of half length.
+// `cell`) over the cell's edges in order of half length.  cur < 0: the scan
+// of (cell, s) is set up by scan_begin on its first group.
 struct ScanState {
-    int cur, end, c;
-    float m1, m2, pn, eps;
+    int cur, end, c, cell;
+    float m1, m2, pe, eps;
     float nx, ny, nz;  // -s
 };
 
-__device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
+// Bounds of the scan of S.cell at s = -(nx, ny, nz).  eps bounds the FP32
+// rounding of every margin of the cell (three fused roundings of terms at
+// most h + |s|_1, plus the FP32 unit vectors and half lengths: below
+// 4 * 2^-24 * (|s|_1 + h_max)); pe is an upper bound of |s| (the hardware
+// square root, 1 ulp, of the rounded sum of squares, raised by 2^-20
+// relative) plus eps, so h - pe is a lower bound of the computed margin of
+// any edge of half length h.
+__device__ __forceinline__ void scan_begin(const GeomDev& g, ScanState& S)
 {
+    const int cell = S.cell;
     S.cur = g.nb_off[cell];
     S.end = g.nb_off[cell + 1];
     S.c = -1;
     S.m1 = S.m2 = __builtin_inff();
+    S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + g.cell_hmax[cell]);
+    const float n2 = fmaf(S.nx, S.nx, fmaf(S.ny, S.ny, S.nz * S.nz));
+    S.pe = fmaf(__builtin_amdgcn_sqrtf(n2), 1.f + 0x1.0p-20f, S.eps);
+}
+
+__device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
+{
+    S.cell = cell;
     S.nx = -sx;
     S.ny = -sy;
     S.nz = -sz;
-    S.pn = __builtin_sqrtf(sx * sx + sy * sy + sz * sz);
-    S.eps = kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]);
+    scan_begin(g, S);
 }
 
 // One group of kEdgePad edges (requires S.cur < S.end; lists are padded with
@@ -351,8 +370,9 @@ __device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx,
 // used, so a group costs one memory round trip.  Margins of edges past the
 // early-exit point are at least min(m2, cap), so evaluating them cannot
 // change m1, c or the capped m2: the group needs no per-edge branch.  The
-// scan ends when the group's last half length minus |s| reaches min(m2, cap)
-// (edges are sorted by h, so every later margin is at least that).
+// scan ends when the group's last half length minus pe reaches min(m2, cap):
+// edges are sorted by h, so every later computed margin is at least that,
+// and the scan's m1, c and capped m2 equal those of the whole list.
 __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets)
 {
     const float4* __restrict__ rec = g.nb_rec + S.cur;
@@ -370,7 +390,7 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
     if (cl >= 0) S.c = S.cur + cl;
     facets += kEdgePad;
     S.cur += kEdgePad;
-    if (r[kEdgePad - 1].w - S.pn >= fminf(S.m2, cap)) S.cur = S.end;
+    if (r[kEdgePad - 1].w - S.pe >= fminf(S.m2, cap)) S.cur = S.end;
 }
 
 struct Scan {
@@ -425,8 +445,9 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
     t.rk_in2 = rk_in > 0.f ? rk_in * rk_in : 0.f;
     t.rk_out2 = rk_out > 0.f ? rk_out * rk_out : 0.f;
     // outside point (kside): distance to the nearest contracted cell is at
-    // least min(alpha_a - m1, alpha_k + m1, alpha_min)
-    const float rout = kside ? fminf(fminf(aa - m1, ak + m1), g.alpha_min) - 2.f * ep : 0.f;
+    // least min(alpha_a - m1, alpha_k + m1, alpha_min + max(min(m2, n2), 0))
+    const float rout =
+        kside ? fminf(fminf(aa - m1, ak + m1), g.alpha_min + fmaxf(fminf(m2a, n2), 0.f)) - 2.f * ep : 0.f;
     t.rout2 = rout > 0.f ? rout * rout : 0.f;
     t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fabsf(e.w));
 }
@@ -520,12 +541,16 @@ struct Pending {
 
 constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classification
 
-__device__ __forceinline__ void pending_start(const GeomDev& g, Pending& P, ScanState& S, int& phase)
+__device__ __forceinline__ void pending_start(Pending& P, ScanState& S, int& phase)
 {
     P.moves = 0;
     P.sa_c = -1;
     P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
-    scan_start(g, P.na, P.qx, P.qy, P.qz, S);
+    S.cell = P.na;
+    S.nx = -P.qx;
+    S.ny = -P.qy;
+    S.nz = -P.qz;
+    S.cur = -1;
     phase = kScanA;
 }
 
@@ -538,7 +563,8 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
                                               int& tmode, uint32_t& nfac, long long& iters, long long& facets,
                                               long long& rechecks)
 {
-    if (S.cur < S.end) scan_chunk(g, cap, S, nfac);
+    if (S.cur < 0) scan_begin(g, S);
+    scan_chunk(g, cap, S, nfac);
     if (S.cur < S.end) return 0;
     int next;  // 1: greedy move to the nearer neighbour, 2: scan k's side
     if (phase == kScanK) {
@@ -593,7 +619,11 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         P.na = k;
         ++iters;
     }
-    scan_start(g, k, kx, ky, kz, S);
+    S.cell = k;
+    S.nx = -kx;
+    S.ny = -ky;
+    S.nz = -kz;
+    S.cur = -1;
     phase = next == 1 ? kScanA : kScanK;
     return 0;
 }
@@ -706,10 +736,12 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     ScanState S;
     S.cur = S.end = 0;
     S.c = -1;
-    S.m1 = S.m2 = S.pn = S.eps = S.nx = S.ny = S.nz = 0.f;
-    uint32_t wave_iters = 0;
+    S.cell = 0;
+    S.m1 = S.m2 = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+    uint32_t wave_iters = 0, lane_iters = 0;
     while (__any(phase != kDone)) {
         ++wave_iters;
+        lane_iters += phase != kDone;
         int complete = 0, tmode = 0;  // tmode 0: keep the tier; 1-4 see classify_round
         if (phase == kStep) {
             const float dx = normal_scaled_f32(rng, nscale);
@@ -739,10 +771,10 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                 complete = 1;
             } else {
                 ++n_class;
-                pending_start(g, P, S, phase);
+                pending_start(P, S, phase);
             }
         }
-#pragma unroll
+#pragma unroll 1
         for (int r = 0; r < R; ++r) {
             if (phase == kScanA || phase == kScanK) {
                 const int res = classify_round(g, cap, phase, S, P, tmode, nfac, iters, facets, rechecks);
@@ -850,7 +882,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
         const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
                      v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
                      v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
-                     v6 = wave_sum_f64((double)rejects);
+                     v6 = wave_sum_f64((double)rejects), v7 = wave_sum_f64((double)lane_iters);
         if (lane == 0) {
             atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
             atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
@@ -860,6 +892,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
             atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
             atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
+            atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
         }
     }
 }
@@ -1217,7 +1250,7 @@ __device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& 
         const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
                      v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
                      v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
-                     v6 = wave_sum_f64((double)rejects);
+                     v6 = wave_sum_f64((double)rejects), v7 = wave_sum_f64(live ? (double)p.n_steps : 0.0);
         if (lane == 0) {
             atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
             atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
@@ -1227,6 +1260,7 @@ __device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& 
             atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
             atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
             atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)p.n_steps);
+            atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
         }
     }
 }

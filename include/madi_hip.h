@@ -48,7 +48,8 @@ extern "C" {
 #define MADI_CNT_CROSSINGS 6     /* proposals with m > 0 */
 #define MADI_CNT_REJECTS 7       /* rejected crossings */
 #define MADI_CNT_WAVE_ITERS 8    /* FP32 walk: state-machine iterations, summed over waves */
-#define MADI_N_COUNTERS 9
+#define MADI_CNT_LANE_ITERS 9    /* FP32 walk: iterations until each lane's walk ended, summed over lanes */
+#define MADI_N_COUNTERS 10
 
 /* Flags summed over walkers (int64 each). */
 #define MADI_FLAG_ESCAPED 0

@@ -48,3 +48,19 @@ def periodic(n_cells=64, rho_per_uL=6.0e5, vi=0.6, seed=0, mean_av=0.6):
                     source_lo=0.0, source_hi=L, mean_AV=mean_av, geometry=_stats(L), rho_requested=rho_per_uL,
                     V_requested=vi / rho / 1e3, kd_node_seed=nodes[0], kd_node_axis=nodes[1],
                     kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], periodic=True)
+
+
+def lattice(n=4, h=2.0, jitter=1e-3, annulus=0.9, seed=0):
+    """Periodic cube of n^3 seeds on a slightly jittered cubic lattice of spacing h (Voronoi cells are near-cubes
+    of side h); every annulus is `annulus` until the caller changes it."""
+    rng = np.random.default_rng(seed)
+    L = n * h
+    ijk = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), axis=-1).reshape(-1, 3)
+    seeds = (ijk + 0.5) * h + rng.uniform(-jitter, jitter, size=(n ** 3, 3))
+    ann = np.full(n ** 3, float(annulus))
+    nodes = build_kdtree(seeds)
+    rho = n ** 3 / L ** 3 * 1e9
+    return Ensemble(seeds=seeds, annulus=ann, rho=rho, V=h ** 3 * 1e-3, vi=0.5, alpha_star=float(annulus), L=L,
+                    source_lo=0.0, source_hi=L, mean_AV=0.6, geometry=_stats(L), rho_requested=rho,
+                    V_requested=h ** 3 * 1e-3, kd_node_seed=nodes[0], kd_node_axis=nodes[1],
+                    kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], periodic=True)

@@ -6,8 +6,8 @@ restates both in FP64 numpy (exact arithmetic, so without the FP32 rounding
 slack) and checks every answer the tier gives against the all-seed SI Eq. S2
 labels of the 27-image tiled periodic box (oracle.periodic_brute).  It pins
 the radii: an inside answer trusted up to m2 instead of m2 - alpha, an outside
-ball not bounded by alpha_min, or a k-side answer without the k scan would
-each give wrong labels here.
+ball not bounded by alpha_min (plus the distance to any third Voronoi cell),
+or a k-side answer without the k scan would each give wrong labels here.
 """
 
 from __future__ import annotations
@@ -69,7 +69,7 @@ def tier_make(arr, a, x, cap):
         xk = x - arr.nb_vec[lo + c]
         n = _margins(arr, k, xk)[0]
         n2 = min(_second(n)[1], cap)
-        t.update(rk_in=n2 - max(ak, 0.0), rk_out=n2, rout=min(aa - m1, ak + m1, amin))
+        t.update(rk_in=n2 - max(ak, 0.0), rk_out=n2, rout=min(aa - m1, ak + m1, amin + max(min(m2, n2), 0.0)))
     return t, inside
 
 
@@ -126,7 +126,14 @@ def test_every_tier_answer_equals_the_brute_force_label(spread):
         rmax = max(t["ra_out"], t["rk_out"], t["rout"], 0.5) * 1.3
         dirs = rng.normal(size=(60, 3))
         dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-        for q in x + dirs * rng.uniform(0, rmax, size=(60, 1)) ** 1.0:
+        qs = x + dirs * rng.uniform(0, rmax, size=(60, 1))
+        if t["rout"] > 0:
+            # and uniformly inside the outside ball, whose third-cell bound
+            # binds near Voronoi edges
+            b = rng.normal(size=(40, 3))
+            b *= (t["rout"] * rng.uniform(0, 1, size=(40, 1)) ** (1 / 3)) / np.linalg.norm(b, axis=1, keepdims=True)
+            qs = np.concatenate([qs, x + b])
+        for q in qs:
             ok, cell, ins = tier_query(t, q)
             if ok:
                 queries.append((arr.cell_xyz[a] + q) % ens.L)
@@ -141,3 +148,33 @@ def test_every_tier_answer_equals_the_brute_force_label(spread):
         assert bool(i) == e_in
         if e_in:
             assert int(l) == cell
+
+
+def test_outside_ball_third_cell_term_is_tight():
+    """Near-cubic cells: a gap point x of cell a, 0.2 from the a->k face (m1) and 0.3 from the a->j face
+    (m2), with j's annulus (0.1) the smallest.  x lies farther than alpha_j from j's face along the a->k
+    plane, so the nearest contracted cell is j's, at exactly alpha_j + m2 = 0.4 (alpha_a - m1 = 0.7,
+    alpha_k + m1 = 1.1): the tier's ball reaches it but not beyond, so a third-cell bound taken any larger
+    would certify points inside cell j."""
+    n, h = 4, 2.0
+    ens = _geom.lattice(n=n, h=h, jitter=1e-4, annulus=0.9, seed=1)
+    idx = lambda i, j, k: (i * n + j) * n + k  # noqa: E731
+    a, k_cell, j_cell = idx(1, 1, 1), idx(2, 1, 1), idx(1, 2, 1)
+    ens.annulus[j_cell] = 0.1
+    arr = geometry_arrays(ens, with_kd=False)
+    cap = float(arr.cell_annulus.max()) + CAP_WIDTH
+    x = np.array([0.5 * h - 0.2, 0.5 * h - 0.3, 0.0])  # relative to seed a
+    a2, x2 = _locate(arr, a, x)
+    assert a2 == a
+    t, inside = tier_make(arr, a, x, cap)
+    assert not inside and t["k"] == k_cell
+    assert abs(t["rout"] - 0.4) < 2e-3
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(4000, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    for radius, expect_inside in ((t["rout"] * (1 - 1e-6), False), (t["rout"] + 0.02, True)):
+        pts = (arr.cell_xyz[a] + x + d * radius) % ens.L
+        lab, ins = O.periodic_brute(ens.seeds, ens.annulus, ens.L, pts)
+        assert bool(np.any(ins)) == expect_inside
+        if expect_inside:
+            assert set(np.asarray(lab)[np.asarray(ins).astype(bool)].tolist()) == {j_cell}
