@@ -113,6 +113,7 @@ class _Active:
     partial: object
     occ: object
     flags: object
+    moments: object
     next_tile: int = 0
     last_gen: int = -1
     admitted: float = 0.0
@@ -181,7 +182,12 @@ class DeviceScheduler:
         a = prep.arrays
         geom = (a.nb_vec.nbytes + a.nb_ids.nbytes * 2 + a.nb_offsets.nbytes + a.cell_xyz.nbytes * 2
                 + a.hint.nbytes + a.cell_annulus.nbytes * 3)
-        return tiles * self.n_cols * 16 + geom + 4096
+        return tiles * self.n_cols * 16 + self.moments_bytes(prep.req.n_walkers) + geom + 4096
+
+    def moments_bytes(self, n_walkers: int) -> int:
+        b = C.c_int64()
+        _hip.check(self.lib.madi_moments_bytes(int(n_walkers), C.byref(self.plan), C.byref(b)), "madi_moments_bytes")
+        return int(b.value)
 
     def can_admit(self, prep: PreparedEntry) -> bool:
         free, _ = self.torch.cuda.mem_get_info()
@@ -198,12 +204,14 @@ class DeviceScheduler:
         params = engine.walk_params(req.n_walkers, cfg, prep.pp, req.walk_seed)
         tiles = -(-req.n_walkers // _hip.TILE_WALKERS)
         partial = torch.empty(tiles * self.n_cols * 2, dtype=torch.float64, device="cuda")
+        # per-walker moments of the streamed variant (unused by the tile variants)
+        moments = torch.empty(self.moments_bytes(req.n_walkers), dtype=torch_uint8(torch), device="cuda")
         occ = torch.empty(cfg.n_grid, dtype=torch.int64, device="cuda")
         flags = torch.empty(_hip.N_FLAGS, dtype=torch.int64, device="cuda")
         s = _hip.stream_ptr()
         _hip.check(self.lib.madi_fill_i64(_hip.dptr(occ), cfg.n_grid, 0, s), "madi_fill_i64")
         _hip.check(self.lib.madi_fill_i64(_hip.dptr(flags), _hip.N_FLAGS, 0, s), "madi_fill_i64")
-        self.active.append(_Active(prep, geom, params, tiles, partial, occ, flags, admitted=time.perf_counter(),
+        self.active.append(_Active(prep, geom, params, tiles, partial, occ, flags, moments, admitted=time.perf_counter(),
                                    device_bytes=self.entry_bytes(prep)))
         return True
 
@@ -232,6 +240,7 @@ class DeviceScheduler:
             w.d_partial = _hip.dptr(a.partial)
             w.d_occupancy = _hip.dptr(a.occ)
             w.d_flags = _hip.dptr(a.flags)
+            w.d_moments = _hip.dptr(a.moments)
             items.append(w)
             owners.append(a)
             a.next_tile += take

@@ -357,11 +357,19 @@ static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != n
 static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
 
 static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
-// FP32 walk classification rounds per iteration (1-2); 0 picks the default.
-static int rounds_of(int r) { return r <= 0 ? 2 : (r > 2 ? 2 : r); }
-// FP32 walk classification mode -> kernel variant (0 per-lane state machine,
-// 1 wave-cooperative lock step); 0 picks the default.
-static int variant_of(int mode) { return mode == 2 ? 1 : 0; }
+// FP32 walk classification mode -> kernel variant (0 per-lane state machine
+// per tile, 1 wave-cooperative lock step per tile, kVariantStream per-lane
+// state machine with lane refill); 0 picks the default, the streamed one.
+constexpr int kVariantStream = 2;
+static int variant_of(int mode) { return mode == 1 ? 0 : (mode == 2 ? 1 : kVariantStream); }
+// FP32 walk classification rounds per iteration (1-2); 0 picks the
+// variant's default (1 streamed, where a round costs every lane of the wave
+// but a walker that waits costs only its own lane; 2 per tile).
+static int rounds_of(int r, int variant)
+{
+    if (r <= 0) return variant == kVariantStream ? 1 : 2;
+    return r > 2 ? 2 : r;
+}
 
 static int make_plan(const madi_phase_plan* plan, int32_t n_grid, PlanDev& P, Events& ev)
 {
@@ -434,6 +442,11 @@ static int make_walk(const madi_walk_params* p, WalkDev& W)
         set_error("p_p must be finite and in [0, 1], got %g", p->pp);
         return MADI_ERR_ARG;
     }
+    if (p->classify_mode < 0 || p->classify_mode > 3 || p->scan_rounds < 0 || p->scan_rounds > 2) {
+        set_error("walk params: classify_mode must be 0-3 and scan_rounds 0-2, got %d and %d", p->classify_mode,
+                  p->scan_rounds);
+        return MADI_ERR_ARG;
+    }
     memset(&W, 0, sizeof W);
     W.n_walkers = p->n_walkers;
     W.n_steps = p->n_steps;
@@ -471,6 +484,162 @@ static int zero_i64(long long* a, int n, hipStream_t s) { return fill_i64(a, n, 
 }  // namespace madi
 
 using namespace madi;
+
+// ------------------------------------------------------------- batching
+
+namespace {
+
+constexpr int64_t kAlign = 256;
+
+int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
+
+// Batch workspace: [PlanDev | BatchHdr | WorkDev table | item walker
+// counters], each part aligned.
+static int64_t batch_off_hdr() { return align_up((int64_t)sizeof(PlanDev)); }
+static int64_t batch_off_table() { return batch_off_hdr() + kAlign; }
+static int64_t batch_off_next(int32_t n_work)
+{
+    return batch_off_table() + align_up((int64_t)n_work * (int64_t)sizeof(WorkDev));
+}
+int64_t batch_bytes(int32_t n_work)
+{
+    return batch_off_next(n_work) + align_up((int64_t)n_work * (int64_t)sizeof(unsigned long long));
+}
+
+// Grid-index span of a plan's events (a plan is valid for entries with at
+// least this many Y-grid times).
+static int32_t plan_grid_span(const madi_phase_plan* plan)
+{
+    int32_t m = 0;
+    for (int p = 0; plan && p < plan->n_pairs && p < MADI_MAX_PAIRS; ++p)
+        m = std::max(m, std::max(plan->j_sum[p], std::max(plan->j_Delta[p], plan->j_delta[p])));
+    return m + 1;
+}
+
+// Launch knobs of a batch: tune's values, else the first item's, else defaults.
+struct BatchKnobs {
+    float cap_width;
+    int rounds, variant, grid, emit_grid;
+};
+
+static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, BatchKnobs& k)
+{
+    k.cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : (first ? first->tier_cap_um : 0.f));
+    k.variant = variant_of(tune && tune->classify_mode > 0 ? tune->classify_mode : (first ? first->classify_mode : 0));
+    k.rounds = rounds_of(tune && tune->scan_rounds > 0 ? tune->scan_rounds : (first ? first->scan_rounds : 0),
+                         k.variant);
+    int dev = 0, cus = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
+    k.grid = cus * per_cu;
+    k.emit_grid = cus * 8;
+    return MADI_OK;
+}
+
+// Bytes of an entry's moment buffer: n_walkers x n_pairs x 3 FP32, then
+// n_walkers active flags.
+static int64_t moments_bytes(int64_t n_walkers, int n_pairs)
+{
+    return align_up(n_walkers * (int64_t)n_pairs * 3 * (int64_t)sizeof(float) + n_walkers);
+}
+
+// Workspace of a one-entry fused walk: [PlanDev | tile slots | BatchHdr |
+// one WorkDev | its walker counter | moments | finalize chunk sums], each
+// part aligned (the walker counter, table and moments for the streamed
+// variant).
+struct FusedLayout {
+    int64_t partial, hdr, table, next, moments, final, total;
+};
+
+static FusedLayout fused_layout(int64_t n_walkers, const madi_phase_plan* plan)
+{
+    FusedLayout f;
+    const int64_t tiles = (n_walkers + kBlock - 1) / kBlock;
+    const int64_t cols = (int64_t)plan->n_pairs * plan->n_b * plan->n_dirs;
+    f.partial = align_up((int64_t)sizeof(PlanDev));
+    f.hdr = f.partial + align_up(tiles * cols * 2 * (int64_t)sizeof(double));
+    f.table = f.hdr + kAlign;
+    f.next = f.table + align_up((int64_t)sizeof(WorkDev));
+    f.moments = f.next + kAlign;
+    f.final = f.moments + moments_bytes(n_walkers, plan->n_pairs);
+    f.total = f.final + align_up(finalize_scratch_doubles(tiles, (int32_t)cols) * (int64_t)sizeof(double));
+    return f;
+}
+
+// Device table of a batch (validates every item); total = sum of tiles,
+// walkers = sum of the items' walkers.  Items of a streamed launch (stream)
+// need their entry's moment buffer.
+static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, WorkDev* table,
+                       int64_t& total, const char* who, bool stream = false, int64_t* walkers = nullptr)
+{
+    int rc;
+    total = 0;
+    int64_t n_w = 0;
+    for (int i = 0; i < n_work; ++i) {
+        const madi_work& w = work[i];
+        WorkDev& D = table[i];
+        memset(&D, 0, sizeof D);
+        if (!w.geom || !w.d_partial || !w.d_occupancy || !w.d_flags) {
+            set_error("%s: work item %d has a NULL geometry or output", who, i);
+            return MADI_ERR_ARG;
+        }
+        if ((rc = make_walk(&w.walk, D.w))) return rc;
+        for (int p = 0; p < plan->n_pairs; ++p)
+            if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
+                plan->j_delta[p] >= w.walk.n_grid) {
+                set_error("%s: work item %d has fewer Y-grid times than the plan needs", who, i);
+                return MADI_ERR_ARG;
+            }
+        if (!graph_ready(w.geom->dev)) {
+            set_error("%s: work item %d geometry has no Delaunay graph", who, i);
+            return MADI_ERR_ARG;
+        }
+        const int64_t tiles = (w.walk.n_walkers + kBlock - 1) / kBlock;
+        if (w.tile_lo < 0 || w.n_tiles < 0 || w.tile_lo + w.n_tiles > tiles) {
+            set_error("%s: work item %d tiles [%lld, %lld) outside the entry's %lld tiles", who, i,
+                      (long long)w.tile_lo, (long long)(w.tile_lo + w.n_tiles), (long long)tiles);
+            return MADI_ERR_ARG;
+        }
+        D.g = w.geom->dev;
+        D.tile_lo = w.tile_lo;
+        D.n_tiles = w.n_tiles;
+        D.work_begin = total;
+        D.partial = w.d_partial;
+        D.occupancy = (long long*)w.d_occupancy;
+        D.flags = (long long*)w.d_flags;
+        total += w.n_tiles;
+        if (stream && !w.d_moments) {
+            set_error("%s: work item %d has no moment buffer (classify_mode 3 needs madi_work.d_moments)", who, i);
+            return MADI_ERR_ARG;
+        }
+        const int64_t hi = std::min<int64_t>(w.walk.n_walkers, (w.tile_lo + w.n_tiles) * kBlock);
+        D.walker_lo = w.tile_lo * kBlock;
+        D.n_walk = std::max<int64_t>(0, hi - D.walker_lo);
+        D.walker_begin = n_w;
+        D.moments = (float*)w.d_moments;
+        D.active = w.d_moments ? (uint8_t*)w.d_moments + w.walk.n_walkers * (int64_t)plan->n_pairs * 3 * 4 : nullptr;
+        n_w += D.n_walk;
+    }
+    if (walkers) *walkers = n_w;
+    return MADI_OK;
+}
+
+// The kernels of one batch on s: the persistent tile kernel, or the
+// streamed walk and the tile sums from its moments.
+static int launch_batch(const WorkDev* table, unsigned long long* item_next, BatchHdr* hdr, const PlanDev* plan,
+                        const Events& ev, const Gf2Col* jp, const BatchKnobs& k, int32_t grid,
+                        unsigned long long* counters, hipStream_t s)
+{
+    if (k.variant != kVariantStream)
+        return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, k.variant, grid, counters, s);
+    int rc = launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
+    if (rc) return rc;
+    return launch_emit_tiles(table, hdr, plan, ev.n_pairs, k.emit_grid, s);
+}
+
+
+}  // namespace
 
 extern "C" {
 
@@ -634,13 +803,21 @@ int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t*
 
 int madi_walk_fused_workspace(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes)
 {
-    if (!plan || !bytes || n_walkers < 0) {
+    if (!plan || !bytes || n_walkers < 0 || plan->n_pairs < 0 || plan->n_pairs > MADI_MAX_PAIRS) {
         set_error("madi_walk_fused_workspace: invalid arguments");
         return MADI_ERR_ARG;
     }
-    const int64_t tiles = (n_walkers + kBlock - 1) / kBlock;
-    const int64_t cols = (int64_t)plan->n_pairs * plan->n_b * plan->n_dirs;
-    *bytes = (int64_t)sizeof(PlanDev) + tiles * cols * 2 * (int64_t)sizeof(double) + 256;
+    *bytes = fused_layout(n_walkers, plan).total;
+    return MADI_OK;
+}
+
+int madi_moments_bytes(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes)
+{
+    if (!plan || !bytes || n_walkers < 0 || plan->n_pairs < 0 || plan->n_pairs > MADI_MAX_PAIRS) {
+        set_error("madi_moments_bytes: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    *bytes = moments_bytes(n_walkers, plan->n_pairs);
     return MADI_OK;
 }
 
@@ -668,7 +845,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         return MADI_ERR_ARG;
     }
     int64_t need = 0;
-    madi_walk_fused_workspace(W.n_walkers, plan, &need);
+    if ((rc = madi_walk_fused_workspace(W.n_walkers, plan, &need))) return rc;
     if (workspace_bytes < need) {
         set_error("madi_walk_fused: workspace of %lld bytes is smaller than the %lld required",
                   (long long)workspace_bytes, (long long)need);
@@ -678,13 +855,53 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     Events ev;
     if ((rc = make_plan(plan, W.n_grid, P, ev))) return rc;
     hipStream_t s = (hipStream_t)stream;
-    PlanDev* dplan = (PlanDev*)d_workspace;
-    double* partial = (double*)((char*)d_workspace + ((sizeof(PlanDev) + 255) / 256) * 256);
+    const FusedLayout lay = fused_layout(W.n_walkers, plan);
+    char* base = (char*)d_workspace;
+    PlanDev* dplan = (PlanDev*)base;
+    double* partial = (double*)(base + lay.partial);
     HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
     if ((rc = zero_i64((long long*)d_occupancy, W.n_grid, s))) return rc;
     if ((rc = zero_i64((long long*)d_flags, MADI_N_FLAGS, s))) return rc;
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
+    const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
+    const int variant = variant_of(p->classify_mode);
+    if (precision == 32 && variant == kVariantStream && tiles > 0) {
+        // one work item covering the entry
+        madi_work item;
+        memset(&item, 0, sizeof item);
+        item.geom = g;
+        item.walk = *p;
+        item.tile_lo = 0;
+        item.n_tiles = tiles;
+        item.d_partial = partial;
+        item.d_occupancy = d_occupancy;
+        item.d_flags = d_flags;
+        item.d_moments = base + lay.moments;
+        std::vector<char> img((size_t)(lay.moments - lay.hdr), 0);
+        int64_t total = 0, walkers = 0;
+        if ((rc = batch_table(&item, 1, plan, (WorkDev*)(img.data() + (lay.table - lay.hdr)), total, "madi_walk_fused",
+                              true, &walkers)))
+            return rc;
+        BatchHdr* h = (BatchHdr*)img.data();
+        h->queue = 0;
+        h->total = total;
+        h->n_work = 1;
+        h->total_walkers = walkers;
+        HIP_TRY(hipMemcpyAsync(base + lay.hdr, img.data(), img.size(), hipMemcpyHostToDevice, s));
+        madi_tune tn;
+        memset(&tn, 0, sizeof tn);
+        tn.scan_rounds = rounds_of(p->scan_rounds, variant);
+        tn.classify_mode = 3;
+        tn.tier_cap_um = p->tier_cap_um;
+        BatchKnobs k;
+        if ((rc = batch_knobs(&tn, p, k))) return rc;
+        const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
+        rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next), (BatchHdr*)(base + lay.hdr),
+                          dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
+        if (rc) return rc;
+        return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
+    }
     FusedOut fo;
     fo.plan = dplan;
     fo.ev = ev;
@@ -693,12 +910,11 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     fo.occupancy = (long long*)d_occupancy;
     fo.flags = (long long*)d_flags;
     if (precision == 32)
-        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds),
-                                   variant_of(p->classify_mode), (unsigned long long*)d_counters, s);
+        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds, variant),
+                                   variant == kVariantStream ? 0 : variant, (unsigned long long*)d_counters, s);
     else
         rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
     if (rc) return rc;
-    const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
     if (tiles == 0) {
         std::vector<double> z(P.n_cols, 0.0);
         HIP_TRY(hipMemcpyAsync(d_cos, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, s));
@@ -706,100 +922,15 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         HIP_TRY(hipStreamSynchronize(s));
         return MADI_OK;
     }
-    return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, s);
+    return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
 }
 
 }  // extern "C"
 
-// ------------------------------------------------------------- batching
-
-namespace {
-
-constexpr int64_t kAlign = 256;
-
-int64_t align_up(int64_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
-
-// Batch workspace: [PlanDev | BatchHdr | WorkDev table], each part aligned.
-static int64_t batch_off_hdr() { return align_up((int64_t)sizeof(PlanDev)); }
-static int64_t batch_off_table() { return batch_off_hdr() + kAlign; }
-int64_t batch_bytes(int32_t n_work) { return batch_off_table() + align_up((int64_t)n_work * (int64_t)sizeof(WorkDev)); }
-
-// Grid-index span of a plan's events (a plan is valid for entries with at
-// least this many Y-grid times).
-static int32_t plan_grid_span(const madi_phase_plan* plan)
-{
-    int32_t m = 0;
-    for (int p = 0; plan && p < plan->n_pairs && p < MADI_MAX_PAIRS; ++p)
-        m = std::max(m, std::max(plan->j_sum[p], std::max(plan->j_Delta[p], plan->j_delta[p])));
-    return m + 1;
-}
-
-// Device table of a batch (validates every item); total = sum of tiles.
-static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_plan* plan, WorkDev* table,
-                       int64_t& total, const char* who)
-{
-    int rc;
-    total = 0;
-    for (int i = 0; i < n_work; ++i) {
-        const madi_work& w = work[i];
-        WorkDev& D = table[i];
-        memset(&D, 0, sizeof D);
-        if (!w.geom || !w.d_partial || !w.d_occupancy || !w.d_flags) {
-            set_error("%s: work item %d has a NULL geometry or output", who, i);
-            return MADI_ERR_ARG;
-        }
-        if ((rc = make_walk(&w.walk, D.w))) return rc;
-        for (int p = 0; p < plan->n_pairs; ++p)
-            if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
-                plan->j_delta[p] >= w.walk.n_grid) {
-                set_error("%s: work item %d has fewer Y-grid times than the plan needs", who, i);
-                return MADI_ERR_ARG;
-            }
-        if (!graph_ready(w.geom->dev)) {
-            set_error("%s: work item %d geometry has no Delaunay graph", who, i);
-            return MADI_ERR_ARG;
-        }
-        const int64_t tiles = (w.walk.n_walkers + kBlock - 1) / kBlock;
-        if (w.tile_lo < 0 || w.n_tiles < 0 || w.tile_lo + w.n_tiles > tiles) {
-            set_error("%s: work item %d tiles [%lld, %lld) outside the entry's %lld tiles", who, i,
-                      (long long)w.tile_lo, (long long)(w.tile_lo + w.n_tiles), (long long)tiles);
-            return MADI_ERR_ARG;
-        }
-        D.g = w.geom->dev;
-        D.tile_lo = w.tile_lo;
-        D.n_tiles = w.n_tiles;
-        D.work_begin = total;
-        D.partial = w.d_partial;
-        D.occupancy = (long long*)w.d_occupancy;
-        D.flags = (long long*)w.d_flags;
-        total += w.n_tiles;
-    }
-    return MADI_OK;
-}
-
-// Launch knobs of a batch: tune's values, else the first item's, else defaults.
-struct BatchKnobs {
-    float cap_width;
-    int rounds, variant, grid;
-};
-
-static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, BatchKnobs& k)
-{
-    k.cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : (first ? first->tier_cap_um : 0.f));
-    k.rounds = rounds_of(tune && tune->scan_rounds > 0 ? tune->scan_rounds : (first ? first->scan_rounds : 0));
-    k.variant = variant_of(tune && tune->classify_mode > 0 ? tune->classify_mode : (first ? first->classify_mode : 0));
-    int dev = 0, cus = 0;
-    HIP_TRY(hipGetDevice(&dev));
-    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
-    k.grid = cus * per_cu;
-    return MADI_OK;
-}
-
-}  // namespace
 
 // A captured batch launch: [copy of the staged header and table to the
-// device, persistent batch kernel] as one hipGraph, replayed per batch.
+// device, persistent batch kernel (or the streamed walk and the tile-sum
+// kernel)] as one hipGraph, replayed per batch.
 struct madi_batch_graph {
     int device = -1;
     int32_t capacity = 0;
@@ -868,25 +999,27 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     // one host image of the workspace, one copy
     std::vector<char> img((size_t)batch_bytes(n_work), 0);
     memcpy(img.data(), &P, sizeof P);
-    int64_t total = 0;
+    BatchKnobs k;
+    if ((rc = batch_knobs(tune, &work[0].walk, k))) return rc;
+    int64_t total = 0, walkers = 0;
     if ((rc = batch_table(work, n_work, plan, (WorkDev*)(img.data() + batch_off_table()), total,
-                          "madi_batch_launch")))
+                          "madi_batch_launch", k.variant == kVariantStream, &walkers)))
         return rc;
     BatchHdr* h = (BatchHdr*)(img.data() + batch_off_hdr());
     h->queue = 0;
     h->total = total;
     h->n_work = n_work;
+    h->total_walkers = walkers;
     if (total == 0) return MADI_OK;
-    BatchKnobs k;
-    if ((rc = batch_knobs(tune, &work[0].walk, k))) return rc;
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
     hipStream_t s = (hipStream_t)stream;
     char* base = (char*)d_workspace;
     HIP_TRY(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, s));
-    const int grid = (int)std::min<int64_t>(k.grid, total);
-    return launch_batch_f32((WorkDev*)(base + batch_off_table()), (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base,
-                            ev, jp, k.cap_width, k.rounds, k.variant, grid, (unsigned long long*)d_counters, s);
+    const int grid = (int)std::min<int64_t>(k.grid, k.variant == kVariantStream ? (walkers + 63) / 64 : total);
+    return launch_batch((WorkDev*)(base + batch_off_table()), (unsigned long long*)(base + batch_off_next(n_work)),
+                        (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base, ev, jp, k, grid,
+                        (unsigned long long*)d_counters, s);
 }
 
 int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const madi_tune* tune, int64_t* d_counters,
@@ -936,9 +1069,10 @@ int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const
     e = hipStreamBeginCapture(gr->cs, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) e = hipMemcpyAsync(gr->d_buf, gr->h_buf, (size_t)gr->bytes, hipMemcpyHostToDevice, gr->cs);
     if (e == hipSuccess) {
-        rc = launch_batch_f32((WorkDev*)(gr->d_buf + batch_off_table()), (BatchHdr*)(gr->d_buf + batch_off_hdr()),
-                              (PlanDev*)gr->d_buf, gr->ev, jp, gr->knobs.cap_width, gr->knobs.rounds,
-                              gr->knobs.variant, gr->knobs.grid, (unsigned long long*)d_counters, gr->cs);
+        rc = launch_batch((WorkDev*)(gr->d_buf + batch_off_table()),
+                          (unsigned long long*)(gr->d_buf + batch_off_next(capacity)),
+                          (BatchHdr*)(gr->d_buf + batch_off_hdr()), (PlanDev*)gr->d_buf, gr->ev, jp, gr->knobs,
+                          gr->knobs.grid, (unsigned long long*)d_counters, gr->cs);
         const hipError_t e2 = hipStreamEndCapture(gr->cs, &gr->graph);
         if (rc) {
             graph_release(gr);
@@ -979,14 +1113,15 @@ int madi_batch_graph_launch(madi_batch_graph* gr, const madi_work* work, int32_t
     }
     // the previous replay has read the staging buffer once it completed
     if (gr->launched) HIP_TRY(hipEventSynchronize(gr->done));
-    int64_t total = 0;
+    int64_t total = 0, walkers = 0;
     int rc = batch_table(work, n_work, &gr->plan, (WorkDev*)(gr->h_buf + batch_off_table()), total,
-                         "madi_batch_graph_launch");
+                         "madi_batch_graph_launch", gr->knobs.variant == kVariantStream, &walkers);
     if (rc) return rc;
     BatchHdr* h = (BatchHdr*)(gr->h_buf + batch_off_hdr());
     h->queue = 0;
     h->total = total;
     h->n_work = n_work;
+    h->total_walkers = walkers;
     if (total == 0) return MADI_OK;
     HIP_TRY(hipGraphLaunch(gr->exec, (hipStream_t)stream));
     HIP_TRY(hipEventRecord(gr->done, (hipStream_t)stream));
@@ -1008,7 +1143,13 @@ int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, in
         return MADI_ERR_ARG;
     }
     const int32_t n_cols = plan->n_pairs * plan->n_b * plan->n_dirs;
-    return launch_finalize_tiles(d_partial, 0, n_tiles, n_cols, d_cos, d_sin, (hipStream_t)stream);
+    hipStream_t s = (hipStream_t)stream;
+    double* scratch = nullptr;
+    HIP_TRY(hipMallocAsync((void**)&scratch,
+                           (size_t)std::max<int64_t>(1, finalize_scratch_doubles(n_tiles, n_cols)) * sizeof(double), s));
+    const int rc = launch_finalize_tiles(d_partial, 0, n_tiles, n_cols, d_cos, d_sin, scratch, s);
+    (void)hipFreeAsync(scratch, s);
+    return rc;
 }
 
 int madi_fill_i64(int64_t* d, int64_t n, int64_t value, void* stream)

@@ -92,6 +92,24 @@ struct GeomDev {
     int32_t free_water;
 };
 
+// Load through a global-address-space view of p.  Pointers read from device
+// memory (a batch's work table) are generic, and generic loads take the flat
+// path, whose waits also cover outstanding LDS operations; every pointer
+// this is used on points to device global memory.
+template <typename T>
+__device__ __forceinline__ T gload(const T* p)
+{
+    return *(const __attribute__((address_space(1))) T*)p;
+}
+
+// The same for float4 records (one 16-byte load of a native vector).
+__device__ __forceinline__ float4 gload(const float4* p)
+{
+    typedef float v4f __attribute__((ext_vector_type(4)));
+    const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 constexpr int kMaxGraphIters = 256;
 
 // Each cell's edge list is padded to a multiple of kEdgePad with sentinel

@@ -47,9 +47,16 @@ struct BatchHdr {
     unsigned long long queue;
     int64_t total;
     int32_t n_work;
-    int32_t pad[3];
+    int32_t pad;
+    int64_t total_walkers;  // streamed walk: walkers of all items
 };
 
+// Streamed walk (lane refill): walkers [walker_lo, walker_lo + n_walk) of
+// the entry, handed out from the item's counter (an array beside the table,
+// zero before the launch, so the table itself stays read-only); walker_begin
+// is the global index of the item's first walker.  Each walker leaves its
+// moments (n_pairs x 3 floats) in `moments` and its active flag in `active`,
+// both indexed by the entry's walker index.
 struct WorkDev {
     GeomDev g;
     WalkDev w;
@@ -59,6 +66,11 @@ struct WorkDev {
     double* partial;
     long long* occupancy;
     long long* flags;
+    int64_t walker_lo;
+    int64_t n_walk;
+    int64_t walker_begin;
+    float* moments;
+    uint8_t* active;
 };
 
 void set_error(const char* fmt, ...);
@@ -83,7 +95,15 @@ int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
 int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
                      float cap_width, int rounds, int variant, int32_t grid, unsigned long long* counters,
                      hipStream_t s);
+int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
+                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                      unsigned long long* counters, hipStream_t s);
+int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,
+                      hipStream_t s);
+// Column sums of an entry's tile slots (see k_finalize_chunks for the order);
+// scratch holds finalize_scratch_doubles(n_tiles, n_cols) doubles.
+int64_t finalize_scratch_doubles(int64_t n_tiles, int32_t n_cols);
 int launch_finalize_tiles(const double* partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols, double* cos_out,
-                          double* sin_out, hipStream_t s);
+                          double* sin_out, double* scratch, hipStream_t s);
 
 }  // namespace madi

@@ -6,7 +6,7 @@
 // pc(pair, b) * (dir . dM(pair)) and the walk kernels need the sums of its
 // cos and sin over walkers.  Sums are formed in FP64 in a fixed order: a
 // wave butterfly, then the block's waves in index order into the tile's slot;
-// madi_finalize_tiles adds tile slots in tile order.  Results are therefore
+// madi_tiles_finalize adds tile slots in a fixed chunked order.  Results are therefore
 // bitwise reproducible whichever workgroup runs a tile.
 #pragma once
 
@@ -48,6 +48,14 @@ __device__ __forceinline__ void sincos_f32(float ph, float& s, float& c)
     r = r - rintf(r);
     s = __builtin_amdgcn_sinf(r);
     c = __builtin_amdgcn_cosf(r);
+}
+
+// FP32 phase of column (p, b, d) for the moment (mx, my, mz): pc(p, b) times
+// the moment's projection on direction d.
+__device__ __forceinline__ float column_phase_f32(const PlanDev& plan, int p, int b, int d, float mx, float my, float mz)
+{
+    const float proj = fmaf(plan.dirs32[3 * d], mx, fmaf(plan.dirs32[3 * d + 1], my, plan.dirs32[3 * d + 2] * mz));
+    return plan.pc32[p * plan.n_b + b] * proj;
 }
 
 // Accumulate one column value pair into the wave's LDS slot and flush full
@@ -104,10 +112,8 @@ __device__ void emit_columns(const PlanDev& plan, const T (&dM)[NP][3], bool act
                             sn = sin(ph);
                         }
                     } else {
-                        const float proj = plan.dirs32[3 * d] * mx + plan.dirs32[3 * d + 1] * my + plan.dirs32[3 * d + 2] * mz;
-                        const float ph = plan.pc32[p * n_b + b] * proj;
                         float s, c;
-                        sincos_f32(ph, s, c);
+                        sincos_f32(column_phase_f32(plan, p, b, d, mx, my, mz), s, c);
                         if (active) {
                             cs = (double)c;
                             sn = (double)s;

@@ -39,8 +39,9 @@ constexpr float kUlp = 4.8e-7f;  // 8 * 2^-24: FP32 margin error per unit of |p|
 // FP64 decision for the point anchor + p (p relative to the anchor).  Moves
 // the anchor to the FP64 nearest seed; returns 0 if the greedy walk does not
 // settle or two seeds coincide.
+template <typename Count>
 __device__ int locate_f64(const GeomDev& g, int& a, float& px, float& py, float& pz, float& ox, float& oy, float& oz,
-                          int& inside, long long& facets)
+                          int& inside, Count& facets)
 {
     double qx = px, qy = py, qz = pz;
     double sx = ox, sy = oy, sz = oz;
@@ -242,7 +243,7 @@ __device__ __forceinline__ int hint_anchor(const GeomDev& g, double x, double y,
     ix = ix < 0 ? 0 : (ix >= n ? n - 1 : ix);
     iy = iy < 0 ? 0 : (iy >= n ? n - 1 : iy);
     iz = iz < 0 ? 0 : (iz >= n ? n - 1 : iz);
-    return g.hint[(ix * n + iy) * n + iz];
+    return gload(g.hint + (ix * n + iy) * n + iz);
 }
 
 __device__ __forceinline__ double wrap_f64(double x, double L)
@@ -262,9 +263,9 @@ __device__ __forceinline__ void anchor_world(const GeomDev& g, double x, double 
     }
     a = hint_anchor(g, x, y, z);
     const double* c = g.cell_xyz + 3 * a;
-    px = (float)(x - c[0]);
-    py = (float)(y - c[1]);
-    pz = (float)(z - c[2]);
+    px = (float)(x - gload(c));
+    py = (float)(y - gload(c + 1));
+    pz = (float)(z - gload(c + 2));
 }
 
 __global__ __launch_bounds__(kBlock) void k_classify_graph(GeomDev g, const double* __restrict__ pts, int64_t n,
@@ -347,11 +348,11 @@ struct ScanState {
 __device__ __forceinline__ void scan_begin(const GeomDev& g, ScanState& S)
 {
     const int cell = S.cell;
-    S.cur = g.nb_off[cell];
-    S.end = g.nb_off[cell + 1];
+    S.cur = gload(g.nb_off + cell);
+    S.end = gload(g.nb_off + cell + 1);
     S.c = -1;
     S.m1 = S.m2 = __builtin_inff();
-    S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + g.cell_hmax[cell]);
+    S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + gload(g.cell_hmax + cell));
     const float n2 = fmaf(S.nx, S.nx, fmaf(S.ny, S.ny, S.nz * S.nz));
     S.pe = fmaf(__builtin_amdgcn_sqrtf(n2), 1.f + 0x1.0p-20f, S.eps);
 }
@@ -378,7 +379,7 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
     const float4* __restrict__ rec = g.nb_rec + S.cur;
     float4 r[kEdgePad];
 #pragma unroll
-    for (int u = 0; u < kEdgePad; ++u) r[u] = rec[u];
+    for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
     int cl = -1;
 #pragma unroll
     for (int u = 0; u < kEdgePad; ++u) {
@@ -400,13 +401,11 @@ struct Scan {
 
 // Whole scan of one cell.
 __device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, float py, float pz, float cap,
-                                          long long& facets)
+                                          uint32_t& facets)
 {
     ScanState S;
     scan_start(g, cell, px, py, pz, S);
-    uint32_t nf = 0;
-    while (S.cur < S.end) scan_chunk(g, cap, S, nf);
-    facets += nf;
+    while (S.cur < S.end) scan_chunk(g, cap, S, facets);
     return Scan{S.m1, S.m2, S.eps, S.c};
 }
 
@@ -421,10 +420,10 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
 {
     const bool plane = c >= 0;
     const int ci = plane ? c : 0;
-    const float4 e0 = g.nb_rec[ci];
-    const int k = plane ? g.nb_id[ci] : a;
+    const float4 e0 = gload(g.nb_rec + ci);
+    const int k = plane ? gload(g.nb_id + ci) : a;
     const float4 e = plane ? e0 : make_float4(0.f, 0.f, 0.f, m2a);
-    const float aa = g.cell_alpha[a], ak = g.cell_alpha[k];
+    const float aa = gload(g.cell_alpha + a), ak = gload(g.cell_alpha + k);
     kside = kside && plane;
     const float ep = kside ? fmaxf(epsa, epsk) : epsa;
     const float ra_in = m2a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m2a - 2.f * ep;
@@ -458,10 +457,10 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
 // walker reaches k's side only by crossing the whole gap).  Returns 0 on a
 // classifier failure.
 __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, float& qz, float& ox, float& oy,
-                             float& oz, float cap, int& inside, Tier& t, long long& iters, long long& facets,
-                             long long& rechecks)
+                             float& oz, float cap, int& inside, Tier& t, uint32_t& iters, uint32_t& facets,
+                             uint32_t& rechecks)
 {
-    for (int it = 0; it < kMaxGraphIters; ++it) {
+    for (int it = 0; it <= kMaxGraphIters; ++it) {
         const Scan s = scan_cell(g, a, qx, qy, qz, cap, facets);
         if (s.m1 < -s.eps) {
             // a neighbour is nearer: move the anchor and rescan
@@ -560,8 +559,8 @@ __device__ __forceinline__ void pending_start(Pending& P, ScanState& S, int& pha
 // when the label is known (tmode: 1 two-sided tier, 2 none, 3 no plane
 // below the cap, 4 a side only), -1 on a classifier failure, 0 otherwise.
 __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& phase, ScanState& S, Pending& P,
-                                              int& tmode, uint32_t& nfac, long long& iters, long long& facets,
-                                              long long& rechecks)
+                                              int& tmode, uint32_t& nfac, uint32_t& iters, uint32_t& facets,
+                                              uint32_t& rechecks)
 {
     if (S.cur < 0) scan_begin(g, S);
     scan_chunk(g, cap, S, nfac);
@@ -577,7 +576,7 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         if (++P.moves > kMaxGraphIters) return -1;  // the greedy walk does not settle
         next = 1;
     } else {
-        const float a1 = g.cell_alpha[P.na];
+        const float a1 = gload(g.cell_alpha + P.na);
         P.pin = S.m1 >= a1;
         P.sa_c = S.c;
         P.sa_m1 = S.m1;
@@ -604,8 +603,8 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
     }
     // the point relative to neighbour k of the binding edge
     const int c = S.c;
-    const double ex = g.nb_e64[3 * c], ey = g.nb_e64[3 * c + 1], ez = g.nb_e64[3 * c + 2];
-    const int k = g.nb_id[c];
+    const double ex = gload(g.nb_e64 + 3 * c), ey = gload(g.nb_e64 + 3 * c + 1), ez = gload(g.nb_e64 + 3 * c + 2);
+    const int k = gload(g.nb_id + c);
     const float kx = (float)((double)P.qx - ex);
     const float ky = (float)((double)P.qy - ey);
     const float kz = (float)((double)P.qz - ez);
@@ -663,7 +662,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     const int64_t w = w_lo + threadIdx.x;
     const bool live = w < p.n_walkers;
     uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
-    long long iters = 0, facets = 0, rechecks = 0;
+    uint32_t iters = 0, facets = 0, rechecks = 0;
     const float cap = g.alpha_max + cap_width;
     // grids up to kMaxLdsGrid keep occupancy counters and the event table in
     // LDS; longer grids count in global memory and walk the event list
@@ -936,8 +935,8 @@ __device__ __forceinline__ Scan group_scan(const GeomDev& g, int cell, float sx,
 // outside point, and the new tier.  All lanes of the group compute the same
 // values.  Returns 0 on a classifier failure.
 __device__ int classify_group(const GeomDev& g, float cap, int G, int j, int& na, float& qx, float& qy, float& qz,
-                              float& ox, float& oy, float& oz, int& pin, Tier& t, uint32_t& facets, long long& iters,
-                              long long& rechecks, long long& facets64)
+                              float& ox, float& oy, float& oz, int& pin, Tier& t, uint32_t& facets, uint32_t& iters,
+                              uint32_t& rechecks, uint32_t& facets64)
 {
     for (int moves = 0; moves <= kMaxGraphIters; ++moves) {
         const Scan s = group_scan(g, na, qx, qy, qz, G, j, facets);
@@ -1004,7 +1003,7 @@ __device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& 
     const int64_t w = w_lo + threadIdx.x;
     const bool live = w < p.n_walkers;
     uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
-    long long iters = 0, facets = 0, rechecks = 0;
+    uint32_t iters = 0, facets = 0, rechecks = 0;
     const float cap = g.alpha_max + cap_width;
     const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
     if (occ_in_lds) {
@@ -1265,6 +1264,392 @@ __device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& 
     }
 }
 
+// ------------------------------------------------------------ streamed walk
+//
+// Lane refill: the walkers of a batch are handed to lanes one at a time, so a
+// lane whose walker ends takes the next one instead of idling until the
+// slowest walker of its wave ends.  Each wave claims kClaimWalkers walkers of
+// a work item at a time from the item's counter, computes their generator
+// states into its LDS slots, and hands them out as lanes free up.  A walker's
+// first classification runs through the same per-lane state machine as its
+// steps, so starting a walker does not stall the wave.  A finished walker
+// stores its moments and active flag; k_emit_tiles then forms every tile's
+// column sums from them exactly as the tile variants do, so results are
+// bitwise those of the other variants.  A wave starts at the item holding its
+// share of the batch's walkers and moves to the next item with walkers left
+// once its lanes are idle; it exits when no item has walkers left (every
+// wave reaches that point: claims only advance the counters).
+
+constexpr int kClaimWalkers = 64;  // one walker per lane: at the end of a batch no wave holds more than
+                                   // one walker per lane that others could have taken
+
+template <int NP>
+struct StreamLds {
+    float evc[kMaxLdsGrid][NP];     // per grid time: summed event signs per pair (the batch shares the plan)
+    int occ[kWaves][kMaxLdsGrid];   // per wave: inside counts of its current item
+    Rng states[kWaves][kClaimWalkers];
+};
+
+// Number of set bits of mask below this lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Claims the next kClaimWalkers walkers of item W for the calling wave (all
+// 64 lanes active) and writes their generator states to st; returns the
+// claim's first walker relative to the item (n_walk or more: none left).
+__device__ __forceinline__ int64_t stream_claim(const WorkDev* __restrict__ W, unsigned long long* next,
+                                                const Gf2Col* __restrict__ jump_pow, Rng* st, int lane)
+{
+    unsigned long long b = 0;
+    if (lane == 0) b = atomicAdd(next, (unsigned long long)kClaimWalkers);
+    b = __shfl(b, 0, 64);
+    const int64_t base = (int64_t)__builtin_amdgcn_readfirstlane((uint32_t)b) |
+                         ((int64_t)__builtin_amdgcn_readfirstlane((uint32_t)(b >> 32)) << 32);
+    if (base >= W->n_walk) return base;
+    st[lane] = rng_for_walker(W->w.seed, W->w.first_walker + (uint64_t)(W->walker_lo + base + lane), jump_pow);
+    wave_lds_fence();
+    return base;
+}
+
+template <int NP, int R>
+__device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
+                            const PlanDev& plan,
+                            const Gf2Col* __restrict__ jump_pow, const Events& ev, float cap_width,
+                            unsigned long long* counters, StreamLds<NP>& L)
+{
+    enum : int { kStep = 0, kDone = 3, kIdle = 4 };
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int32_t n_work = hdr->n_work;
+    const int64_t total_w = hdr->total_walkers;
+    int* occ = L.occ[wv];
+    Rng* st = L.states[wv];
+    for (int j = lane; j < kMaxLdsGrid; j += 64) occ[j] = 0;
+    wave_lds_fence();
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
+    uint32_t iters = 0, facets = 0, rechecks = 0;
+    uint32_t wave_iters = 0, lane_iters = 0;
+    const int n_pairs = plan.n_pairs;
+    if (n_work > 0 && total_w > 0) {
+        // home item: the one holding this wave's share of the walkers
+        const int64_t n_waves = (int64_t)gridDim.x * kWaves;
+        const int64_t pos = ((int64_t)blockIdx.x * kWaves + wv) * total_w / n_waves;
+        int lo = 0, hi = n_work - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (work[mid].walker_begin <= pos) lo = mid;
+            else hi = mid - 1;
+        }
+        int e = __builtin_amdgcn_readfirstlane(lo);
+        int dry = 0;  // items found without walkers in a row
+        while (dry < n_work) {
+            const WorkDev* __restrict__ W = work + e;
+            unsigned long long* next = item_next + e;
+            int64_t claim = stream_claim(W, next, jump_pow, st, lane);
+            if (claim >= W->n_walk) {
+                ++dry;
+                e = e + 1 == n_work ? 0 : e + 1;
+                continue;
+            }
+            dry = 0;
+            const GeomDev& g = W->g;
+            const WalkDev& p = W->w;
+            const int64_t n_walk = W->n_walk, walker_lo = W->walker_lo;
+            const float cap = g.alpha_max + cap_width;
+            const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
+            const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
+            const float ts = (float)p.ts;
+            int64_t hand = claim, hand_end = claim + kClaimWalkers < n_walk ? claim + kClaimWalkers : n_walk;
+            bool item_dry = false;
+
+            // per-lane walker state
+            int phase = kIdle, init = 0;
+            int64_t w = 0;
+            Rng rng{0, 0};
+            int a = 0, in = 0, cell = 0, err = 0, escaped = 0;
+            uint32_t n_esc = 0, n_err = 0;
+            float px = 0.f, py = 0.f, pz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
+            float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
+            Tier t;
+            tier_clear(t);
+            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+            float dM[NP][3];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
+            int step = 0, until_sample = 0, j_sample = 0, next_ev = 0;
+            Pending P;
+            P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
+            P.na = P.pin = P.moves = 0;
+            P.sa_c = -1;
+            P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
+            int pcell = 0, rj = 0;
+            ScanState S;
+            S.cur = S.end = 0;
+            S.c = -1;
+            S.cell = 0;
+            S.m1 = S.m2 = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+            for (;;) {
+                // hand walkers to idle lanes
+                uint64_t need = __ballot(phase == kIdle);
+                if (need != 0 && !item_dry) {
+                    bool got = false;
+                    while (need != 0) {
+                        if (hand >= hand_end) {
+                            const int64_t nb = stream_claim(W, next, jump_pow, st, lane);
+                            if (nb >= n_walk) {
+                                item_dry = true;
+                                break;
+                            }
+                            claim = nb;
+                            hand = nb;
+                            hand_end = nb + kClaimWalkers < n_walk ? nb + kClaimWalkers : n_walk;
+                        }
+                        const int64_t rank = (int64_t)lanes_below(need);
+                        const bool mine = ((need >> lane) & 1ull) && rank < hand_end - hand;
+                        if (mine) {
+                            w = hand + rank;
+                            rng = st[w - claim];
+                            got = true;
+                        }
+                        const uint64_t taken = __ballot(mine);
+                        hand += __popcll(taken);
+                        need &= ~taken;
+                        wave_lds_fence();  // the slots are read before a claim rewrites them
+                    }
+                    if (got) {
+                        // walker start (walk_tile_f32's set-up, the first classification queued)
+                        const double span = g.source_hi - g.source_lo;
+                        const double x0 = g.source_lo + uniform_f64(rng) * span;
+                        const double y0 = g.source_lo + uniform_f64(rng) * span;
+                        const double z0 = g.source_lo + uniform_f64(rng) * span;
+                        lox = (float)(-x0);
+                        loy = (float)(-y0);
+                        loz = (float)(-z0);
+                        hix = (float)(g.L - x0);
+                        hiy = (float)(g.L - y0);
+                        hiz = (float)(g.L - z0);
+                        Sx = Sy = Sz = Cx = Cy = Cz = 0.f;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
+                        step = 0;
+                        until_sample = p.steps_per_h;
+                        j_sample = 0;
+                        next_ev = 0;
+                        err = escaped = 0;
+                        if (g.free_water) {
+                            tier_free_water(t);
+                            a = cell = in = 0;
+                            px = py = pz = ox = oy = oz = 0.f;
+                            init = 0;
+                            phase = p.n_steps > 0 ? kStep : kDone;
+                        } else {
+                            anchor_world(g, x0, y0, z0, P.na, P.qx, P.qy, P.qz);
+                            P.ox = -P.qx;
+                            P.oy = -P.qy;
+                            P.oz = -P.qz;
+                            ++n_class;
+                            pending_start(P, S, phase);
+                            init = 1;
+                        }
+                    }
+                }
+                if (__ballot(phase != kIdle) == 0) break;
+                ++wave_iters;
+                lane_iters += phase != kIdle;
+                int complete = 0, tmode = 0;
+                if (phase == kStep) {
+                    const float dx = normal_scaled_f32(rng, nscale);
+                    const float dy = normal_scaled_f32(rng, nscale);
+                    const float dz = normal_scaled_f32(rng, nscale);
+                    const uint64_t ku = rng_next(rng) >> 11;
+                    rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
+                    P.qx = px + dx;
+                    P.qy = py + dy;
+                    P.qz = pz + dz;
+                    P.ox = ox;
+                    P.oy = oy;
+                    P.oz = oz;
+                    P.na = a;
+                    const float nDx = ox + P.qx, nDy = oy + P.qy, nDz = oz + P.qz;
+                    if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
+                        escaped = 1;
+                        phase = kDone;
+                        // samples after an escape keep the last label
+                        if (in)
+                            for (int j = j_sample + 1; j < p.n_grid; ++j) {
+                                if (occ_in_lds) atomicAdd(&occ[j], 1);
+                                else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
+                            }
+                    } else if (tier_query(t, a, P.qx, P.qy, P.qz, pcell, P.pin)) {
+                        ++n_tier;
+                        complete = 1;
+                    } else {
+                        ++n_class;
+                        pending_start(P, S, phase);
+                    }
+                }
+#pragma unroll 1
+                for (int r = 0; r < R; ++r) {
+                    if (phase == kScanA || phase == kScanK) {
+                        const int res = classify_round(g, cap, phase, S, P, tmode, nfac, iters, facets, rechecks);
+                        if (res > 0) {
+                            complete = 1;
+                            pcell = P.na;
+                            phase = kStep;  // completed below
+                        } else if (res < 0) {
+                            err = 1;
+                            escaped = 1;
+                            phase = kDone;
+                        }
+                    }
+                }
+                if (complete && init) {
+                    // the walker's first label: no step, no acceptance test
+                    tier_of(g, P, tmode, S, cap, t);
+                    a = P.na;
+                    in = P.pin;
+                    cell = P.na;
+                    px = P.qx;
+                    py = P.qy;
+                    pz = P.qz;
+                    ox = -P.qx;
+                    oy = -P.qy;
+                    oz = -P.qz;
+                    if (in) {
+                        if (occ_in_lds) atomicAdd(&occ[0], 1);
+                        else atomicAdd((unsigned long long*)&W->occupancy[0], 1ull);
+                    }
+                    init = 0;
+                    phase = p.n_steps > 0 ? kStep : kDone;
+                } else if (complete) {
+                    int m = 0;
+                    if (in != P.pin) m = 1;
+                    else if (in && P.pin && cell != pcell) m = 2;
+                    bool accept = true;
+                    if (m > 0) {
+                        ++crossings;
+                        if ((rj >> (m - 1)) & 1) {
+                            accept = false;
+                            ++rejects;
+                        }
+                    }
+                    if (accept) {
+                        if (tmode != 0) tier_of(g, P, tmode, S, cap, t);
+                        a = P.na;
+                        in = P.pin;
+                        cell = pcell;
+                        px = P.qx;
+                        py = P.qy;
+                        pz = P.qz;
+                        ox = P.ox;
+                        oy = P.oy;
+                        oz = P.oz;
+                    }
+                    const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
+                    const float yx = Dx - Cx, tx = Sx + yx;
+                    Cx = (tx - Sx) - yx;
+                    Sx = tx;
+                    const float yy = Dy - Cy, ty = Sy + yy;
+                    Cy = (ty - Sy) - yy;
+                    Sy = ty;
+                    const float yz = Dz - Cz, tz = Sz + yz;
+                    Cz = (tz - Sz) - yz;
+                    Sz = tz;
+                    ++step;
+                    if (--until_sample == 0) {
+                        until_sample = p.steps_per_h;
+                        const int j = ++j_sample;
+                        if (in) {
+                            if (occ_in_lds) atomicAdd(&occ[j], 1);
+                            else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
+                        }
+                        const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
+                        const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
+                        const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+                        if (occ_in_lds) {
+#pragma unroll
+                            for (int u = 0; u < NP; ++u) {
+                                const float sg = L.evc[j][u];
+                                dM[u][0] = fmaf(sg, Yx, dM[u][0]);
+                                dM[u][1] = fmaf(sg, Yy, dM[u][1]);
+                                dM[u][2] = fmaf(sg, Yz, dM[u][2]);
+                            }
+                        } else {
+                            while (next_ev < ev.n && ev.j[next_ev] == j) {
+                                const int q = ev.pair[next_ev];
+                                const float sg = (float)ev.sign[next_ev];
+#pragma unroll
+                                for (int u = 0; u < NP; ++u)
+                                    if (u == q) {
+                                        dM[u][0] += sg * Yx;
+                                        dM[u][1] += sg * Yy;
+                                        dM[u][2] += sg * Yz;
+                                    }
+                                ++next_ev;
+                            }
+                        }
+                    }
+                    phase = step >= p.n_steps ? kDone : kStep;
+                }
+                if (phase == kDone) {
+                    // the walker's moments and active flag, by its index in the entry
+                    const int64_t ew = walker_lo + w;
+                    float* m = W->moments + ew * (int64_t)(n_pairs * 3);
+#pragma unroll
+                    for (int u = 0; u < NP; ++u)
+                        if (u < n_pairs) {
+                            m[3 * u] = dM[u][0];
+                            m[3 * u + 1] = dM[u][1];
+                            m[3 * u + 2] = dM[u][2];
+                        }
+                    W->active[ew] = escaped ? 0 : 1;
+                    n_esc += escaped;
+                    n_err += err;
+                    phase = kIdle;
+                }
+            }
+            // item done: its inside counts and flags
+            if (occ_in_lds) {
+                wave_lds_fence();
+                for (int j = lane; j < p.n_grid; j += 64) {
+                    const int v = occ[j];
+                    if (v) atomicAdd((unsigned long long*)&W->occupancy[j], (unsigned long long)v);
+                    occ[j] = 0;
+                }
+                wave_lds_fence();
+            }
+            {
+                const double ne = wave_sum_f64((double)n_esc), nr = wave_sum_f64((double)n_err);
+                if (lane == 0) {
+                    if (ne > 0.0) atomicAdd((unsigned long long*)&W->flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
+                    if (nr > 0.0)
+                        atomicAdd((unsigned long long*)&W->flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
+                }
+            }
+            e = e + 1 == n_work ? 0 : e + 1;
+        }
+    }
+    facets += nfac;
+    if (counters) {
+        const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
+                     v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
+                     v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
+                     v6 = wave_sum_f64((double)rejects), v7 = wave_sum_f64((double)lane_iters);
+        if (lane == 0) {
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)v2);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)v3);
+            atomicAdd(&counters[MADI_CNT_FACETS], (unsigned long long)v4);
+            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
+            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
+            atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
+            atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
+        }
+    }
+}
+
 // One entry: block b runs tile b.
 // Classification variants (M): 0 per-lane state machine with R rounds per
 // iteration, 1 wave-cooperative lock step.
@@ -1336,17 +1721,132 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     }
 }
 
-// Sum tile slots [tile_lo, tile_lo + n_tiles) per column in tile order.
-__global__ void k_finalize_tiles(const double* __restrict__ partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols,
-                                 double* __restrict__ cos_out, double* __restrict__ sin_out)
+// Streamed walk over a batch (see stream_wave).  The event table is the
+// batch's, so every workgroup builds it once.
+template <int NP, int R>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stream_f32(
+    const WorkDev* __restrict__ work, unsigned long long* __restrict__ item_next, const BatchHdr* __restrict__ hdr,
+    const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
+    unsigned long long* counters)
+{
+    __shared__ StreamLds<NP> lds;
+    for (int j = threadIdx.x; j < kMaxLdsGrid; j += kBlock)
+#pragma unroll
+        for (int u = 0; u < NP; ++u) lds.evc[j][u] = 0.f;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int e = 0; e < ev.n; ++e)
+            if (ev.j[e] < kMaxLdsGrid) lds.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
+    __syncthreads();
+    stream_wave<NP, R>(work, item_next, hdr, *plan, jump_pow, ev, cap_width, counters, lds);
+}
+
+// Column sums of every tile of a streamed batch from its walkers' stored
+// moments, bit for bit those of emit_columns: per column, each group of 64
+// walkers is summed in FP64 as wave_sum_f64's butterfly does (lane l with
+// l ^ 32 first, then with l ^ 16, ...: the pairwise tree over the walkers
+// taken in bit-reversed order of their index in the group), and the four
+// group sums are added to 0.0 in group order.  Here a thread owns whole
+// columns and forms that tree with a carry stack, so the sums need no
+// cross-lane traffic or barriers.
+template <int NP>
+__global__ __launch_bounds__(kBlock) void k_emit_tiles(const WorkDev* __restrict__ work, const BatchHdr* hdr,
+                                                       const PlanDev* __restrict__ plan)
+{
+    __shared__ float mom[kBlock][3 * NP];
+    __shared__ uint8_t act[kBlock];
+    const int64_t total = hdr->total;
+    const int32_t n_work = hdr->n_work;
+    const PlanDev& P = *plan;
+    const int n_pairs = P.n_pairs, n_b = P.n_b, n_dirs = P.n_dirs, n_cols = P.n_cols;
+    for (int64_t t = blockIdx.x; t < total; t += gridDim.x) {
+        int lo = 0, hi = n_work - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (work[mid].work_begin <= t) lo = mid;
+            else hi = mid - 1;
+        }
+        const WorkDev& W = work[lo];
+        const int64_t tile = W.tile_lo + (t - W.work_begin);
+        {
+            const int64_t w = tile * kBlock + threadIdx.x;
+            const bool live = w < W.w.n_walkers;
+            act[threadIdx.x] = live ? W.active[w] : 0;
+            const float* m = W.moments + (live ? w : 0) * (int64_t)(n_pairs * 3);
+#pragma unroll
+            for (int u = 0; u < NP; ++u)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) mom[threadIdx.x][3 * u + c] = (live && u < n_pairs) ? m[3 * u + c] : 0.f;
+        }
+        __syncthreads();
+        double* slot = W.partial + (size_t)tile * n_cols * 2;
+        for (int col = threadIdx.x; col < n_cols; col += kBlock) {
+            const int d = col % n_dirs, pb = col / n_dirs, b = pb % n_b, p = pb / n_b;
+            double a_c = 0.0, a_s = 0.0;
+            for (int grp = 0; grp < kWaves; ++grp) {
+                double st_c[7], st_s[7];
+#pragma unroll
+                for (int k = 0; k < 64; ++k) {
+                    // bit-reversed 6-bit index: the butterfly's pairing order
+                    const int l = ((k & 1) << 5) | ((k & 2) << 3) | ((k & 4) << 1) | ((k & 8) >> 1) |
+                                  ((k & 16) >> 3) | ((k & 32) >> 5);
+                    const int wi = grp * 64 + l;
+                    float sv, cv;
+                    sincos_f32(column_phase_f32(P, p, b, d, mom[wi][3 * p], mom[wi][3 * p + 1], mom[wi][3 * p + 2]),
+                               sv, cv);
+                    double vc = act[wi] ? (double)cv : 0.0, vs = act[wi] ? (double)sv : 0.0;
+                    int lvl = 0;
+#pragma unroll
+                    for (int q = k; q & 1; q >>= 1, ++lvl) {
+                        vc = st_c[lvl] + vc;
+                        vs = st_s[lvl] + vs;
+                    }
+                    st_c[lvl] = vc;
+                    st_s[lvl] = vs;
+                }
+                a_c += st_c[6];
+                a_s += st_s[6];
+            }
+            slot[2 * col] = a_c;
+            slot[2 * col + 1] = a_s;
+        }
+        __syncthreads();
+    }
+}
+
+// Per column, an entry's tile slots are summed in a fixed order: chunks of
+// kFinalChunk consecutive tiles from 0.0 in tile order, then the chunk sums
+// from 0.0 in chunk order.  Pass 1: one thread per (chunk, column).
+constexpr int kFinalChunk = 32;
+
+__global__ void k_finalize_chunks(const double* __restrict__ partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols,
+                                  int64_t n_chunks, double* __restrict__ chunk_out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_chunks * n_cols) return;
+    const int64_t q = i / n_cols;
+    const int col = (int)(i - q * n_cols);
+    const int64_t t0 = q * kFinalChunk, t1 = t0 + kFinalChunk < n_tiles ? t0 + kFinalChunk : n_tiles;
+    double a = 0.0, b = 0.0;
+    for (int64_t t = t0; t < t1; ++t) {
+        const double* s = partial + ((size_t)(tile_lo + t) * n_cols + col) * 2;
+        a += s[0];
+        b += s[1];
+    }
+    chunk_out[(q * n_cols + col) * 2] = a;
+    chunk_out[(q * n_cols + col) * 2 + 1] = b;
+}
+
+// Pass 2: one thread per column adds the chunk sums in chunk order.
+__global__ void k_finalize_sum(const double* __restrict__ chunk, int64_t n_chunks, int32_t n_cols,
+                               double* __restrict__ cos_out, double* __restrict__ sin_out)
 {
     const int col = blockIdx.x * blockDim.x + threadIdx.x;
     if (col >= n_cols) return;
     double a = 0.0, b = 0.0;
-    for (int64_t t = 0; t < n_tiles; ++t) {
-        const double* s = partial + ((size_t)(tile_lo + t) * n_cols + col) * 2;
-        a += s[0];
-        b += s[1];
+    for (int64_t q = 0; q < n_chunks; ++q) {
+        a += chunk[(q * n_cols + col) * 2];
+        b += chunk[(q * n_cols + col) * 2 + 1];
     }
     cos_out[col] = a;
     sin_out[col] = b;
@@ -1406,12 +1906,44 @@ int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, co
     return check_launch("k_batch_f32");
 }
 
+int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
+                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                      unsigned long long* counters, hipStream_t s)
+{
+#define MADI_LAUNCH_STREAM(NPV, RV, CV)                                                                             \
+    hipLaunchKernelGGL((k_stream_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, item_next, hdr, plan, ev,    \
+                       jump_pow, cap_width, counters)
+    MADI_DISPATCH_NP(ev.n_pairs, rounds, 0, MADI_LAUNCH_STREAM);
+#undef MADI_LAUNCH_STREAM
+    return check_launch("k_stream_f32");
+}
+
+int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,
+                      hipStream_t s)
+{
+    if (n_pairs <= 2) hipLaunchKernelGGL((k_emit_tiles<2>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan);
+    else if (n_pairs <= 4) hipLaunchKernelGGL((k_emit_tiles<4>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan);
+    else hipLaunchKernelGGL((k_emit_tiles<MADI_MAX_PAIRS>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan);
+    return check_launch("k_emit_tiles");
+}
+
+int64_t finalize_scratch_doubles(int64_t n_tiles, int32_t n_cols)
+{
+    return ((n_tiles + kFinalChunk - 1) / kFinalChunk) * (int64_t)n_cols * 2;
+}
+
 int launch_finalize_tiles(const double* partial, int64_t tile_lo, int64_t n_tiles, int32_t n_cols, double* cos_out,
-                          double* sin_out, hipStream_t s)
+                          double* sin_out, double* scratch, hipStream_t s)
 {
     if (n_cols == 0) return MADI_OK;
-    hipLaunchKernelGGL(k_finalize_tiles, dim3((n_cols + 127) / 128), dim3(128), 0, s, partial, tile_lo, n_tiles, n_cols,
-                       cos_out, sin_out);
+    const int64_t n_chunks = (n_tiles + kFinalChunk - 1) / kFinalChunk;
+    if (n_chunks > 0) {
+        const int64_t n = n_chunks * n_cols;
+        hipLaunchKernelGGL(k_finalize_chunks, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, partial, tile_lo,
+                           n_tiles, n_cols, n_chunks, scratch);
+    }
+    hipLaunchKernelGGL(k_finalize_sum, dim3((n_cols + 127) / 128), dim3(128), 0, s, scratch, n_chunks, n_cols, cos_out,
+                       sin_out);
     return check_launch("k_finalize_tiles");
 }
 

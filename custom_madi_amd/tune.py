@@ -3,7 +3,8 @@
 The knobs it searches change only how the GPU does the work, never the
 result (the batch tests check bitwise equality across them):
 
-* ``classify_mode``: 1 per-lane state machine, 2 wave-cooperative lock step;
+* ``classify_mode``: 1 per-lane state machine per tile, 2 wave-cooperative
+  lock step per tile, 3 per-lane state machine with lane refill;
 * ``scan_rounds``: classification rounds per state-machine iteration (1, 2);
 * ``grid_per_cu``: persistent workgroups per CU of the batch kernel.
 
@@ -29,9 +30,9 @@ from typing import Iterable, List, Optional
 from . import _hip, engine
 
 DEFAULT_CANDIDATES = [
-    (1, 2, 3), (1, 2, 4), (1, 2, 6),
-    (1, 1, 4),
-    (2, 1, 3), (2, 1, 4),
+    (3, 1, 3), (3, 1, 4), (3, 2, 3),
+    (1, 2, 4), (1, 2, 6),
+    (2, 1, 4),
 ]  # (classify_mode, scan_rounds, grid_per_cu)
 
 

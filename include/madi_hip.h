@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 1
+#define MADI_ABI_VERSION 2
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -161,7 +161,8 @@ typedef struct {
     uint64_t seed;          /* walk seed (device RNG) */
     uint64_t first_walker;  /* RNG subsequence index of walker 0 */
     int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-2 (0 = 2) */
-    int32_t classify_mode;  /* FP32 walk: 0 default, 1 per-lane state machine, 2 wave-cooperative lock step */
+    int32_t classify_mode;  /* FP32 walk: 0 default (3), 1 per-lane state machine per tile, 2 wave-cooperative
+                               lock step per tile, 3 per-lane state machine with lane refill (streamed) */
 } madi_walk_params;
 
 /* Reference-semantics FP64 walk with the exact classifier at every endpoint.
@@ -198,12 +199,19 @@ typedef struct {
 } madi_phase_plan;
 
 /* Bytes of scratch the fused walk needs for n_walkers (fixed-slot per-tile
- * partial sums). */
+ * partial sums, the streamed variant's per-walker moments and its one-item
+ * work table; any classify_mode). */
 int madi_walk_fused_workspace(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes);
+
+/* Bytes of an entry's per-walker moment buffer (madi_work.d_moments):
+ * n_walkers x n_pairs x 3 FP32 moments followed by n_walkers active flags.
+ * No reference counterpart: the streamed variant hands walkers to lanes in
+ * any order and forms the tile sums from this buffer afterwards. */
+int madi_moments_bytes(int64_t n_walkers, const madi_phase_plan* plan, int64_t* bytes);
 
 /* Production walk with the phase reduction fused in: FP32 seed-relative
  * state, Delaunay-graph classifier certified in FP64, compensated Y, FP64
- * cross-walker sums in fixed tile order.  precision 32 selects this kernel;
+ * cross-walker sums in a fixed order.  precision 32 selects this kernel;
  * precision 64 runs the same fused reduction on the FP64 exact-classifier
  * walk (the FP64 oracle run on the device).  Replaces the walk + reduce
  * pair of _walk_and_reduce_one_ensemble (madi/walker_gpu.py:989-1063) for
@@ -233,6 +241,7 @@ typedef struct {
     double* d_partial;
     int64_t* d_occupancy;
     int64_t* d_flags;
+    void* d_moments;  /* the entry's madi_moments_bytes buffer; required by classify_mode 3, else unused */
 } madi_work;
 
 typedef struct {
@@ -247,8 +256,11 @@ int madi_batch_workspace(int32_t n_work, int64_t* bytes);
 
 /* One persistent, work-stealing launch of the FP32 production walk over many
  * entries of uneven size: workgroups take tiles from an atomic counter until
- * every tile of every item is done.  Each tile writes its own slot, so an
- * entry's result is bitwise identical alone or in any mix.  Replaces the
+ * every tile of every item is done (classify_mode 1, 2), or waves take
+ * walkers from per-item counters as lanes free up and a second kernel forms
+ * the tile sums (classify_mode 3).  Each tile writes its own slot, so an
+ * entry's result is bitwise identical alone or in any mix, and equal across
+ * classify modes.  Replaces the
  * one-entry-at-a-time loop of build_library_from_triplets
  * (madi/library.py:546-629) with one launch per batch or per serving
  * generation.  Asynchronous on stream. */
@@ -271,8 +283,9 @@ int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const
 int madi_batch_graph_launch(madi_batch_graph* graph, const madi_work* work, int32_t n_work, void* stream);
 int madi_batch_graph_destroy(madi_batch_graph* graph);
 
-/* Sum an entry's tile slots [0, n_tiles) in tile order into d_cos / d_sin
- * (n_cols FP64 each). */
+/* Sum an entry's tile slots [0, n_tiles) into d_cos / d_sin (n_cols FP64
+ * each) in a fixed order: chunks of 32 consecutive tiles in tile order, then
+ * the chunk sums in chunk order (the order madi_walk_fused uses). */
 int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
                         double* d_sin, void* stream);
 

@@ -159,15 +159,15 @@ def test_dropin_fp64_fused_equals_y_reduce_path():
 
 def test_tuner_choices_give_identical_results(spec):
     """The classification mode (per-lane state machine with 1 or 2 rounds per
-    iteration, wave-cooperative lock step) changes only
-    where and when a proposal is classified, never the result: every choice
-    must give the same sums bit for bit, in the single-entry and the
-    persistent batch kernels."""
+    iteration, wave-cooperative lock step, state machine with lane refill)
+    changes only where and when a proposal is classified and which lane walks
+    a walker, never the result: every choice must give the same sums bit for
+    bit, in the single-entry and the persistent batch kernels."""
     req = batch.library_requests(1, seed=31, walkers=(20_000, 20_000), cells=(200, 200), T_max_ms=12.0)[0]
     prep = batch.prepare_entry(req)
     geom = DeviceGeometry(prep.arrays)
     ref = None
-    for rounds, cw in ((1, 1), (2, 1), (1, 2)):
+    for rounds, cw in ((1, 1), (2, 1), (1, 2), (1, 3), (2, 3)):
         if True:
             say(f"single entry: scan_rounds={rounds} classify_mode={cw}")
             p = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed, scan_rounds=rounds,
@@ -179,7 +179,7 @@ def test_tuner_choices_give_identical_results(spec):
             else:
                 assert np.array_equal(res.cos, ref.cos) and np.array_equal(res.sin, ref.sin)
                 assert np.array_equal(res.occupancy, ref.occupancy)
-    for cw in (1, 2):
+    for cw in (1, 2, 3):
         say(f"batch: classify_mode={cw}")
         out = batch.run_batched([prep], spec, classify_mode=cw)[0]
         assert np.array_equal(out.cos, ref.cos) and np.array_equal(out.occupancy, ref.occupancy)

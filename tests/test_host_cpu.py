@@ -293,3 +293,34 @@ def test_hip_library_exports_every_declared_symbol():
     for name in sorted(declared):
         assert hasattr(lib, name), name
     assert set(_hip.SIGNATURES) >= declared
+
+
+def test_ctypes_structs_match_the_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and field offsets."""
+    import ctypes
+    import shutil
+    import subprocess
+    from custom_madi_amd import _hip
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    pairs = {"madi_device_info": _hip.DeviceInfo, "madi_geom_desc": _hip.GeomDesc, "madi_walk_params": _hip.WalkParams,
+             "madi_phase_plan": _hip.PhasePlan, "madi_work": _hip.Work, "madi_tune": _hip.Tune}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "madi_hip.h"', "int main(void) {"]
+    for cname, py in pairs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-I", str(ROOT / "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        name, key, val = ln.split()
+        got[(name, key)] = int(val)
+    for cname, py in pairs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
