@@ -308,8 +308,8 @@ def main(argv=None):
     ap.add_argument("--scan-rounds", type=int, default=0,
                     help="FP32 walk: classification rounds per iteration, 1-2 (0 default)")
     ap.add_argument("--classify-mode", type=int, default=0,
-                    help="FP32 walk: 1 per-lane state machine per tile, 2 wave-cooperative lock step per tile, "
-                         "3 state machine with lane refill (0 default)")
+                    help="FP32 walk: 1 per-lane state machine per tile, 3 state machine with lane refill, "
+                         "4 walker waves with classifier waves (0 default)")
     ap.add_argument("--gen-tiles", type=int, default=65536)
     ap.add_argument("--no-tune", action="store_true", help="c2/c3/c4: default knobs instead of the autotuner")
     ap.add_argument("--fp64-entries", type=int, default=16, help="c4: entries checked against the FP64 walk")

@@ -358,16 +358,21 @@ static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullpt
 
 static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
 // FP32 walk classification mode -> kernel variant (0 per-lane state machine
-// per tile, 1 wave-cooperative lock step per tile, kVariantStream per-lane
-// state machine with lane refill); 0 picks the default, the streamed one.
-constexpr int kVariantStream = 2;
-static int variant_of(int mode) { return mode == 1 ? 0 : (mode == 2 ? 1 : kVariantStream); }
+// per tile, kVariantStream per-lane state machine with lane refill,
+// kVariantSplit walker and classifier waves); 0 picks the default, the
+// split one.  Mode 2 (a wave-cooperative lock-step variant) was removed.
+constexpr int kVariantStream = 2, kVariantSplit = 3;
+static int variant_of(int mode) { return mode == 1 ? 0 : (mode == 3 ? kVariantStream : kVariantSplit); }
+// Variants that hand walkers to lanes one at a time (moments + tile emission).
+static bool streamed(int variant) { return variant == kVariantStream || variant == kVariantSplit; }
 // FP32 walk classification rounds per iteration (1-2); 0 picks the
 // variant's default (1 streamed, where a round costs every lane of the wave
-// but a walker that waits costs only its own lane; 2 per tile).
+// but a walker that waits costs only its own lane; 2 per tile).  For the
+// split variant the knob selects the workgroup's shape instead: 1 = three
+// walker waves and one classifier wave, 2 = two and two.
 static int rounds_of(int r, int variant)
 {
-    if (r <= 0) return variant == kVariantStream ? 1 : 2;
+    if (r <= 0) return streamed(variant) ? 1 : 2;
     return r > 2 ? 2 : r;
 }
 
@@ -442,8 +447,9 @@ static int make_walk(const madi_walk_params* p, WalkDev& W)
         set_error("p_p must be finite and in [0, 1], got %g", p->pp);
         return MADI_ERR_ARG;
     }
-    if (p->classify_mode < 0 || p->classify_mode > 3 || p->scan_rounds < 0 || p->scan_rounds > 2) {
-        set_error("walk params: classify_mode must be 0-3 and scan_rounds 0-2, got %d and %d", p->classify_mode,
+    if (p->classify_mode < 0 || p->classify_mode > 4 || p->classify_mode == 2 || p->scan_rounds < 0 ||
+        p->scan_rounds > 2) {
+        set_error("walk params: classify_mode must be 0, 1, 3 or 4 and scan_rounds 0-2, got %d and %d", p->classify_mode,
                   p->scan_rounds);
         return MADI_ERR_ARG;
     }
@@ -524,6 +530,12 @@ struct BatchKnobs {
 
 static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, BatchKnobs& k)
 {
+    if (tune && (tune->classify_mode < 0 || tune->classify_mode > 4 || tune->classify_mode == 2 ||
+                 tune->scan_rounds < 0 || tune->scan_rounds > 2)) {
+        set_error("madi_tune: classify_mode must be 0, 1, 3 or 4 and scan_rounds 0-2, got %d and %d",
+                  tune->classify_mode, tune->scan_rounds);
+        return MADI_ERR_ARG;
+    }
     k.cap_width = cap_width_of(tune && tune->tier_cap_um > 0.f ? tune->tier_cap_um : (first ? first->tier_cap_um : 0.f));
     k.variant = variant_of(tune && tune->classify_mode > 0 ? tune->classify_mode : (first ? first->classify_mode : 0));
     k.rounds = rounds_of(tune && tune->scan_rounds > 0 ? tune->scan_rounds : (first ? first->scan_rounds : 0),
@@ -610,7 +622,8 @@ static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_p
         D.flags = (long long*)w.d_flags;
         total += w.n_tiles;
         if (stream && !w.d_moments) {
-            set_error("%s: work item %d has no moment buffer (classify_mode 3 needs madi_work.d_moments)", who, i);
+            set_error("%s: work item %d has no moment buffer (classify_mode 3 and 4 need madi_work.d_moments)", who,
+                      i);
             return MADI_ERR_ARG;
         }
         const int64_t hi = std::min<int64_t>(w.walk.n_walkers, (w.tile_lo + w.n_tiles) * kBlock);
@@ -631,9 +644,11 @@ static int launch_batch(const WorkDev* table, unsigned long long* item_next, Bat
                         const Events& ev, const Gf2Col* jp, const BatchKnobs& k, int32_t grid,
                         unsigned long long* counters, hipStream_t s)
 {
-    if (k.variant != kVariantStream)
-        return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, k.variant, grid, counters, s);
-    int rc = launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
+    if (!streamed(k.variant))
+        return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
+    int rc = k.variant == kVariantSplit
+                 ? launch_split_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s)
+                 : launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
     if (rc) return rc;
     return launch_emit_tiles(table, hdr, plan, ev.n_pairs, k.emit_grid, s);
 }
@@ -866,7 +881,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     if ((rc = get_jump_pow(&jp))) return rc;
     const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
     const int variant = variant_of(p->classify_mode);
-    if (precision == 32 && variant == kVariantStream && tiles > 0) {
+    if (precision == 32 && streamed(variant) && tiles > 0) {
         // one work item covering the entry
         madi_work item;
         memset(&item, 0, sizeof item);
@@ -892,7 +907,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         madi_tune tn;
         memset(&tn, 0, sizeof tn);
         tn.scan_rounds = rounds_of(p->scan_rounds, variant);
-        tn.classify_mode = 3;
+        tn.classify_mode = variant == kVariantSplit ? 4 : 3;  // the resolved variant
         tn.tier_cap_um = p->tier_cap_um;
         BatchKnobs k;
         if ((rc = batch_knobs(&tn, p, k))) return rc;
@@ -911,7 +926,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     fo.flags = (long long*)d_flags;
     if (precision == 32)
         rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds, variant),
-                                   variant == kVariantStream ? 0 : variant, (unsigned long long*)d_counters, s);
+                                   (unsigned long long*)d_counters, s);
     else
         rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
     if (rc) return rc;
@@ -1003,7 +1018,7 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     if ((rc = batch_knobs(tune, &work[0].walk, k))) return rc;
     int64_t total = 0, walkers = 0;
     if ((rc = batch_table(work, n_work, plan, (WorkDev*)(img.data() + batch_off_table()), total,
-                          "madi_batch_launch", k.variant == kVariantStream, &walkers)))
+                          "madi_batch_launch", streamed(k.variant), &walkers)))
         return rc;
     BatchHdr* h = (BatchHdr*)(img.data() + batch_off_hdr());
     h->queue = 0;
@@ -1016,7 +1031,7 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     hipStream_t s = (hipStream_t)stream;
     char* base = (char*)d_workspace;
     HIP_TRY(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, s));
-    const int grid = (int)std::min<int64_t>(k.grid, k.variant == kVariantStream ? (walkers + 63) / 64 : total);
+    const int grid = (int)std::min<int64_t>(k.grid, streamed(k.variant) ? (walkers + 63) / 64 : total);
     return launch_batch((WorkDev*)(base + batch_off_table()), (unsigned long long*)(base + batch_off_next(n_work)),
                         (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base, ev, jp, k, grid,
                         (unsigned long long*)d_counters, s);
@@ -1115,7 +1130,7 @@ int madi_batch_graph_launch(madi_batch_graph* gr, const madi_work* work, int32_t
     if (gr->launched) HIP_TRY(hipEventSynchronize(gr->done));
     int64_t total = 0, walkers = 0;
     int rc = batch_table(work, n_work, &gr->plan, (WorkDev*)(gr->h_buf + batch_off_table()), total,
-                         "madi_batch_graph_launch", gr->knobs.variant == kVariantStream, &walkers);
+                         "madi_batch_graph_launch", streamed(gr->knobs.variant), &walkers);
     if (rc) return rc;
     BatchHdr* h = (BatchHdr*)(gr->h_buf + batch_off_hdr());
     h->queue = 0;

@@ -91,13 +91,16 @@ int launch_reduce_f64(const double* Y, int64_t n, int32_t G, const int32_t* jd, 
 int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_t* cell, int8_t* inside, int8_t* ok,
                           unsigned long long* counters, hipStream_t s);
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
-                          float cap_width, int rounds, int variant, unsigned long long* counters, hipStream_t s);
+                          float cap_width, int rounds, unsigned long long* counters, hipStream_t s);
 int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
-                     float cap_width, int rounds, int variant, int32_t grid, unsigned long long* counters,
-                     hipStream_t s);
+                     float cap_width, int rounds, int32_t grid, unsigned long long* counters, hipStream_t s);
 int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
                       const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
                       unsigned long long* counters, hipStream_t s);
+// shape 1: 3 walker waves + 1 classifier wave per workgroup; 2: 2 + 2
+int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, int32_t grid,
+                     unsigned long long* counters, hipStream_t s);
 int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,
                       hipStream_t s);
 // Column sums of an entry's tile slots (see k_finalize_chunks for the order);

@@ -160,6 +160,7 @@ __device__ __forceinline__ Refresh locate_f32(const GeomDev& g, int& a, float& p
 // the tier was built, relative to the anchor.
 struct Tier {
     float ux, uy, uz, h;           // plane of edge a->k, unit normal towards k
+    float ux2, uy2, uz2, h2;       // a's plane of second-smallest margin (never crossed: 0, 0, 0, >= 0)
     float cx, cy, cz;              // centre
     float ra_in2, ra_out2;         // squared radii on a's side: inside / outside answers
     float rk_in2, rk_out2;         // the same on k's side (0 = no k-side answers)
@@ -173,6 +174,8 @@ __device__ __forceinline__ void tier_clear(Tier& t)
 {
     t.ux = t.uy = t.uz = 0.f;
     t.h = 0.f;
+    t.ux2 = t.uy2 = t.uz2 = 0.f;
+    t.h2 = 0.f;
     t.cx = t.cy = t.cz = 0.f;
     t.ra_in2 = t.ra_out2 = t.rk_in2 = t.rk_out2 = t.rout2 = 0.f;
     t.alpha_a = t.alpha_k = 0.f;
@@ -191,14 +194,18 @@ __device__ __forceinline__ void tier_free_water(Tier& t)
 
 // Tier answer for q (relative to the anchor a); returns 1 when certified.
 //
-// Let x be the centre, m2 the second-smallest margin of a's planes at x and
-// n2 the second-smallest of k's planes at x (both capped).  Margins change by
-// at most |q - x|, so for |q - x| < m2 every plane of a other than a->k keeps
-// a positive margin: q lies in a's Voronoi cell iff s = h - q.u >= 0, and is
-// then outside the contracted cell iff s < alpha_a; inside needs every other
-// plane at alpha_a or more as well, i.e. |q - x| < m2 - alpha_a.  The same
-// holds on k's side with n2 and alpha_k.  An outside answer therefore has the
-// larger radius m2 (n2): walkers in the extracellular gap, whose second plane
+// Let x be the centre, m1 <= m2 <= m3 the smallest margins of a's planes at x
+// (planes 1 = a->k and 2 kept in the tier; the radii use a lower bound of m3)
+// and n2 the second-smallest margin of k's planes at x (all capped).
+// Margins change by at most |q - x|, so for |q - x| < m3 every plane of a
+// other than planes 1 and 2 keeps a positive margin: q lies in a's Voronoi
+// cell iff s1 = h - q.u >= 0 and s2 = h2 - q.u2 >= 0, and is then outside the
+// contracted cell iff min(s1, s2) < alpha_a; inside needs every other plane
+// at alpha_a or more as well, i.e. |q - x| < m3 - alpha_a.  Across plane 1 the
+// same holds on k's side with n2 and alpha_k (q is in Vor(k) once it is on
+// k's side of the a-k bisector and within n2 of x, whatever a's other planes
+// say); across plane 2 alone there is no answer.  An outside answer has the
+// larger radius m3 (n2): walkers in the extracellular gap, whose next plane
 // is often closer than alpha + a step, keep their tier much longer.
 //
 // The outside ball (rout2): every contracted cell j is the Voronoi cell of j
@@ -210,6 +217,10 @@ __device__ __forceinline__ void tier_free_water(Tier& t)
 // within the smallest of these distances of x is outside every cell,
 // whichever seed is nearest; near Voronoi edges, where m2 is small, this
 // ball is the larger certificate.
+//
+// Rounding: a decision closer than eps to 0 or alpha is left to a full
+// classification, except that both sides of a bisector are outside there
+// (alpha > eps on each side), which needs no decision.
 __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float qy, float qz, int& cell, int& inside)
 {
     const float dx = qx - t.cx, dy = qy - t.cy, dz = qz - t.cz;
@@ -221,8 +232,10 @@ __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float 
     }
     const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
     if (s >= 0.f) {
-        const int ins = s >= t.alpha_a;
-        if (fabsf(s - t.alpha_a) <= t.eps || !(d2 < (ins ? t.ra_in2 : t.ra_out2))) return 0;
+        const float s2 = t.h2 - (qx * t.ux2 + qy * t.uy2 + qz * t.uz2);
+        const float sm = fminf(s, s2);
+        const int ins = sm >= t.alpha_a;
+        if (s2 < 0.f || fabsf(sm - t.alpha_a) <= t.eps || !(d2 < (ins ? t.ra_in2 : t.ra_out2))) return 0;
         cell = a;
         inside = ins;
         return 1;
@@ -333,8 +346,8 @@ __device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 // `cell`) over the cell's edges in order of half length.  cur < 0: the scan
 // of (cell, s) is set up by scan_begin on its first group.
 struct ScanState {
-    int cur, end, c, cell;
-    float m1, m2, pe, eps;
+    int cur, end, c, c2, cell;
+    float m1, m2, m3, xb, pe, eps;  // xb: lower bound of every unvisited edge's margin
     float nx, ny, nz;  // -s
 };
 
@@ -350,8 +363,8 @@ __device__ __forceinline__ void scan_begin(const GeomDev& g, ScanState& S)
     const int cell = S.cell;
     S.cur = gload(g.nb_off + cell);
     S.end = gload(g.nb_off + cell + 1);
-    S.c = -1;
-    S.m1 = S.m2 = __builtin_inff();
+    S.c = S.c2 = -1;
+    S.m1 = S.m2 = S.m3 = S.xb = __builtin_inff();
     S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + gload(g.cell_hmax + cell));
     const float n2 = fmaf(S.nx, S.nx, fmaf(S.ny, S.ny, S.nz * S.nz));
     S.pe = fmaf(__builtin_amdgcn_sqrtf(n2), 1.f + 0x1.0p-20f, S.eps);
@@ -380,23 +393,33 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
     float4 r[kEdgePad];
 #pragma unroll
     for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
-    int cl = -1;
+    // chunk-local codes: cl -1 = the previous c; cl2 -2 = the previous c2,
+    // -1 = the previous c
+    int cl = -1, cl2 = -2;
 #pragma unroll
     for (int u = 0; u < kEdgePad; ++u) {
         const float m = fmaf(S.nx, r[u].x, fmaf(S.ny, r[u].y, fmaf(S.nz, r[u].z, r[u].w)));
-        cl = m < S.m1 ? u : cl;
+        const bool lt1 = m < S.m1, lt2 = m < S.m2;
+        cl2 = lt1 ? cl : (lt2 ? u : cl2);
+        cl = lt1 ? u : cl;
+        S.m3 = fminf(S.m3, fmaxf(S.m2, m));
         S.m2 = fminf(S.m2, fmaxf(S.m1, m));
         S.m1 = fminf(S.m1, m);
     }
+    S.c2 = cl2 == -2 ? S.c2 : (cl2 == -1 ? S.c : S.cur + cl2);
     if (cl >= 0) S.c = S.cur + cl;
     facets += kEdgePad;
     S.cur += kEdgePad;
-    if (r[kEdgePad - 1].w - S.pe >= fminf(S.m2, cap)) S.cur = S.end;
+    const float bound = r[kEdgePad - 1].w - S.pe;
+    if (bound >= fminf(S.m2, cap)) {
+        S.xb = bound;
+        S.cur = S.end;
+    }
 }
 
 struct Scan {
-    float m1, m2, eps;
-    int c;  // edge of m1 (-1: none below the cap)
+    float m1, m2, m3, eps;  // m3: a lower bound of the third-smallest margin
+    int c, c2;              // edges of m1 and m2 (-1: none)
 };
 
 // Whole scan of one cell.
@@ -406,33 +429,40 @@ __device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, 
     ScanState S;
     scan_start(g, cell, px, py, pz, S);
     while (S.cur < S.end) scan_chunk(g, cap, S, facets);
-    return Scan{S.m1, S.m2, S.eps, S.c};
+    return Scan{S.m1, S.m2, fminf(S.m3, S.xb), S.eps, S.c, S.c2};
 }
 
 // Tier from the scans at q: binding edge c of anchor a with smallest margin
-// m1 and a-side bound m2a; k-side bound n2 when the k side was scanned (an
-// outside point).  c < 0 means no edge below the
-// cap (every margin is at least cap: pass m2a = cap; the tier then answers
-// "inside a" up to cap - alpha_a); m2a = -inf gives a tier that answers
-// nothing.  One code path for every case, so the walk's update is branch-free.
-__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float qx, float qy, float qz, float m1,
-                                          float m2a, float epsa, bool kside, float n2, float epsk, Tier& t)
+// m1, edge c2 of the second-smallest margin m2a, a lower bound m3a of the
+// third-smallest; k-side bound n2 when the k side was scanned (an outside
+// point).  c < 0 means no edge below the cap (every margin is at least cap:
+// pass m2a = m3a = cap; the tier then answers "inside a" up to cap -
+// alpha_a); m2a = m3a = -inf gives a tier that answers nothing.  One code
+// path for every case, so the walk's update is branch-free.
+__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2, float qx, float qy, float qz,
+                                          float m1, float m2a, float m3a, float epsa, bool kside, float n2, float epsk,
+                                          Tier& t)
 {
-    const bool plane = c >= 0;
-    const int ci = plane ? c : 0;
-    const float4 e0 = gload(g.nb_rec + ci);
+    const bool plane = c >= 0, plane2 = plane && c2 >= 0;
+    const int ci = plane ? c : 0, ci2 = plane2 ? c2 : 0;
+    const float4 e0 = gload(g.nb_rec + ci), e20 = gload(g.nb_rec + ci2);
     const int k = plane ? gload(g.nb_id + ci) : a;
     const float4 e = plane ? e0 : make_float4(0.f, 0.f, 0.f, m2a);
+    const float4 e2 = plane2 ? e20 : make_float4(0.f, 0.f, 0.f, fmaxf(m3a, 0.f));
     const float aa = gload(g.cell_alpha + a), ak = gload(g.cell_alpha + k);
     kside = kside && plane;
     const float ep = kside ? fmaxf(epsa, epsk) : epsa;
-    const float ra_in = m2a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m2a - 2.f * ep;
+    const float ra_in = m3a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m3a - 2.f * ep;
     const float rk_in = kside ? n2 - fmaxf(ak, 0.f) - 2.f * ep : 0.f;
     const float rk_out = kside ? n2 - 2.f * ep : 0.f;
     t.ux = e.x;
     t.uy = e.y;
     t.uz = e.z;
     t.h = e.w;
+    t.ux2 = e2.x;
+    t.uy2 = e2.y;
+    t.uz2 = e2.z;
+    t.h2 = e2.w;
     t.cx = qx;
     t.cy = qy;
     t.cz = qz;
@@ -448,7 +478,7 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, float 
     const float rout =
         kside ? fminf(fminf(aa - m1, ak + m1), g.alpha_min + fmaxf(fminf(m2a, n2), 0.f)) - 2.f * ep : 0.f;
     t.rout2 = rout > 0.f ? rout * rout : 0.f;
-    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fabsf(e.w));
+    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fmaxf(fabsf(e.w), fabsf(e2.w)));
 }
 
 // Full classification of q from anchor a: greedy location, label, FP64
@@ -484,7 +514,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             return 1;
         }
         if (s.c < 0 || s.m1 >= cap) {
-            tier_make(g, a, -1, qx, qy, qz, 0.f, cap, s.eps, false, 0.f, 0.f, t);
+            tier_make(g, a, -1, -1, qx, qy, qz, 0.f, cap, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         const bool kside = !inside;
@@ -500,7 +530,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             n2 = fminf(sk.m2, cap);
             epsk = sk.eps;
         }
-        tier_make(g, a, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, kside, n2, epsk, t);
+        tier_make(g, a, s.c, s.c2, qx, qy, qz, s.m1, fminf(s.m2, cap), fminf(s.m3, cap), s.eps, kside, n2, epsk, t);
         return 1;
     }
     return 0;
@@ -514,28 +544,20 @@ struct TileOut {
 
 constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
 
-// One slot of a wave's classification queue: a request (anchor, point,
-// offset) replaced in place by its answer (anchor, moved point and offset,
-// status, and the new tier; the tier's centre is the point).
-struct SvcSlot {
-    float4 v[6];
-};
-
 // LDS a tile walk needs.
 template <int NP>
 struct TileLds {
     double cols[kWaves * kColChunk * 2];
     int occ[kMaxLdsGrid];
     float evc[kMaxLdsGrid][NP];  // per grid time: summed event signs per (delta, Delta) pair
-    SvcSlot svc[kBlock];  // classification requests and answers (wave-cooperative variant)
 };
 
 // A proposal under classification: point relative to anchor na, the
 // anchor's offset, and the a-side results kept for the tier.
 struct Pending {
     float qx, qy, qz, ox, oy, oz;
-    int na, pin, moves, sa_c;
-    float sa_m1, sa_m2, sa_eps;
+    int na, pin, moves, sa_c, sa_c2;
+    float sa_m1, sa_m2, sa_m3, sa_eps;
 };
 
 constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classification
@@ -543,8 +565,8 @@ constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classificatio
 __device__ __forceinline__ void pending_start(Pending& P, ScanState& S, int& phase)
 {
     P.moves = 0;
-    P.sa_c = -1;
-    P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
+    P.sa_c = P.sa_c2 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
     S.cell = P.na;
     S.nx = -P.qx;
     S.ny = -P.qy;
@@ -579,8 +601,10 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         const float a1 = gload(g.cell_alpha + P.na);
         P.pin = S.m1 >= a1;
         P.sa_c = S.c;
+        P.sa_c2 = S.c2;
         P.sa_m1 = S.m1;
         P.sa_m2 = fminf(S.m2, cap);
+        P.sa_m3 = fminf(fminf(S.m3, S.xb), cap);
         P.sa_eps = S.eps;
         if (fabsf(S.m1 - a1) <= S.eps) {
             ++rechecks;
@@ -631,9 +655,11 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
 __device__ __forceinline__ void tier_of(const GeomDev& g, const Pending& P, int tmode, const ScanState& S, float cap,
                                         Tier& t)
 {
-    const float m2a = tmode == 2 ? -__builtin_inff() : (tmode == 3 ? cap : P.sa_m2);
-    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, P.qx, P.qy, P.qz, P.sa_m1, m2a, P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps,
-              t);
+    const float none = -__builtin_inff();
+    const float m2a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m2);
+    const float m3a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m3);
+    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, tmode == 3 ? -1 : P.sa_c2, P.qx, P.qy, P.qz, P.sa_m1, m2a, m3a,
+              P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps, t);
 }
 
 // One tile of kBlock walkers [w_lo, w_lo + kBlock) of an entry: the whole
@@ -729,14 +755,14 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     Pending P;
     P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
     P.na = P.pin = P.moves = 0;
-    P.sa_c = -1;
-    P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
+    P.sa_c = P.sa_c2 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
     int pcell = 0, rj = 0;
     ScanState S;
     S.cur = S.end = 0;
-    S.c = -1;
+    S.c = S.c2 = -1;
     S.cell = 0;
-    S.m1 = S.m2 = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+    S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
     uint32_t wave_iters = 0, lane_iters = 0;
     while (__any(phase != kDone)) {
         ++wave_iters;
@@ -896,372 +922,12 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     }
 }
 
-// Scan of `cell` at point s by a group of G lanes (G a power of two, the
-// group an aligned block of G lanes, j the lane's index in it): lane j takes
-// edges j, j + G, ... of the padded list, then (m1, c, m2) are merged across
-// the group (ties to the lower edge index).  Every lane of the group ends
-// with the same result, equal to scan_cell's m1, c and min(m2, cap) (the
-// sequential scan's early exit skips only edges whose margins are at least
-// min(m2, cap); edges are visited in index order in both).
-__device__ __forceinline__ Scan group_scan(const GeomDev& g, int cell, float sx, float sy, float sz, int G, int j,
-                                           uint32_t& facets)
-{
-    const int lo = g.nb_off[cell], hi = g.nb_off[cell + 1];
-    const float nx = -sx, ny = -sy, nz = -sz;
-    float m1 = __builtin_inff(), m2 = __builtin_inff();
-    int c = -1;
-    for (int k = lo + j; k < hi; k += G) {
-        const float4 r = g.nb_rec[k];
-        const float m = fmaf(nx, r.x, fmaf(ny, r.y, fmaf(nz, r.z, r.w)));
-        const bool lt = m < m1;
-        m2 = fminf(m2, fmaxf(m1, m));
-        c = lt ? k : c;
-        m1 = fminf(m1, m);
-    }
-    facets += (uint32_t)((hi - lo) / G);
-    for (int off = G >> 1; off >= 1; off >>= 1) {
-        const float pm1 = __shfl_xor(m1, off, 64), pm2 = __shfl_xor(m2, off, 64);
-        const int pc = __shfl_xor(c, off, 64);
-        const bool take = pm1 < m1 || (pm1 == m1 && (uint32_t)pc < (uint32_t)c);
-        m2 = fminf(fmaxf(m1, pm1), fminf(m2, pm2));
-        c = take ? pc : c;
-        m1 = fminf(m1, pm1);
-    }
-    return Scan{m1, m2, kUlp * (fabsf(sx) + fabsf(sy) + fabsf(sz) + g.cell_hmax[cell]), c};
-}
-
-// Full classification of one request by its group (see classify_round for
-// the steps): greedy location, label with FP64 re-decision, k side for an
-// outside point, and the new tier.  All lanes of the group compute the same
-// values.  Returns 0 on a classifier failure.
-__device__ int classify_group(const GeomDev& g, float cap, int G, int j, int& na, float& qx, float& qy, float& qz,
-                              float& ox, float& oy, float& oz, int& pin, Tier& t, uint32_t& facets, uint32_t& iters,
-                              uint32_t& rechecks, uint32_t& facets64)
-{
-    for (int moves = 0; moves <= kMaxGraphIters; ++moves) {
-        const Scan s = group_scan(g, na, qx, qy, qz, G, j, facets);
-        if (s.m1 < -s.eps) {
-            const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
-            qx = (float)((double)qx - ex);
-            qy = (float)((double)qy - ey);
-            qz = (float)((double)qz - ez);
-            ox = (float)((double)ox + ex);
-            oy = (float)((double)oy + ey);
-            oz = (float)((double)oz + ez);
-            na = g.nb_id[s.c];
-            if (j == 0) ++iters;
-            continue;
-        }
-        const float a1 = g.cell_alpha[na];
-        pin = s.m1 >= a1;
-        if (fabsf(s.m1 - a1) <= s.eps) {
-            if (j == 0) ++rechecks;
-            if (!locate_f64(g, na, qx, qy, qz, ox, oy, oz, pin, facets64)) return 0;
-            tier_clear(t);
-            return 1;
-        }
-        if (s.c < 0 || s.m1 >= cap) {
-            tier_make(g, na, -1, qx, qy, qz, 0.f, cap, s.eps, false, 0.f, 0.f, t);
-            return 1;
-        }
-        if (pin) {
-            tier_make(g, na, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, false, 0.f, 0.f, t);
-            return 1;
-        }
-        const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
-        const Scan sk = group_scan(g, g.nb_id[s.c], (float)((double)qx - ex), (float)((double)qy - ey),
-                                   (float)((double)qz - ez), G, j, facets);
-        tier_make(g, na, s.c, qx, qy, qz, s.m1, fminf(s.m2, cap), s.eps, true, fminf(sk.m2, cap), sk.eps, t);
-        return 1;
-    }
-    return 0;  // the greedy walk does not settle
-}
-
 // Orders LDS accesses of different lanes of one wave (a wave's LDS
 // instructions complete in order; this keeps the compiler from moving them).
 __device__ __forceinline__ void wave_lds_fence()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-}
-
-// One tile, lock-step variant: the block's walkers take every step together.
-// The proposals of a wave that the plane tier cannot answer (n of them) are
-// classified cooperatively by the wave: request r goes to the aligned group
-// of G = 8, 4, 2 or 1 lanes (the largest with 64 / G >= n), which scans each
-// cell's edges G at a time and merges the results with shuffles.  A step
-// then costs the step code plus about (edges / G) per classification instead
-// of the longest classification of the wave, and samples are uniform.
-// Results equal the state-machine variant's bit for bit.
-template <int NP>
-__device__ __forceinline__ void walk_tile_coop(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
-                                               int64_t w_lo, const PlanDev& plan, const Events& ev,
-                                               const TileOut& out, float cap_width, unsigned long long* counters,
-                                               TileLds<NP>& L)
-{
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int64_t w = w_lo + threadIdx.x;
-    const bool live = w < p.n_walkers;
-    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
-    uint32_t iters = 0, facets = 0, rechecks = 0;
-    const float cap = g.alpha_max + cap_width;
-    const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
-    if (occ_in_lds) {
-        for (int j = threadIdx.x; j < p.n_grid; j += kBlock) {
-            L.occ[j] = 0;
-#pragma unroll
-            for (int u = 0; u < NP; ++u) L.evc[j][u] = 0.f;
-        }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int e = 0; e < ev.n; ++e) L.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
-    }
-    __syncthreads();
-
-    Rng rng{0, 0};
-    double x0 = 0.0, y0 = 0.0, z0 = 0.0;
-    if (live) {
-        rng = rng_for_walker(p.seed, p.first_walker + (uint64_t)w, jump_pow);
-        const double span = g.source_hi - g.source_lo;
-        x0 = g.source_lo + uniform_f64(rng) * span;
-        y0 = g.source_lo + uniform_f64(rng) * span;
-        z0 = g.source_lo + uniform_f64(rng) * span;
-    }
-    int a = 0, in = 0, cell = 0, err = 0, escaped = 0;
-    float px = 0.f, py = 0.f, pz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
-    Tier t;
-    tier_clear(t);
-    if (live && !g.free_water) {
-        anchor_world(g, x0, y0, z0, a, px, py, pz);
-        ox = -px;
-        oy = -py;
-        oz = -pz;
-        ++n_class;
-        if (!classify_full(g, a, px, py, pz, ox, oy, oz, cap, in, t, iters, facets, rechecks)) {
-            err = 1;
-            escaped = 1;
-        }
-        cell = a;
-        ox = -px;
-        oy = -py;
-        oz = -pz;
-    } else if (live) {
-        tier_free_water(t);
-    }
-    const float lox = (float)(-x0), loy = (float)(-y0), loz = (float)(-z0);
-    const float hix = (float)(g.L - x0), hiy = (float)(g.L - y0), hiz = (float)(g.L - z0);
-
-    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
-    float dM[NP][3];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
-    {
-        const int c0 = (int)__popcll(__ballot(live && !err && in));
-        if (lane == 0 && c0) {
-            if (occ_in_lds) atomicAdd(&L.occ[0], c0);
-            else atomicAdd((unsigned long long*)&out.occupancy[0], (unsigned long long)c0);
-        }
-    }
-
-    const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
-    const float ts = (float)p.ts;
-    SvcSlot* const slots = L.svc + wave * 64;
-    int until_sample = p.steps_per_h, j_sample = 0, next_ev = 0;
-    for (int32_t step = 0; step < p.n_steps; ++step) {
-        const bool act = live && !escaped;
-        int need = 0, pin = in, pcell = cell, na = a, rj = 0;
-        float qx = px, qy = py, qz = pz, nox = ox, noy = oy, noz = oz;
-        if (act) {
-            const float dx = normal_scaled_f32(rng, nscale);
-            const float dy = normal_scaled_f32(rng, nscale);
-            const float dz = normal_scaled_f32(rng, nscale);
-            const uint64_t ku = rng_next(rng) >> 11;
-            rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
-            qx = px + dx;
-            qy = py + dy;
-            qz = pz + dz;
-            const float nDx = ox + qx, nDy = oy + qy, nDz = oz + qz;
-            if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
-                escaped = 1;  // the walker stops; later samples keep its last label
-            } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
-                ++n_tier;
-            } else {
-                need = 1;
-            }
-        }
-        const uint64_t mask = __ballot(need);
-        Tier nt;
-        if (mask) {
-            const int n = (int)__popcll(mask);
-            const int lg = n <= 8 ? 3 : (n <= 16 ? 2 : (n <= 32 ? 1 : 0));
-            const int G = 1 << lg;
-            const int rank = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-            if (need) {
-                ++n_class;
-                slots[rank].v[0] = make_float4(__int_as_float(a), qx, qy, qz);
-                slots[rank].v[1] = make_float4(ox, oy, oz, 0.f);
-            }
-            wave_lds_fence();
-            const int r = lane >> lg, j = lane & (G - 1);
-            if (r < n) {
-                const float4 r0 = slots[r].v[0], r1 = slots[r].v[1];
-                int sa = __float_as_int(r0.x), spin = 0;
-                float sx = r0.y, sy = r0.z, sz = r0.w, sox = r1.x, soy = r1.y, soz = r1.z;
-                Tier st;
-                const int ok = classify_group(g, cap, G, j, sa, sx, sy, sz, sox, soy, soz, spin, st, nfac, iters,
-                                              rechecks, facets);
-                wave_lds_fence();
-                if (j == 0) {
-                    slots[r].v[0] = make_float4(__int_as_float(sa), sx, sy, sz);
-                    slots[r].v[1] = make_float4(sox, soy, soz, __int_as_float(ok | (spin << 1)));
-                    slots[r].v[2] = make_float4(st.ux, st.uy, st.uz, st.h);
-                    slots[r].v[3] = make_float4(st.ra_in2, st.ra_out2, st.rk_in2, st.rk_out2);
-                    slots[r].v[4] = make_float4(st.alpha_a, st.alpha_k, st.eps, __int_as_float(st.k));
-                    slots[r].v[5] = make_float4(st.rout2, 0.f, 0.f, 0.f);
-                }
-            }
-            wave_lds_fence();
-            if (need) {
-                const SvcSlot& s = slots[rank];
-                const float4 r0 = s.v[0], r1 = s.v[1], r2 = s.v[2], r3 = s.v[3], r4 = s.v[4], r5 = s.v[5];
-                const int st = __float_as_int(r1.w);
-                if (!(st & 1)) {
-                    err = 1;
-                    escaped = 1;
-                } else {
-                    nt.rout2 = r5.x;
-                    na = __float_as_int(r0.x);
-                    qx = r0.y;
-                    qy = r0.z;
-                    qz = r0.w;
-                    nox = r1.x;
-                    noy = r1.y;
-                    noz = r1.z;
-                    pin = (st >> 1) & 1;
-                    pcell = na;
-                    nt.ux = r2.x;
-                    nt.uy = r2.y;
-                    nt.uz = r2.z;
-                    nt.h = r2.w;
-                    nt.cx = qx;
-                    nt.cy = qy;
-                    nt.cz = qz;
-                    nt.ra_in2 = r3.x;
-                    nt.ra_out2 = r3.y;
-                    nt.rk_in2 = r3.z;
-                    nt.rk_out2 = r3.w;
-                    nt.alpha_a = r4.x;
-                    nt.alpha_k = r4.y;
-                    nt.eps = r4.z;
-                    nt.k = __float_as_int(r4.w);
-                }
-            }
-            wave_lds_fence();
-        }
-        if (act && !escaped) {
-            int m = 0;
-            if (in != pin) m = 1;
-            else if (in && pin && cell != pcell) m = 2;
-            bool accept = true;
-            if (m > 0) {
-                ++crossings;
-                if ((rj >> (m - 1)) & 1) {
-                    accept = false;
-                    ++rejects;
-                }
-            }
-            if (accept) {
-                if (need) t = nt;
-                a = na;
-                in = pin;
-                cell = pcell;
-                px = qx;
-                py = qy;
-                pz = qz;
-                ox = nox;
-                oy = noy;
-                oz = noz;
-            }
-            const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
-            const float yx = Dx - Cx, tx = Sx + yx;
-            Cx = (tx - Sx) - yx;
-            Sx = tx;
-            const float yy = Dy - Cy, ty = Sy + yy;
-            Cy = (ty - Sy) - yy;
-            Sy = ty;
-            const float yz = Dz - Cz, tz = Sz + yz;
-            Cz = (tz - Sz) - yz;
-            Sz = tz;
-        }
-        if (--until_sample == 0) {
-            until_sample = p.steps_per_h;
-            const int j = ++j_sample;
-            const int cj = (int)__popcll(__ballot(live && in));
-            if (lane == 0 && cj) {
-                if (occ_in_lds) atomicAdd(&L.occ[j], cj);
-                else atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)cj);
-            }
-            const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
-            // Y(t_j) = ts * (sum_{k<=n} D_k - D_n / 2), relative to the start point
-            const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
-            const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
-            const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
-            if (occ_in_lds) {
-#pragma unroll
-                for (int u = 0; u < NP; ++u) {
-                    const float sg = L.evc[j][u];
-                    dM[u][0] = fmaf(sg, Yx, dM[u][0]);
-                    dM[u][1] = fmaf(sg, Yy, dM[u][1]);
-                    dM[u][2] = fmaf(sg, Yz, dM[u][2]);
-                }
-            } else {
-                while (next_ev < ev.n && ev.j[next_ev] == j) {
-                    const int q = ev.pair[next_ev];
-                    const float sg = (float)ev.sign[next_ev];
-#pragma unroll
-                    for (int u = 0; u < NP; ++u)
-                        if (u == q) {
-                            dM[u][0] += sg * Yx;
-                            dM[u][1] += sg * Yy;
-                            dM[u][2] += sg * Yz;
-                        }
-                    ++next_ev;
-                }
-            }
-        }
-    }
-    facets += nfac;
-    {
-        const long long ne = wave_popc(live && escaped);
-        const long long nr = wave_popc(live && err);
-        if (lane == 0) {
-            if (ne) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
-            if (nr) atomicAdd((unsigned long long*)&out.flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
-        }
-    }
-    __syncthreads();
-    if (occ_in_lds)
-        for (int j = threadIdx.x; j < p.n_grid; j += kBlock)
-            if (L.occ[j]) atomicAdd((unsigned long long*)&out.occupancy[j], (unsigned long long)L.occ[j]);
-    emit_columns<float, NP>(plan, dM, live && !escaped, L.cols, out.slot);
-    if (counters) {
-        const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
-                     v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
-                     v4 = wave_sum_f64((double)facets), v5 = wave_sum_f64((double)crossings),
-                     v6 = wave_sum_f64((double)rejects), v7 = wave_sum_f64(live ? (double)p.n_steps : 0.0);
-        if (lane == 0) {
-            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
-            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
-            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)v2);
-            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)v3);
-            atomicAdd(&counters[MADI_CNT_FACETS], (unsigned long long)v4);
-            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
-            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
-            atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)p.n_steps);
-            atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
-        }
-    }
 }
 
 // ------------------------------------------------------------ streamed walk
@@ -1381,14 +1047,14 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
             Pending P;
             P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
             P.na = P.pin = P.moves = 0;
-            P.sa_c = -1;
-            P.sa_m1 = P.sa_m2 = P.sa_eps = 0.f;
+            P.sa_c = P.sa_c2 = -1;
+            P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
             int pcell = 0, rj = 0;
             ScanState S;
             S.cur = S.end = 0;
-            S.c = -1;
+            S.c = S.c2 = -1;
             S.cell = 0;
-            S.m1 = S.m2 = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+            S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
             for (;;) {
                 // hand walkers to idle lanes
                 uint64_t need = __ballot(phase == kIdle);
@@ -1650,21 +1316,550 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
     }
 }
 
-// One entry: block b runs tile b.
-// Classification variants (M): 0 per-lane state machine with R rounds per
-// iteration, 1 wave-cooperative lock step.
-template <int NP, int R, int M>
-__device__ __forceinline__ void walk_tile(const GeomDev& g, const WalkDev& p, const Gf2Col* __restrict__ jump_pow,
-                                          int64_t w_lo, const PlanDev& plan, const Events& ev, const TileOut& out,
-                                          float cap_width, unsigned long long* counters, TileLds<NP>& L)
+// ------------------------------------------------------------- split walk
+//
+// Walker waves and classifier waves (classify_mode 4).  In the per-lane state
+// machine every wave runs the step code and the classification code in each
+// iteration, each with only part of its lanes active.  Here the WW walker
+// waves of a workgroup only step (tier hits complete at once) and send their
+// tier misses, with the walker's first classification, through an LDS
+// request ring to the workgroup's classifier wave, which keeps its 64 lanes
+// busy with requests (the same per-lane classification state machine,
+// kSplitRounds rounds per iteration) and writes each answer (label, moved
+// point, new tier) into the asking lane's answer slot.  A walker lane waits
+// for its answer; walker lanes are refilled with new walkers as in the
+// streamed walk, and moments are stored for k_emit_tiles, so results are
+// bitwise those of every other variant.
+//
+// LDS protocol: a producer reserves ring tickets with an atomic on `tail`,
+// writes the request and then publishes it by storing ticket + 1 to the
+// slot's `seq` with release order; the classifier reads slots in ticket
+// order and takes one once `seq` (acquire) shows it.  Answers are published
+// the same way through the answer slot's `status`.  The ring has a slot per
+// walker lane and a lane has at most one request outstanding, so a slot is
+// always consumed before its ticket comes round again.  No wait is
+// unbounded: a walker wave that sees no lane progress for kSplitStallLimit
+// iterations abandons its waiting walkers as classifier failures, and the
+// classifier wave exits once every walker wave has finished and the ring is
+// empty.
+
+constexpr int kReqSlots = 256;  // >= walker lanes per workgroup (power of two)
+constexpr int kSplitRounds = 2;  // classification rounds per classifier iteration
+constexpr uint32_t kSplitStallLimit = 1u << 26;
+
+struct ReqSlot {
+    int seq, na, owner, pad0;
+    float qx, qy, qz, ox, oy, oz;
+    int pad[2];
+};
+
+struct RespSlot {
+    int status, na, flags, pad;  // flags: bit 0 inside, bits 1-3 tmode, bit 4 classifier failure
+    float qx, qy, qz, ox, oy, oz;
+    float pad2[2];
+    Tier t;
+};
+
+template <int NP, int WW, int CW>
+struct SplitLds {
+    float evc[kMaxLdsGrid][NP];
+    int occ[WW][kMaxLdsGrid];
+    Rng states[WW][kClaimWalkers];
+    ReqSlot req[CW][kReqSlots];  // one ring per classifier wave; walker wave w uses ring w mod CW
+    RespSlot resp[WW * 64];
+    GeomDev geo[WW];        // each walker wave's current entry (its requests are classified on it)
+    unsigned int tail[CW];  // request tickets issued per ring
+    unsigned int done;      // walker waves finished
+};
+
+__device__ __forceinline__ int lds_load_acquire(const int* p)
 {
-    if constexpr (M == 1)
-        walk_tile_coop<NP>(g, p, jump_pow, w_lo, plan, ev, out, cap_width, counters, L);
-    else
-        walk_tile_f32<NP, R>(g, p, jump_pow, w_lo, plan, ev, out, cap_width, counters, L);
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-template <int NP, int R, int M>
+__device__ __forceinline__ void lds_store_release(int* p, int v)
+{
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// One walker wave (wv < WW): the streamed walk's walker life cycle with the
+// classification replaced by a request to the classifier wave.
+template <int NP, int WW, int CW>
+__device__ void split_walker(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
+                             const PlanDev& plan, const Gf2Col* __restrict__ jump_pow, const Events& ev,
+                             unsigned long long* counters, SplitLds<NP, WW, CW>& L, int wv)
+{
+    enum : int { kStep = 0, kWait = 1, kDone = 3, kIdle = 4 };
+    const int lane = threadIdx.x & 63;
+    const int32_t n_work = hdr->n_work;
+    const int64_t total_w = hdr->total_walkers;
+    int* occ = L.occ[wv];
+    Rng* st = L.states[wv];
+    RespSlot& my_resp = L.resp[wv * 64 + lane];
+    for (int j = lane; j < kMaxLdsGrid; j += 64) occ[j] = 0;
+    wave_lds_fence();
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0;
+    uint32_t wave_iters = 0, lane_iters = 0;
+    const int n_pairs = plan.n_pairs;
+    if (n_work > 0 && total_w > 0) {
+        const int64_t n_waves = (int64_t)gridDim.x * WW;
+        const int64_t pos = ((int64_t)blockIdx.x * WW + wv) * total_w / n_waves;
+        int lo = 0, hi = n_work - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (work[mid].walker_begin <= pos) lo = mid;
+            else hi = mid - 1;
+        }
+        int e = __builtin_amdgcn_readfirstlane(lo);
+        int dry = 0;
+        bool aborted = false;
+        while (dry < n_work) {
+            const WorkDev* __restrict__ W = work + e;
+            unsigned long long* next = item_next + e;
+            int64_t claim = stream_claim(W, next, jump_pow, st, lane);
+            if (claim >= W->n_walk) {
+                ++dry;
+                e = e + 1 == n_work ? 0 : e + 1;
+                continue;
+            }
+            dry = 0;
+            const GeomDev& g = W->g;
+            const WalkDev& p = W->w;
+            // the classifier reads this wave's entry from LDS; no request of
+            // this wave is outstanding while it changes
+            if (lane == 0) L.geo[wv] = g;
+            wave_lds_fence();
+            const int64_t n_walk = W->n_walk, walker_lo = W->walker_lo;
+            const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
+            const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
+            const float ts = (float)p.ts;
+            int64_t hand = claim, hand_end = claim + kClaimWalkers < n_walk ? claim + kClaimWalkers : n_walk;
+            bool item_dry = false;
+
+            int phase = kIdle, init = 0, enq = 0;
+            unsigned int ticket = 0;
+            int64_t w = 0;
+            Rng rng{0, 0};
+            int a = 0, in = 0, cell = 0, err = 0, escaped = 0;
+            uint32_t n_esc = 0, n_err = 0, stall = 0;
+            float px = 0.f, py = 0.f, pz = 0.f, ox = 0.f, oy = 0.f, oz = 0.f;
+            float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
+            Tier t;
+            tier_clear(t);
+            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+            float dM[NP][3];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
+            int step = 0, until_sample = 0, j_sample = 0, next_ev = 0, rj = 0;
+            float qx = 0.f, qy = 0.f, qz = 0.f, qox = 0.f, qoy = 0.f, qoz = 0.f;
+            int qna = 0;
+            for (;;) {
+                uint64_t need = __ballot(phase == kIdle);
+                if (need != 0 && !item_dry) {
+                    bool got = false;
+                    while (need != 0) {
+                        if (hand >= hand_end) {
+                            const int64_t nb = stream_claim(W, next, jump_pow, st, lane);
+                            if (nb >= n_walk) {
+                                item_dry = true;
+                                break;
+                            }
+                            claim = nb;
+                            hand = nb;
+                            hand_end = nb + kClaimWalkers < n_walk ? nb + kClaimWalkers : n_walk;
+                        }
+                        const int64_t rank = (int64_t)lanes_below(need);
+                        const bool mine = ((need >> lane) & 1ull) && rank < hand_end - hand;
+                        if (mine) {
+                            w = hand + rank;
+                            rng = st[w - claim];
+                            got = true;
+                        }
+                        const uint64_t taken = __ballot(mine);
+                        hand += __popcll(taken);
+                        need &= ~taken;
+                        wave_lds_fence();
+                    }
+                    if (got) {
+                        const double span = g.source_hi - g.source_lo;
+                        const double x0 = g.source_lo + uniform_f64(rng) * span;
+                        const double y0 = g.source_lo + uniform_f64(rng) * span;
+                        const double z0 = g.source_lo + uniform_f64(rng) * span;
+                        lox = (float)(-x0);
+                        loy = (float)(-y0);
+                        loz = (float)(-z0);
+                        hix = (float)(g.L - x0);
+                        hiy = (float)(g.L - y0);
+                        hiz = (float)(g.L - z0);
+                        Sx = Sy = Sz = Cx = Cy = Cz = 0.f;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
+                        step = 0;
+                        until_sample = p.steps_per_h;
+                        j_sample = 0;
+                        next_ev = 0;
+                        err = escaped = 0;
+                        if (g.free_water) {
+                            tier_free_water(t);
+                            a = cell = in = 0;
+                            px = py = pz = ox = oy = oz = 0.f;
+                            init = 0;
+                            phase = p.n_steps > 0 ? kStep : kDone;
+                        } else {
+                            anchor_world(g, x0, y0, z0, qna, qx, qy, qz);
+                            qox = -qx;
+                            qoy = -qy;
+                            qoz = -qz;
+                            ++n_class;
+                            init = 1;
+                            enq = 1;
+                            phase = kWait;
+                        }
+                    }
+                }
+                if (__ballot(phase != kIdle) == 0) break;
+                ++wave_iters;
+                lane_iters += phase != kIdle;
+                int complete = 0, fromq = 0, rflags = 0, rna = 0;
+                if (phase == kStep) {
+                    const float dx = normal_scaled_f32(rng, nscale);
+                    const float dy = normal_scaled_f32(rng, nscale);
+                    const float dz = normal_scaled_f32(rng, nscale);
+                    const uint64_t ku = rng_next(rng) >> 11;
+                    rj = (ku >= p.rej1 ? 1 : 0) | (ku >= p.rej2 ? 2 : 0);
+                    qx = px + dx;
+                    qy = py + dy;
+                    qz = pz + dz;
+                    qox = ox;
+                    qoy = oy;
+                    qoz = oz;
+                    qna = a;
+                    const float nDx = ox + qx, nDy = oy + qy, nDz = oz + qz;
+                    int pcell = 0, pin = 0;
+                    if (!g.periodic && (nDx < lox || nDx >= hix || nDy < loy || nDy >= hiy || nDz < loz || nDz >= hiz)) {
+                        escaped = 1;
+                        phase = kDone;
+                        if (in)
+                            for (int j = j_sample + 1; j < p.n_grid; ++j) {
+                                if (occ_in_lds) atomicAdd(&occ[j], 1);
+                                else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
+                            }
+                    } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
+                        ++n_tier;
+                        complete = 1;
+                        rna = pcell;
+                        rflags = pin;
+                    } else {
+                        ++n_class;
+                        enq = 1;
+                        phase = kWait;
+                    }
+                }
+                // publish this iteration's requests
+                {
+                    const uint64_t em = __ballot(enq != 0);
+                    if (em != 0) {
+                        const int first = __builtin_ctzll(em);
+                        unsigned int base = 0;
+                        if (lane == first) base = atomicAdd(&L.tail[wv % CW], (unsigned int)__popcll(em));
+                        base = __builtin_amdgcn_readfirstlane(__shfl(base, first, 64));
+                        if (enq) {
+                            ticket = base + lanes_below(em);
+                            ReqSlot& rq = L.req[wv % CW][ticket & (kReqSlots - 1)];
+                            rq.na = qna;
+                            rq.owner = wv * 64 + lane;
+                            rq.qx = qx;
+                            rq.qy = qy;
+                            rq.qz = qz;
+                            rq.ox = qox;
+                            rq.oy = qoy;
+                            rq.oz = qoz;
+                            lds_store_release(&rq.seq, (int)(ticket + 1u));
+                            enq = 0;
+                        }
+                    }
+                }
+                if (phase == kWait && lds_load_acquire(&my_resp.status) == (int)(ticket + 1u)) {
+                    rflags = my_resp.flags;
+                    rna = my_resp.na;
+                    qx = my_resp.qx;
+                    qy = my_resp.qy;
+                    qz = my_resp.qz;
+                    qox = my_resp.ox;
+                    qoy = my_resp.oy;
+                    qoz = my_resp.oz;
+                    complete = 1;
+                    fromq = 1;
+                    if (rflags & 16) {
+                        complete = 0;
+                        err = 1;
+                        escaped = 1;
+                        phase = kDone;
+                    } else {
+                        phase = kStep;
+                    }
+                }
+                if (complete && init) {
+                    // the walker's first label: no step, no acceptance test
+                    t = my_resp.t;
+                    a = rna;
+                    in = rflags & 1;
+                    cell = rna;
+                    px = qx;
+                    py = qy;
+                    pz = qz;
+                    ox = -qx;
+                    oy = -qy;
+                    oz = -qz;
+                    if (in) {
+                        if (occ_in_lds) atomicAdd(&occ[0], 1);
+                        else atomicAdd((unsigned long long*)&W->occupancy[0], 1ull);
+                    }
+                    init = 0;
+                    phase = p.n_steps > 0 ? kStep : kDone;
+                } else if (complete) {
+                    const int pin = rflags & 1, pcell = rna;
+                    int m = 0;
+                    if (in != pin) m = 1;
+                    else if (in && pin && cell != pcell) m = 2;
+                    bool accept = true;
+                    if (m > 0) {
+                        ++crossings;
+                        if ((rj >> (m - 1)) & 1) {
+                            accept = false;
+                            ++rejects;
+                        }
+                    }
+                    if (accept) {
+                        if (fromq) t = my_resp.t;
+                        a = fromq ? rna : a;
+                        in = pin;
+                        cell = pcell;
+                        px = qx;
+                        py = qy;
+                        pz = qz;
+                        ox = qox;
+                        oy = qoy;
+                        oz = qoz;
+                    }
+                    const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
+                    const float yx = Dx - Cx, tx = Sx + yx;
+                    Cx = (tx - Sx) - yx;
+                    Sx = tx;
+                    const float yy = Dy - Cy, ty = Sy + yy;
+                    Cy = (ty - Sy) - yy;
+                    Sy = ty;
+                    const float yz = Dz - Cz, tz = Sz + yz;
+                    Cz = (tz - Sz) - yz;
+                    Sz = tz;
+                    ++step;
+                    if (--until_sample == 0) {
+                        until_sample = p.steps_per_h;
+                        const int j = ++j_sample;
+                        if (in) {
+                            if (occ_in_lds) atomicAdd(&occ[j], 1);
+                            else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
+                        }
+                        const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
+                        const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
+                        const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+                        if (occ_in_lds) {
+#pragma unroll
+                            for (int u = 0; u < NP; ++u) {
+                                const float sg = L.evc[j][u];
+                                dM[u][0] = fmaf(sg, Yx, dM[u][0]);
+                                dM[u][1] = fmaf(sg, Yy, dM[u][1]);
+                                dM[u][2] = fmaf(sg, Yz, dM[u][2]);
+                            }
+                        } else {
+                            while (next_ev < ev.n && ev.j[next_ev] == j) {
+                                const int q = ev.pair[next_ev];
+                                const float sg = (float)ev.sign[next_ev];
+#pragma unroll
+                                for (int u = 0; u < NP; ++u)
+                                    if (u == q) {
+                                        dM[u][0] += sg * Yx;
+                                        dM[u][1] += sg * Yy;
+                                        dM[u][2] += sg * Yz;
+                                    }
+                                ++next_ev;
+                            }
+                        }
+                    }
+                    phase = step >= p.n_steps ? kDone : kStep;
+                }
+                if (phase == kDone) {
+                    const int64_t ew = walker_lo + w;
+                    float* m = W->moments + ew * (int64_t)(n_pairs * 3);
+#pragma unroll
+                    for (int u = 0; u < NP; ++u)
+                        if (u < n_pairs) {
+                            m[3 * u] = dM[u][0];
+                            m[3 * u + 1] = dM[u][1];
+                            m[3 * u + 2] = dM[u][2];
+                        }
+                    W->active[ew] = escaped ? 0 : 1;
+                    n_esc += escaped;
+                    n_err += err;
+                    phase = kIdle;
+                }
+                // bounded wait: a wave whose lanes all wait makes no progress;
+                // past the limit its waiting walkers count as classifier
+                // failures (fatal for the entry) and the wave stops walking,
+                // keeping its geometry slot for the requests still queued
+                if (__ballot(phase != kWait && phase != kIdle) == 0 && __ballot(phase == kWait) != 0) {
+                    __builtin_amdgcn_s_sleep(1);  // leave the issue slots to the classifier wave
+                    if (++stall > kSplitStallLimit) {
+                        n_err += phase == kWait;
+                        n_esc += phase == kWait;
+                        aborted = true;
+                        break;
+                    }
+                } else {
+                    stall = 0;
+                }
+            }
+            if (occ_in_lds) {
+                wave_lds_fence();
+                for (int j = lane; j < p.n_grid; j += 64) {
+                    const int v = occ[j];
+                    if (v) atomicAdd((unsigned long long*)&W->occupancy[j], (unsigned long long)v);
+                    occ[j] = 0;
+                }
+                wave_lds_fence();
+            }
+            {
+                const double ne = wave_sum_f64((double)n_esc), nr = wave_sum_f64((double)n_err);
+                if (lane == 0) {
+                    if (ne > 0.0) atomicAdd((unsigned long long*)&W->flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
+                    if (nr > 0.0)
+                        atomicAdd((unsigned long long*)&W->flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
+                }
+            }
+            if (aborted) break;
+            e = e + 1 == n_work ? 0 : e + 1;
+        }
+    }
+    if (lane == 0) __hip_atomic_fetch_add(&L.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (counters) {
+        const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
+                     v5 = wave_sum_f64((double)crossings), v6 = wave_sum_f64((double)rejects),
+                     v7 = wave_sum_f64((double)lane_iters);
+        if (lane == 0) {
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)v0);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)v1);
+            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)v5);
+            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)v6);
+            atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
+            atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
+        }
+    }
+}
+
+// The classifier wave: requests taken in ticket order into idle lanes,
+// classified with the per-lane state machine on the asking wave's entry (its
+// geometry slot), answered with the new tier.  Lanes may work on different
+// entries.
+template <int NP, int WW, int CW>
+__device__ void split_classifier(float cap_width, unsigned long long* counters, SplitLds<NP, WW, CW>& L, int cw)
+{
+    const int lane = threadIdx.x & 63;
+    // walker lanes wait for this wave's answers: let it issue ahead of the
+    // walker waves sharing its SIMD
+    __builtin_amdgcn_s_setprio(3);
+    uint32_t nfac = 0, iters = 0, facets = 0, rechecks = 0, c_iters = 0, c_busy = 0;
+    int busy = 0, owner = 0, cphase = 0;
+    unsigned int ticket = 0, head = 0;
+    Pending P;
+    P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
+    P.na = P.pin = P.moves = 0;
+    P.sa_c = P.sa_c2 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+    ScanState S;
+    S.cur = S.end = 0;
+    S.c = S.c2 = -1;
+    S.cell = 0;
+    S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+    for (;;) {
+        const uint64_t need = __ballot(!busy);
+        if (need != 0) {
+            const unsigned int tk = head + (unsigned int)lane;
+            const int sq = lds_load_acquire(&L.req[cw][tk & (kReqSlots - 1)].seq);
+            const uint64_t rm = __ballot(sq == (int)(tk + 1u));
+            const int n_ready = ~rm == 0ull ? 64 : (int)__builtin_ctzll(~rm);
+            const int n_need = (int)__popcll(need);
+            const int take = n_ready < n_need ? n_ready : n_need;
+            const int rank = (int)lanes_below(need);
+            if (!busy && rank < take) {
+                ticket = head + (unsigned int)rank;
+                const ReqSlot& rq = L.req[cw][ticket & (kReqSlots - 1)];
+                P.na = rq.na;
+                owner = rq.owner;
+                P.qx = rq.qx;
+                P.qy = rq.qy;
+                P.qz = rq.qz;
+                P.ox = rq.ox;
+                P.oy = rq.oy;
+                P.oz = rq.oz;
+                busy = 1;
+                pending_start(P, S, cphase);
+            }
+            head += (unsigned int)take;
+        }
+        if (__ballot(busy) == 0) {
+            if (lds_load_acquire((const int*)&L.done) == WW &&
+                head == (unsigned int)__hip_atomic_load(&L.tail[cw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
+                break;
+            __builtin_amdgcn_s_sleep(2);
+            continue;
+        }
+        ++c_iters;
+        c_busy += busy;
+        const GeomDev& g = L.geo[owner >> 6];
+        const float cap = g.alpha_max + cap_width;
+        // an anchor outside the entry's cells cannot come from a live request;
+        // keep every access in bounds regardless
+        if (busy && (unsigned)P.na >= (unsigned)g.n_cells) P.na = 0;
+        int res = 0, tmode = 0;
+#pragma unroll 1
+        for (int r = 0; r < kSplitRounds; ++r) {
+            if (busy && res == 0) res = classify_round(g, cap, cphase, S, P, tmode, nfac, iters, facets, rechecks);
+        }
+        if (res != 0) {
+            RespSlot& rs = L.resp[owner];
+            if (res > 0) {
+                Tier t;
+                tier_of(g, P, tmode, S, cap, t);
+                rs.t = t;
+            }
+            rs.na = P.na;
+            rs.flags = (P.pin & 1) | (tmode << 1) | (res < 0 ? 16 : 0);
+            rs.qx = P.qx;
+            rs.qy = P.qy;
+            rs.qz = P.qz;
+            rs.ox = P.ox;
+            rs.oy = P.oy;
+            rs.oz = P.oz;
+            lds_store_release(&rs.status, (int)(ticket + 1u));
+            busy = 0;
+        }
+    }
+    facets += nfac;
+    if (counters) {
+        const double v2 = wave_sum_f64((double)iters), v3 = wave_sum_f64((double)rechecks),
+                     v4 = wave_sum_f64((double)facets), v8 = wave_sum_f64((double)c_busy);
+        if (lane == 0) {
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)v2);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)v3);
+            atomicAdd(&counters[MADI_CNT_FACETS], (unsigned long long)v4);
+            atomicAdd(&counters[MADI_CNT_CLASS_ITERS], (unsigned long long)c_iters);
+            atomicAdd(&counters[MADI_CNT_CLASS_BUSY], (unsigned long long)v8);
+        }
+    }
+}
+
+// One entry: block b runs tile b.
+template <int NP, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_walk_f32(
     GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow, FusedOut fo, float cap_width,
     unsigned long long* counters)
@@ -1674,7 +1869,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
     out.slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
     out.occupancy = fo.occupancy;
     out.flags = fo.flags;
-    walk_tile<NP, R, M>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters, lds);
+    walk_tile_f32<NP, R>(g, p, jump_pow, (int64_t)blockIdx.x * kBlock, *fo.plan, fo.ev, out, cap_width, counters, lds);
 }
 
 // Persistent work-stealing launch over many entries: each workgroup takes
@@ -1682,7 +1877,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 // by binary search on work_begin, and runs that tile.  Every workgroup exits
 // once the counter passes the total, so the grid drains (workgroups beyond
 // the total exit at once).
-template <int NP, int R, int M>
+template <int NP, int R>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_batch_f32(
     const WorkDev* __restrict__ work, BatchHdr* hdr, const PlanDev* __restrict__ plan, Events ev,
     const Gf2Col* __restrict__ jump_pow, float cap_width, unsigned long long* counters)
@@ -1717,7 +1912,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         out.slot = W.partial + (size_t)tile * plan->n_cols * 2;
         out.occupancy = W.occupancy;
         out.flags = W.flags;
-        walk_tile<NP, R, M>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds);
+        walk_tile_f32<NP, R>(W.g, W.w, jump_pow, tile * kBlock, *plan, ev, out, cap_width, counters, lds);
     }
 }
 
@@ -1739,6 +1934,36 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
             if (ev.j[e] < kMaxLdsGrid) lds.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
     __syncthreads();
     stream_wave<NP, R>(work, item_next, hdr, *plan, jump_pow, ev, cap_width, counters, lds);
+}
+
+// Split walk over a batch (see split_walker / split_classifier): WW walker
+// waves and CW classifier waves per workgroup.
+template <int NP, int WW, int CW>
+__global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_split_f32(
+    const WorkDev* __restrict__ work, unsigned long long* __restrict__ item_next, const BatchHdr* __restrict__ hdr,
+    const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
+    unsigned long long* counters)
+{
+    static_assert(kReqSlots >= 64 * ((WW + CW - 1) / CW), "a ring needs a slot per walker lane it serves");
+    __shared__ SplitLds<NP, WW, CW> lds;
+    constexpr int kThreads = (WW + CW) * 64;
+    for (int j = threadIdx.x; j < kMaxLdsGrid; j += kThreads)
+#pragma unroll
+        for (int u = 0; u < NP; ++u) lds.evc[j][u] = 0.f;
+    for (int j = threadIdx.x; j < CW * kReqSlots; j += kThreads) lds.req[j / kReqSlots][j % kReqSlots].seq = 0;
+    for (int j = threadIdx.x; j < WW * 64; j += kThreads) lds.resp[j].status = 0;
+    if (threadIdx.x < CW) lds.tail[threadIdx.x] = 0;
+    if (threadIdx.x == 0) lds.done = 0;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int e = 0; e < ev.n; ++e)
+            if (ev.j[e] < kMaxLdsGrid) lds.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
+    __syncthreads();
+    // roles rotate with the workgroup index, so classifier waves spread over
+    // the CU's SIMDs
+    const int r = (int)((threadIdx.x / 64 + blockIdx.x) % (WW + CW));
+    if (r >= WW) split_classifier<NP, WW, CW>(cap_width, counters, lds, r - WW);
+    else split_walker<NP, WW, CW>(work, item_next, hdr, *plan, jump_pow, ev, counters, lds, r);
 }
 
 // Column sums of every tile of a streamed batch from its walkers' stored
@@ -1863,45 +2088,39 @@ int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_
     return check_launch("k_classify_graph");
 }
 
-#define MADI_DISPATCH_C(NPV, RV, M_VAL, LAUNCH) \
-    do {                                        \
-        if ((M_VAL) == 1) LAUNCH(NPV, 1, 1);    \
-        else LAUNCH(NPV, RV, 0);                \
+#define MADI_DISPATCH_R(NPV, R_VAL, LAUNCH) \
+    do {                                    \
+        if ((R_VAL) <= 1) LAUNCH(NPV, 1);   \
+        else LAUNCH(NPV, 2);                \
     } while (0)
-#define MADI_DISPATCH_R(NPV, R_VAL, C_VAL, LAUNCH)              \
-    do {                                                        \
-        if ((R_VAL) <= 1) MADI_DISPATCH_C(NPV, 1, C_VAL, LAUNCH); \
-        else MADI_DISPATCH_C(NPV, 2, C_VAL, LAUNCH);            \
-    } while (0)
-#define MADI_DISPATCH_NP(NP_VAL, R_VAL, C_VAL, LAUNCH)                      \
-    do {                                                                    \
-        const int np_ = (NP_VAL);                                           \
-        if (np_ <= 2) MADI_DISPATCH_R(2, R_VAL, C_VAL, LAUNCH);             \
-        else if (np_ <= 4) MADI_DISPATCH_R(4, R_VAL, C_VAL, LAUNCH);        \
-        else MADI_DISPATCH_R(MADI_MAX_PAIRS, R_VAL, C_VAL, LAUNCH);         \
+#define MADI_DISPATCH_NP(NP_VAL, R_VAL, LAUNCH)                      \
+    do {                                                             \
+        const int np_ = (NP_VAL);                                    \
+        if (np_ <= 2) MADI_DISPATCH_R(2, R_VAL, LAUNCH);             \
+        else if (np_ <= 4) MADI_DISPATCH_R(4, R_VAL, LAUNCH);        \
+        else MADI_DISPATCH_R(MADI_MAX_PAIRS, R_VAL, LAUNCH);         \
     } while (0)
 
 int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
-                          float cap_width, int rounds, int variant, unsigned long long* counters, hipStream_t s)
+                          float cap_width, int rounds, unsigned long long* counters, hipStream_t s)
 {
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
-#define MADI_LAUNCH_WALK(NPV, RV, CV)                                                                               \
-    hipLaunchKernelGGL((k_walk_f32<NPV, RV, CV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, \
-                       jump_pow, fo, cap_width, counters)
-    MADI_DISPATCH_NP(fo.ev.n_pairs, rounds, variant, MADI_LAUNCH_WALK);
+#define MADI_LAUNCH_WALK(NPV, RV)                                                                                   \
+    hipLaunchKernelGGL((k_walk_f32<NPV, RV>), dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, fo,       \
+                       cap_width, counters)
+    MADI_DISPATCH_NP(fo.ev.n_pairs, rounds, MADI_LAUNCH_WALK);
 #undef MADI_LAUNCH_WALK
     return check_launch("k_walk_f32");
 }
 
 int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
-                     float cap_width, int rounds, int variant, int32_t grid, unsigned long long* counters,
-                     hipStream_t s)
+                     float cap_width, int rounds, int32_t grid, unsigned long long* counters, hipStream_t s)
 {
-#define MADI_LAUNCH_BATCH(NPV, RV, CV)                                                                              \
-    hipLaunchKernelGGL((k_batch_f32<NPV, RV, CV>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan, ev, jump_pow,  \
+#define MADI_LAUNCH_BATCH(NPV, RV)                                                                                  \
+    hipLaunchKernelGGL((k_batch_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, hdr, plan, ev, jump_pow,       \
                        cap_width, counters)
-    MADI_DISPATCH_NP(ev.n_pairs, rounds, variant, MADI_LAUNCH_BATCH);
+    MADI_DISPATCH_NP(ev.n_pairs, rounds, MADI_LAUNCH_BATCH);
 #undef MADI_LAUNCH_BATCH
     return check_launch("k_batch_f32");
 }
@@ -1910,12 +2129,33 @@ int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const 
                       const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
                       unsigned long long* counters, hipStream_t s)
 {
-#define MADI_LAUNCH_STREAM(NPV, RV, CV)                                                                             \
+#define MADI_LAUNCH_STREAM(NPV, RV)                                                                                 \
     hipLaunchKernelGGL((k_stream_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, item_next, hdr, plan, ev,    \
                        jump_pow, cap_width, counters)
-    MADI_DISPATCH_NP(ev.n_pairs, rounds, 0, MADI_LAUNCH_STREAM);
+    MADI_DISPATCH_NP(ev.n_pairs, rounds, MADI_LAUNCH_STREAM);
 #undef MADI_LAUNCH_STREAM
     return check_launch("k_stream_f32");
+}
+
+int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, int32_t grid,
+                     unsigned long long* counters, hipStream_t s)
+{
+#define MADI_LAUNCH_SPLIT(NPV, WWV, CWV)                                                                             \
+    hipLaunchKernelGGL((k_split_f32<NPV, WWV, CWV>), dim3(grid), dim3((WWV + CWV) * 64), 0, s, work, item_next, hdr, \
+                       plan, ev, jump_pow, cap_width, counters)
+#define MADI_SPLIT_SHAPE(NPV)                         \
+    do {                                              \
+        if (shape <= 1) MADI_LAUNCH_SPLIT(NPV, 3, 1); \
+        else MADI_LAUNCH_SPLIT(NPV, 2, 2);            \
+    } while (0)
+    const int np = ev.n_pairs;
+    if (np <= 2) MADI_SPLIT_SHAPE(2);
+    else if (np <= 4) MADI_SPLIT_SHAPE(4);
+    else MADI_SPLIT_SHAPE(MADI_MAX_PAIRS);
+#undef MADI_SPLIT_SHAPE
+#undef MADI_LAUNCH_SPLIT
+    return check_launch("k_split_f32");
 }
 
 int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,

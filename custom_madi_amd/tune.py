@@ -3,9 +3,10 @@
 The knobs it searches change only how the GPU does the work, never the
 result (the batch tests check bitwise equality across them):
 
-* ``classify_mode``: 1 per-lane state machine per tile, 2 wave-cooperative
-  lock step per tile, 3 per-lane state machine with lane refill;
-* ``scan_rounds``: classification rounds per state-machine iteration (1, 2);
+* ``classify_mode``: 1 per-lane state machine per tile, 3 per-lane state
+  machine with lane refill, 4 walker waves with classifier waves;
+* ``scan_rounds``: classification rounds per state-machine iteration (1, 2),
+  or for mode 4 the workgroup shape (1: 3 walker + 1 classifier waves, 2: 2 + 2);
 * ``grid_per_cu``: persistent workgroups per CU of the batch kernel.
 
 The plane-tier cap is left at its default: it changes which proposals take a
@@ -30,9 +31,9 @@ from typing import Iterable, List, Optional
 from . import _hip, engine
 
 DEFAULT_CANDIDATES = [
-    (3, 1, 3), (3, 1, 4), (3, 2, 3),
-    (1, 2, 4), (1, 2, 6),
-    (2, 1, 4),
+    (4, 1, 3), (4, 1, 4), (4, 2, 4),
+    (3, 1, 3), (3, 1, 4),
+    (1, 2, 4),
 ]  # (classify_mode, scan_rounds, grid_per_cu)
 
 

@@ -49,7 +49,9 @@ extern "C" {
 #define MADI_CNT_REJECTS 7       /* rejected crossings */
 #define MADI_CNT_WAVE_ITERS 8    /* FP32 walk: state-machine iterations, summed over waves */
 #define MADI_CNT_LANE_ITERS 9    /* FP32 walk: iterations until each lane's walk ended, summed over lanes */
-#define MADI_N_COUNTERS 10
+#define MADI_CNT_CLASS_ITERS 10  /* split FP32 walk: classifier-wave iterations, summed over waves */
+#define MADI_CNT_CLASS_BUSY 11   /* split FP32 walk: busy classifier lanes, summed over those iterations */
+#define MADI_N_COUNTERS 12
 
 /* Flags summed over walkers (int64 each). */
 #define MADI_FLAG_ESCAPED 0
@@ -160,9 +162,11 @@ typedef struct {
     double pp;
     uint64_t seed;          /* walk seed (device RNG) */
     uint64_t first_walker;  /* RNG subsequence index of walker 0 */
-    int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-2 (0 = 2) */
-    int32_t classify_mode;  /* FP32 walk: 0 default (3), 1 per-lane state machine per tile, 2 wave-cooperative
-                               lock step per tile, 3 per-lane state machine with lane refill (streamed) */
+    int32_t scan_rounds;    /* FP32 walk: classification rounds per iteration, 1-2 (0 = the variant's default);
+                               classify_mode 4: 1 = 3 walker + 1 classifier waves per workgroup, 2 = 2 + 2 */
+    int32_t classify_mode;  /* FP32 walk: 0 default (4), 1 per-lane state machine per tile, 3 per-lane state
+                               machine with lane refill (streamed), 4 streamed walker waves with classifier
+                               waves (split); 2 (a removed lock-step variant) is rejected */
 } madi_walk_params;
 
 /* Reference-semantics FP64 walk with the exact classifier at every endpoint.
@@ -241,7 +245,7 @@ typedef struct {
     double* d_partial;
     int64_t* d_occupancy;
     int64_t* d_flags;
-    void* d_moments;  /* the entry's madi_moments_bytes buffer; required by classify_mode 3, else unused */
+    void* d_moments;  /* the entry's madi_moments_bytes buffer; required by classify_mode 3 and 4 */
 } madi_work;
 
 typedef struct {
@@ -256,9 +260,9 @@ int madi_batch_workspace(int32_t n_work, int64_t* bytes);
 
 /* One persistent, work-stealing launch of the FP32 production walk over many
  * entries of uneven size: workgroups take tiles from an atomic counter until
- * every tile of every item is done (classify_mode 1, 2), or waves take
+ * every tile of every item is done (classify_mode 1), or waves take
  * walkers from per-item counters as lanes free up and a second kernel forms
- * the tile sums (classify_mode 3).  Each tile writes its own slot, so an
+ * the tile sums (classify_mode 3, 4).  Each tile writes its own slot, so an
  * entry's result is bitwise identical alone or in any mix, and equal across
  * classify modes.  Replaces the
  * one-entry-at-a-time loop of build_library_from_triplets

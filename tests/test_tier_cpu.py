@@ -3,9 +3,10 @@
 A tier built by a full classification at x answers later proposals q near x
 without classifying them (csrc/prod.hip: tier_make / tier_query).  This test
 restates both in FP64 numpy (exact arithmetic, so without the FP32 rounding
-slack) and checks every answer the tier gives against the all-seed SI Eq. S2
+slack, and with the exact third margin where the kernel uses a lower bound of
+it) and checks every answer the tier gives against the all-seed SI Eq. S2
 labels of the 27-image tiled periodic box (oracle.periodic_brute).  It pins
-the radii: an inside answer trusted up to m2 instead of m2 - alpha, an outside
+the radii: an inside answer trusted up to m3 instead of m3 - alpha, an outside
 ball not bounded by alpha_min (plus the distance to any third Voronoi cell),
 or a k-side answer without the k scan would each give wrong labels here.
 """
@@ -48,21 +49,26 @@ def _locate(arr, a, p):
 
 
 def tier_make(arr, a, x, cap):
-    """The tier built at x (relative to anchor a, a the nearest seed)."""
+    """The tier built at x (relative to anchor a, a the nearest seed): the two
+    planes of smallest margin (m1, m2) and the third margin m3."""
     m, u, h, lo = _margins(arr, a, x)
-    m1, m2, c = _second(m)
-    m2 = min(m2, cap)
+    order = np.argsort(m, kind="stable")
+    m1, c = m[order[0]], int(order[0])
+    m2 = min(m[order[1]], cap) if len(m) > 1 else cap
+    c2 = int(order[1]) if len(m) > 1 else None
+    m3 = min(m[order[2]], cap) if len(m) > 2 else cap
     aa = arr.cell_annulus[a]
     amin = float(arr.cell_annulus.min())
     inside = m1 >= aa
     t = dict(c=x.copy(), a=a, alpha_a=aa)
     if m1 >= cap:
-        t.update(u=np.zeros(3), h=cap, k=a, alpha_k=0.0, ra_in=cap - max(aa, 0), ra_out=cap - 0.0,
-                 rk_in=0.0, rk_out=0.0, rout=0.0)
+        t.update(u=np.zeros(3), h=cap, u2=np.zeros(3), h2=cap, k=a, alpha_k=0.0, ra_in=cap - max(aa, 0),
+                 ra_out=cap - 0.0, rk_in=0.0, rk_out=0.0, rout=0.0)
         return t, inside
     k = int(arr.nb_ids[lo + c])
     ak = arr.cell_annulus[k]
-    t.update(u=u[c], h=h[c], k=k, alpha_k=ak, ra_in=m2 - max(aa, 0.0), ra_out=m2)
+    u2, h2 = (u[c2], h[c2]) if c2 is not None else (np.zeros(3), cap)
+    t.update(u=u[c], h=h[c], u2=u2, h2=h2, k=k, alpha_k=ak, ra_in=m3 - max(aa, 0.0), ra_out=m3)
     if inside:
         t.update(rk_in=0.0, rk_out=0.0, rout=0.0)
     else:
@@ -79,10 +85,13 @@ def tier_query(t, q):
     if d < max(t["rout"], 0.0):
         return True, None, False
     s = t["h"] - q @ t["u"]
-    if s >= 0:
-        ins = s >= t["alpha_a"]
+    s2 = t["h2"] - q @ t["u2"]
+    if s >= 0 and s2 >= 0:
+        ins = min(s, s2) >= t["alpha_a"]
         ok = d < max(t["ra_in"] if ins else t["ra_out"], 0.0)
         return ok, t["a"], ins
+    if s >= 0:
+        return False, None, None  # across the second plane: no certificate
     r = -s
     ins = r >= t["alpha_k"]
     ok = d < max(t["rk_in"] if ins else t["rk_out"], 0.0)
