@@ -378,7 +378,7 @@ def main(argv=None):
     kavg = float(np.mean(kern_ms)) * 1e-3
     achieved = flops / kavg / 1e12
     traffic = None
-    pmc = ROOT / "profiles" / "stream_f32_c1_traffic.json"
+    pmc = ROOT / "profiles" / "split_f32_c1_traffic.json"
     if pmc.is_file():
         traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
 
