@@ -99,16 +99,19 @@ def _noop(x):
     return x
 
 
-def _tuning(args, spec, log):
-    """Tuner knobs for the batched runs: explicit flags, else the autotuner's choice."""
+def _tuning(args, spec, log, sample=None, tag=""):
+    """Tuner knobs for the batched runs: explicit flags, else the autotuner's
+    choice, timed on `sample` (requests of the workload itself) when given."""
     if args.classify_mode or args.scan_rounds or args.grid_per_cu or args.no_tune:
         return {"classify_mode": args.classify_mode, "scan_rounds": args.scan_rounds,
                 "grid_per_cu": args.grid_per_cu}, "flags"
-    from custom_madi_amd import tune
+    from custom_madi_amd import batch, tune
     t0 = time.perf_counter()
-    choice = tune.autotune(spec, log=log)
+    preps = [batch.prepare_entry(r) for r in sample] if sample else None
+    choice = tune.autotune(spec, preps=preps, log=log, tag=tag)
     log(f"tuned in {time.perf_counter() - t0:.1f} s: {choice}")
-    return choice.kwargs(), f"autotuner ({tune.cache_path()})"
+    src = f"autotuner ({tune.cache_path()}" + (f", sample of 6 {tag} entries)" if tag else ")")
+    return choice.kwargs(), src
 
 
 def run_c2(args):
@@ -129,7 +132,7 @@ def run_c2(args):
     warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
     batch.run_plain(warm, spec)
 
-    knobs, knob_src = _tuning(args, spec, log)
+    knobs, knob_src = _tuning(args, spec, log, sample=reqs[:: max(1, len(reqs) // 6)][:6], tag="c2")
     walk_knobs = {k: knobs[k] for k in ("classify_mode", "scan_rounds")}
 
     # batched / served form: all requests submitted at t0
@@ -245,7 +248,7 @@ def run_c4(args):
     log = lambda m: print(f"[bench c4] {m}", file=sys.stderr, flush=True)  # noqa: E731
     walker_steps = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs)
     log(f"{len(reqs)} of {len(allreqs)} entries, {walker_steps:.3e} walker-steps")
-    knobs, knob_src = _tuning(args, spec, log)
+    knobs, knob_src = _tuning(args, spec, log, sample=reqs[:: max(1, len(reqs) // 6)][:6], tag="c4")
     sch = batch.DeviceScheduler(spec, tier_cap_um=args.tier_cap, max_tiles_per_generation=args.gen_tiles, **knobs)
     srv = serve.Server(sch, pool=pool)
     t0 = time.perf_counter()

@@ -31,7 +31,7 @@ from typing import Iterable, List, Optional
 from . import _hip, engine
 
 DEFAULT_CANDIDATES = [
-    (4, 1, 3), (4, 1, 4), (4, 2, 4),
+    (4, 1, 3), (4, 1, 4),
     (3, 1, 3), (3, 1, 4),
     (1, 2, 4),
 ]  # (classify_mode, scan_rounds, grid_per_cu)
@@ -54,9 +54,10 @@ def cache_path() -> Path:
     return Path(p) if p else Path.home() / ".cache" / "custom_madi_amd" / "tune.json"
 
 
-def _key(spec: engine.PhaseSpec) -> str:
+def _key(spec: engine.PhaseSpec, tag: str = "") -> str:
     info = _hip.device_info()
-    return f"{info['name']}|{info['arch']}|{info['cu_count']}|{tuple(spec.shape)}"
+    key = f"{info['name']}|{info['arch']}|{info['cu_count']}|{tuple(spec.shape)}"
+    return f"{key}|{tag}" if tag else key
 
 
 def _load() -> dict:
@@ -66,15 +67,15 @@ def _load() -> dict:
         return {}
 
 
-def cached(spec: engine.PhaseSpec) -> Optional[TuneChoice]:
-    rec = _load().get(_key(spec))
+def cached(spec: engine.PhaseSpec, tag: str = "") -> Optional[TuneChoice]:
+    rec = _load().get(_key(spec, tag))
     return TuneChoice(**rec) if rec else None
 
 
-def _store(spec: engine.PhaseSpec, choice: TuneChoice) -> None:
+def _store(spec: engine.PhaseSpec, choice: TuneChoice, tag: str = "") -> None:
     path = cache_path()
     data = _load()
-    data[_key(spec)] = asdict(choice)
+    data[_key(spec, tag)] = asdict(choice)
     try:
         path.parent.mkdir(parents=True, exist_ok=True)
         path.write_text(json.dumps(data, indent=1))
@@ -91,11 +92,16 @@ def sample_entries(n: int = 6, walkers=(250_000, 600_000), seed: int = 71):
 
 def autotune(spec: engine.PhaseSpec, preps: Optional[list] = None,
              candidates: Iterable[tuple] = DEFAULT_CANDIDATES, use_cache: bool = True,
-             log=None) -> TuneChoice:
-    """Time each candidate on the sample and return (and cache) the fastest."""
+             log=None, tag: str = "") -> TuneChoice:
+    """Time each candidate on the sample and return (and cache) the fastest.
+
+    ``preps`` is the sample (default: ``sample_entries()``); a workload with
+    its own sample passes a ``tag`` so its choice is cached apart (the best
+    variant depends on how often a workload's walkers need a classification).
+    """
     from . import batch
     if use_cache:
-        c = cached(spec)
+        c = cached(spec, tag)
         if c is not None:
             return c
     torch = _hip.require_gpu()
@@ -114,5 +120,5 @@ def autotune(spec: engine.PhaseSpec, preps: Optional[list] = None,
             log(f"[tune] classify_mode={mode} scan_rounds={rounds} grid_per_cu={per_cu}: {ch.trial_s * 1e3:.1f} ms")
         if best is None or ch.trial_s < best.trial_s:
             best = ch
-    _store(spec, best)
+    _store(spec, best, tag)
     return best

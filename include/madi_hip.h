@@ -251,7 +251,7 @@ typedef struct {
 typedef struct {
     int32_t grid_per_cu;  /* persistent workgroups per CU (0 = 4) */
     float tier_cap_um;    /* plane-tier cap (0 = first item's walk.tier_cap_um) */
-    int32_t scan_rounds;  /* 1-2; 0 = first item's walk.scan_rounds */
+    int32_t scan_rounds;  /* 1-2 (split walk: its shape); 0 = first item's walk.scan_rounds */
     int32_t classify_mode;  /* as madi_walk_params.classify_mode; 0 = first item's */
 } madi_tune;
 
