@@ -332,6 +332,24 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
         if ((rc = upload(g, alpha.data(), alpha.size(), &D.cell_alpha))) return rc;
         if ((rc = upload(g, alpha64.data(), alpha64.size(), &D.cell_alpha64))) return rc;
         if ((rc = upload(g, hmax.data(), hmax.size(), &D.cell_hmax))) return rc;
+        // one-load records of a cell and of each edge's neighbour, so a scan
+        // of the next cell starts without looking up its list first
+        auto bits = [](float v) {
+            int32_t b;
+            memcpy(&b, &v, sizeof b);
+            return b;
+        };
+        std::vector<int4> crec(nc), krec(padded);
+        std::vector<float> kalpha(padded);
+        for (int c = 0; c < nc; ++c) crec[c] = make_int4(off[c], off[c + 1], bits(hmax[c]), bits(alpha[c]));
+        for (int64_t j = 0; j < padded; ++j) {
+            const int k = ids[j];
+            krec[j] = make_int4(k, off[k], off[k + 1], bits(hmax[k]));
+            kalpha[j] = alpha[k];
+        }
+        if ((rc = upload(g, crec.data(), crec.size(), &D.cell_rec))) return rc;
+        if ((rc = upload(g, krec.data(), krec.size(), &D.nb_krec))) return rc;
+        if ((rc = upload(g, kalpha.data(), kalpha.size(), &D.nb_kalpha))) return rc;
         if ((rc = upload(g, d->cell_xyz, (size_t)nc * 3, &D.cell_xyz))) return rc;
         if (d->hint_n < 1 || !d->hint) {
             set_error("geometry: a hint grid is required with the Delaunay graph");

@@ -78,6 +78,9 @@ struct GeomDev {
     const float* cell_alpha;    // per cell
     const double* cell_alpha64;
     const float* cell_hmax;     // max half edge length per cell
+    const int4* cell_rec;       // per cell: (list offset, list end, hmax bits, alpha bits), one load
+    const int4* nb_krec;        // per edge slot: neighbour k and k's (list offset, list end, hmax bits)
+    const float* nb_kalpha;     // per edge slot: alpha of the neighbour
     const double* cell_xyz;     // per cell world coordinates
     const int32_t* hint;        // hint_n^3 cells: a seed near each hint-grid box centre
     int32_t n_cells;
@@ -108,6 +111,13 @@ __device__ __forceinline__ float4 gload(const float4* p)
     typedef float v4f __attribute__((ext_vector_type(4)));
     const v4f v = *(const __attribute__((address_space(1))) v4f*)p;
     return make_float4(v.x, v.y, v.z, v.w);
+}
+
+__device__ __forceinline__ int4 gload(const int4* p)
+{
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i v = *(const __attribute__((address_space(1))) v4i*)p;
+    return make_int4(v.x, v.y, v.z, v.w);
 }
 
 constexpr int kMaxGraphIters = 256;

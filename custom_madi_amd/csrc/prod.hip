@@ -348,26 +348,35 @@ __device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 struct ScanState {
     int cur, end, c, c2, cell;
     float m1, m2, m3, xb, pe, eps;  // xb: lower bound of every unvisited edge's margin
+    float alpha;       // the cell's annulus
     float nx, ny, nz;  // -s
 };
 
-// Bounds of the scan of S.cell at s = -(nx, ny, nz).  eps bounds the FP32
+// Bounds of the scan of S.cell at s = -(nx, ny, nz), given the cell's edge
+// list [off, end), largest half length hmax and annulus.  eps bounds the FP32
 // rounding of every margin of the cell (three fused roundings of terms at
 // most h + |s|_1, plus the FP32 unit vectors and half lengths: below
 // 4 * 2^-24 * (|s|_1 + h_max)); pe is an upper bound of |s| (the hardware
 // square root, 1 ulp, of the rounded sum of squares, raised by 2^-20
 // relative) plus eps, so h - pe is a lower bound of the computed margin of
 // any edge of half length h.
-__device__ __forceinline__ void scan_begin(const GeomDev& g, ScanState& S)
+__device__ __forceinline__ void scan_setup(ScanState& S, int off, int end, float hmax, float alpha)
 {
-    const int cell = S.cell;
-    S.cur = gload(g.nb_off + cell);
-    S.end = gload(g.nb_off + cell + 1);
+    S.cur = off;
+    S.end = end;
+    S.alpha = alpha;
     S.c = S.c2 = -1;
     S.m1 = S.m2 = S.m3 = S.xb = __builtin_inff();
-    S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + gload(g.cell_hmax + cell));
+    S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + hmax);
     const float n2 = fmaf(S.nx, S.nx, fmaf(S.ny, S.ny, S.nz * S.nz));
     S.pe = fmaf(__builtin_amdgcn_sqrtf(n2), 1.f + 0x1.0p-20f, S.eps);
+}
+
+// The same with the cell's record read from the geometry (one load).
+__device__ __forceinline__ void scan_begin(const GeomDev& g, ScanState& S)
+{
+    const int4 r = gload(g.cell_rec + S.cell);
+    scan_setup(S, r.x, r.y, __int_as_float(r.z), __int_as_float(r.w));
 }
 
 __device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx, float sy, float sz, ScanState& S)
@@ -598,7 +607,7 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         if (++P.moves > kMaxGraphIters) return -1;  // the greedy walk does not settle
         next = 1;
     } else {
-        const float a1 = gload(g.cell_alpha + P.na);
+        const float a1 = S.alpha;  // the annulus of P.na, the cell just scanned
         P.pin = S.m1 >= a1;
         P.sa_c = S.c;
         P.sa_c2 = S.c2;
@@ -625,10 +634,13 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         }
         next = 2;
     }
-    // the point relative to neighbour k of the binding edge
+    // the point relative to neighbour k of the binding edge, and k's record:
+    // one round of loads, after which k's scan starts with its first group
     const int c = S.c;
     const double ex = gload(g.nb_e64 + 3 * c), ey = gload(g.nb_e64 + 3 * c + 1), ez = gload(g.nb_e64 + 3 * c + 2);
-    const int k = gload(g.nb_id + c);
+    const int4 kr = gload(g.nb_krec + c);
+    const float ka = gload(g.nb_kalpha + c);
+    const int k = kr.x;
     const float kx = (float)((double)P.qx - ex);
     const float ky = (float)((double)P.qy - ey);
     const float kz = (float)((double)P.qz - ez);
@@ -646,7 +658,7 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
     S.nx = -kx;
     S.ny = -ky;
     S.nz = -kz;
-    S.cur = -1;
+    scan_setup(S, kr.y, kr.z, __int_as_float(kr.w), ka);
     phase = next == 1 ? kScanA : kScanK;
     return 0;
 }
