@@ -269,6 +269,10 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
         for (int c = 0; c < nc; ++c) {
             off[c] = (int32_t)padded;
             const int64_t deg = d->nb_offsets[c + 1] - d->nb_offsets[c];
+            if (deg > kMaxCellEdges) {
+                set_error("geometry: a cell has more than 2^24 - 8 edges");
+                return MADI_ERR_ARG;
+            }
             padded += (deg + kEdgePad - 1) / kEdgePad * kEdgePad;
             if (padded > INT32_MAX) {
                 set_error("geometry: too many edges");

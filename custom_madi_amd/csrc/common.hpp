@@ -127,5 +127,8 @@ constexpr int kMaxGraphIters = 256;
 // vector of length 1e30): their margin is +inf in FP32 and their bisector is
 // never the nearer one in FP64, so scans may read whole groups of kEdgePad.
 constexpr int kEdgePad = 8;
+// Largest padded edge list of one cell: the split walk's requests carry a
+// list length in 24 bits.
+constexpr int kMaxCellEdges = (1 << 24) - kEdgePad;
 
 }  // namespace madi

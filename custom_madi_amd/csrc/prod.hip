@@ -168,6 +168,8 @@ struct Tier {
     float alpha_a, alpha_k;
     float eps;                     // rounding bound of s over the validity ball
     int k;
+    int a_off, a_end;              // the anchor's edge list (a_off < 0: not known)
+    float a_hmax;                  // and its largest half length
 };
 
 __device__ __forceinline__ void tier_clear(Tier& t)
@@ -181,6 +183,9 @@ __device__ __forceinline__ void tier_clear(Tier& t)
     t.alpha_a = t.alpha_k = 0.f;
     t.eps = 0.f;
     t.k = -1;
+    t.a_off = -1;
+    t.a_end = 0;
+    t.a_hmax = 0.f;
 }
 
 // Free water: every point is outside.
@@ -458,7 +463,11 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2
     const int k = plane ? gload(g.nb_id + ci) : a;
     const float4 e = plane ? e0 : make_float4(0.f, 0.f, 0.f, m2a);
     const float4 e2 = plane2 ? e20 : make_float4(0.f, 0.f, 0.f, fmaxf(m3a, 0.f));
-    const float aa = gload(g.cell_alpha + a), ak = gload(g.cell_alpha + k);
+    const int4 ar = gload(g.cell_rec + a);
+    const float aa = __int_as_float(ar.w), ak = gload(g.cell_alpha + k);
+    t.a_off = ar.x;
+    t.a_end = ar.y;
+    t.a_hmax = __int_as_float(ar.z);
     kside = kside && plane;
     const float ep = kside ? fmaxf(epsa, epsk) : epsa;
     const float ra_in = m3a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m3a - 2.f * ep;
@@ -809,6 +818,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
             } else {
                 ++n_class;
                 pending_start(P, S, phase);
+                if (t.a_off >= 0) scan_setup(S, t.a_off, t.a_end, t.a_hmax, t.alpha_a);  // the anchor is the tier's
             }
         }
 #pragma unroll 1
@@ -1165,6 +1175,7 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
                     } else {
                         ++n_class;
                         pending_start(P, S, phase);
+                        if (t.a_off >= 0) scan_setup(S, t.a_off, t.a_end, t.a_hmax, t.alpha_a);  // the tier's anchor
                     }
                 }
 #pragma unroll 1
@@ -1359,18 +1370,22 @@ constexpr int kReqSlots = 256;  // >= walker lanes per workgroup (power of two)
 constexpr int kSplitRounds = 2;  // classification rounds per classifier iteration
 constexpr uint32_t kSplitStallLimit = 1u << 26;
 
+// 48 bytes: a ring of kReqSlots per classifier wave
 struct ReqSlot {
-    int seq, na, owner, pad0;
+    int seq, na;
+    int off;       // the anchor's edge list (< 0: not known, the classifier loads it)
+    int len_own;   // bits 0-23: that list's padded length; bits 24-31: the requesting lane
+    float hmax, alpha;  // the anchor's largest half length and annulus
     float qx, qy, qz, ox, oy, oz;
-    int pad[2];
 };
 
+// 128 bytes, the tier first (16-byte aligned in the array)
 struct RespSlot {
-    int status, na, flags, pad;  // flags: bit 0 inside, bits 1-3 tmode, bit 4 classifier failure
-    float qx, qy, qz, ox, oy, oz;
-    float pad2[2];
     Tier t;
+    int status, na, flags;  // flags: bit 0 inside, bits 1-3 tmode, bit 4 classifier failure
+    float qx, qy, qz, ox, oy, oz;
 };
+static_assert(sizeof(ReqSlot) == 48 && sizeof(RespSlot) == 128, "split ring slots");
 
 template <int NP, int WW, int CW>
 struct SplitLds {
@@ -1579,7 +1594,11 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
                             ticket = base + lanes_below(em);
                             ReqSlot& rq = L.req[wv % CW][ticket & (kReqSlots - 1)];
                             rq.na = qna;
-                            rq.owner = wv * 64 + lane;
+                            // a step's anchor is the tier's: its record rides along
+                            rq.off = init ? -1 : t.a_off;
+                            rq.len_own = ((t.a_end - t.a_off) & 0xffffff) | ((wv * 64 + lane) << 24);
+                            rq.hmax = t.a_hmax;
+                            rq.alpha = t.alpha_a;
                             rq.qx = qx;
                             rq.qy = qy;
                             rq.qz = qz;
@@ -1806,15 +1825,19 @@ __device__ void split_classifier(float cap_width, unsigned long long* counters, 
                 ticket = head + (unsigned int)rank;
                 const ReqSlot& rq = L.req[cw][ticket & (kReqSlots - 1)];
                 P.na = rq.na;
-                owner = rq.owner;
+                const int len_own = rq.len_own;
+                owner = (int)((unsigned)len_own >> 24);
                 P.qx = rq.qx;
                 P.qy = rq.qy;
                 P.qz = rq.qz;
                 P.ox = rq.ox;
                 P.oy = rq.oy;
                 P.oz = rq.oz;
+                const int off = rq.off, end = off + (len_own & 0xffffff);
+                const float hmax = rq.hmax, alpha = rq.alpha;
                 busy = 1;
                 pending_start(P, S, cphase);
+                if (off >= 0) scan_setup(S, off, end, hmax, alpha);
             }
             head += (unsigned int)take;
         }
@@ -1957,6 +1980,7 @@ __global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(
     unsigned long long* counters)
 {
     static_assert(kReqSlots >= 64 * ((WW + CW - 1) / CW), "a ring needs a slot per walker lane it serves");
+    static_assert(WW <= 4, "a request names its lane in 8 bits");
     __shared__ SplitLds<NP, WW, CW> lds;
     constexpr int kThreads = (WW + CW) * 64;
     for (int j = threadIdx.x; j < kMaxLdsGrid; j += kThreads)
