@@ -28,11 +28,11 @@ MADI_ERR_BUSY = 6
 MADI_MAX_PAIRS = 8
 MADI_MAX_B = 16
 MADI_MAX_DIRS = 64
-N_COUNTERS = 12
+N_COUNTERS = 13
 N_FLAGS = 2
 COUNTER_NAMES = (
     "classify", "safe_hit", "graph_iters", "fp64_recheck", "facets", "kd_nodes", "crossings", "rejects",
-    "wave_iters", "lane_iters", "class_iters", "class_busy",
+    "wave_iters", "lane_iters", "class_iters", "class_busy", "exits",
 )
 
 _p = C.c_void_p

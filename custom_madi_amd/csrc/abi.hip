@@ -548,7 +548,16 @@ static int32_t plan_grid_span(const madi_phase_plan* plan)
 struct BatchKnobs {
     float cap_width;
     int rounds, variant, grid, emit_grid;
+    bool exits;  // compile the exit certificate into the streamed walks
 };
+
+// Whether any item of a batch table can use the exit certificate.
+static bool table_exits(const WorkDev* table, int32_t n_work)
+{
+    for (int i = 0; i < n_work; ++i)
+        if (exit_certificate_pays(table[i].g.alpha_max, table[i].w.sigma)) return true;
+    return false;
+}
 
 static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, BatchKnobs& k)
 {
@@ -568,6 +577,7 @@ static int batch_knobs(const madi_tune* tune, const madi_walk_params* first, Bat
     const int per_cu = tune && tune->grid_per_cu > 0 ? tune->grid_per_cu : 4;
     k.grid = cus * per_cu;
     k.emit_grid = cus * 8;
+    k.exits = true;  // callers that know their entries narrow it
     return MADI_OK;
 }
 
@@ -669,8 +679,10 @@ static int launch_batch(const WorkDev* table, unsigned long long* item_next, Bat
     if (!streamed(k.variant))
         return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
     int rc = k.variant == kVariantSplit
-                 ? launch_split_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s)
-                 : launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
+                 ? launch_split_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, k.exits, grid, counters,
+                                    s)
+                 : launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, k.exits, grid,
+                                     counters, s);
     if (rc) return rc;
     return launch_emit_tiles(table, hdr, plan, ev.n_pairs, k.emit_grid, s);
 }
@@ -933,6 +945,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         tn.tier_cap_um = p->tier_cap_um;
         BatchKnobs k;
         if ((rc = batch_knobs(&tn, p, k))) return rc;
+        k.exits = table_exits((const WorkDev*)(img.data() + (lay.table - lay.hdr)), 1);
         const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
         rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next), (BatchHdr*)(base + lay.hdr),
                           dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
@@ -1048,6 +1061,7 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     h->n_work = n_work;
     h->total_walkers = walkers;
     if (total == 0) return MADI_OK;
+    k.exits = table_exits((const WorkDev*)(img.data() + batch_off_table()), n_work);
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
     hipStream_t s = (hipStream_t)stream;

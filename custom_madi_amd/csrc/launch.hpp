@@ -28,6 +28,15 @@ struct WalkDev {
     uint64_t rej2;
 };
 
+// The walk's exit certificate (an inside walker's proposal past its cell,
+// rejected for either crossing kind, needs no classification) pays where
+// such exits come past a tier's ball, i.e. where the annulus is short next to
+// a step: below 2.5 step deviations for the geometry's largest annulus.
+__host__ __device__ inline bool exit_certificate_pays(float alpha_max, double sigma)
+{
+    return alpha_max < 2.5 * sigma;
+}
+
 struct FusedOut {
     const PlanDev* plan;   // device copy
     Events ev;
@@ -94,12 +103,14 @@ int launch_walk_f32_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
                           float cap_width, int rounds, unsigned long long* counters, hipStream_t s);
 int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, const Events& ev, const Gf2Col* jump_pow,
                      float cap_width, int rounds, int32_t grid, unsigned long long* counters, hipStream_t s);
+// exits: compile in the exit certificate (worth it when some entry of the
+// batch passes exit_certificate_pays; results are the same either way)
 int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
-                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, bool exits, int32_t grid,
                       unsigned long long* counters, hipStream_t s);
 // shape 1: 3 walker waves + 1 classifier wave per workgroup; 2: 2 + 2
 int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, int32_t grid,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, bool exits, int32_t grid,
                      unsigned long long* counters, hipStream_t s);
 int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,
                       hipStream_t s);

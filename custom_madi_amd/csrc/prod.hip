@@ -197,6 +197,17 @@ __device__ __forceinline__ void tier_free_water(Tier& t)
     t.ra_out2 = __builtin_inff();
 }
 
+// |q|_1 bound below which tier_query may certify an exit of an inside
+// walker's cell, 0 for none.  In a periodic box 0.49 L keeps the anchor's
+// seed nearer to q than any periodic image of it.  Exits past the tier's
+// ball need a step longer than the annulus, rare once every annulus is 2.5
+// step deviations or more: the certificate is then not worth its test.
+__device__ __forceinline__ float exit_bound(const GeomDev& g, const WalkDev& p)
+{
+    if (!exit_certificate_pays(g.alpha_max, p.sigma)) return 0.f;
+    return g.periodic ? (float)(0.49 * g.L) : __builtin_inff();
+}
+
 // Tier answer for q (relative to the anchor a); returns 1 when certified.
 //
 // Let x be the centre, m1 <= m2 <= m3 the smallest margins of a's planes at x
@@ -251,6 +262,27 @@ __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float 
     cell = t.k;
     inside = ins;
     return 1;
+}
+
+// Exit certificate for a proposal q the tier does not label, from a walker
+// inside its anchor cell a whose draw rejects both crossing kinds (rj == 3):
+// q lies outside a's contracted cell when plane 1 (on a's side: plane 1 or
+// 2) has a margin below alpha_a by more than the rounding bound at q (the
+// centre's plus kUlp * |q - x|_1).  Both planes are edges of a when k != a:
+// without a second edge below the cap, plane 2 is (0, h2 = cap) and never
+// passes.  |q|_1 < leave (exit_bound) keeps q nearer to a's seed than to any
+// periodic image of it, so q lies in another cell or outside a's contracted
+// one: the walker crosses one membrane or two, and rj == 3 rejects either.
+// The caller then completes the step as a rejected crossing, the outcome
+// its true label gives.
+__device__ __forceinline__ bool tier_exit(const Tier& t, int a, float qx, float qy, float qz, float leave)
+{
+    const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
+    const float s2 = t.h2 - (qx * t.ux2 + qy * t.uy2 + qz * t.uz2);
+    const float sm = s >= 0.f ? fminf(s, s2) : s;
+    const float lim =
+        t.alpha_a - fmaf(kUlp * 1.125f, fabsf(qx - t.cx) + fabsf(qy - t.cy) + fabsf(qz - t.cz), t.eps);
+    return t.k != a && sm < lim && fabsf(qx) + fabsf(qy) + fabsf(qz) < leave;
 }
 
 // Hint-grid anchor for a world point.
@@ -679,7 +711,10 @@ __device__ __forceinline__ void tier_of(const GeomDev& g, const Pending& P, int 
     const float none = -__builtin_inff();
     const float m2a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m2);
     const float m3a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m3);
-    tier_make(g, P.na, tmode == 3 ? -1 : P.sa_c, tmode == 3 ? -1 : P.sa_c2, P.qx, P.qy, P.qz, P.sa_m1, m2a, m3a,
+    // tmode 2 (FP64 re-decision) may have moved the anchor off the scanned
+    // cell, whose edges are then not a's: no planes (the tier answers nothing)
+    const bool planes = tmode != 2 && tmode != 3;
+    tier_make(g, P.na, planes ? P.sa_c : -1, planes ? P.sa_c2 : -1, P.qx, P.qy, P.qz, P.sa_m1, m2a, m3a,
               P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps, t);
 }
 
@@ -1001,8 +1036,8 @@ __device__ __forceinline__ int64_t stream_claim(const WorkDev* __restrict__ W, u
     return base;
 }
 
-template <int NP, int R>
-__device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
+template <int NP, int R, bool EXIT>
+__device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
                             const PlanDev& plan,
                             const Gf2Col* __restrict__ jump_pow, const Events& ev, float cap_width,
                             unsigned long long* counters, StreamLds<NP>& L)
@@ -1015,7 +1050,7 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
     Rng* st = L.states[wv];
     for (int j = lane; j < kMaxLdsGrid; j += 64) occ[j] = 0;
     wave_lds_fence();
-    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0;
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, nfac = 0, n_exit = 0;
     uint32_t iters = 0, facets = 0, rechecks = 0;
     uint32_t wave_iters = 0, lane_iters = 0;
     const int n_pairs = plan.n_pairs;
@@ -1047,6 +1082,7 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
             const float cap = g.alpha_max + cap_width;
             const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
             const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
+            const float leave = EXIT ? exit_bound(g, p) : 0.f;  // EXIT: the certificate is compiled in
             const float ts = (float)p.ts;
             int64_t hand = claim, hand_end = claim + kClaimWalkers < n_walk ? claim + kClaimWalkers : n_walk;
             bool item_dry = false;
@@ -1171,6 +1207,14 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
                             }
                     } else if (tier_query(t, a, P.qx, P.qy, P.qz, pcell, P.pin)) {
                         ++n_tier;
+                        complete = 1;
+                    } else if (EXIT && leave > 0.f && in && cell == a && rj == 3 &&
+                               tier_exit(t, a, P.qx, P.qy, P.qz, leave)) {
+                        // a certified exit completes as the rejected crossing "outside, cell a"
+                        pcell = a;
+                        P.pin = 0;
+                        ++n_tier;
+                        ++n_exit;
                         complete = 1;
                     } else {
                         ++n_class;
@@ -1336,6 +1380,10 @@ __device__ void stream_wave(const WorkDev* __restrict__ work, unsigned long long
             atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
             atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
         }
+        if (EXIT) {
+            const double v8 = wave_sum_f64((double)n_exit);
+            if (lane == 0) atomicAdd(&counters[MADI_CNT_EXITS], (unsigned long long)v8);
+        }
     }
 }
 
@@ -1411,8 +1459,8 @@ __device__ __forceinline__ void lds_store_release(int* p, int v)
 
 // One walker wave (wv < WW): the streamed walk's walker life cycle with the
 // classification replaced by a request to the classifier wave.
-template <int NP, int WW, int CW>
-__device__ void split_walker(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
+template <int NP, int WW, int CW, bool EXIT>
+__device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
                              const PlanDev& plan, const Gf2Col* __restrict__ jump_pow, const Events& ev,
                              unsigned long long* counters, SplitLds<NP, WW, CW>& L, int wv)
 {
@@ -1425,7 +1473,7 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
     RespSlot& my_resp = L.resp[wv * 64 + lane];
     for (int j = lane; j < kMaxLdsGrid; j += 64) occ[j] = 0;
     wave_lds_fence();
-    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0;
+    uint32_t n_class = 0, n_tier = 0, crossings = 0, rejects = 0, n_exit = 0;
     uint32_t wave_iters = 0, lane_iters = 0;
     const int n_pairs = plan.n_pairs;
     if (n_work > 0 && total_w > 0) {
@@ -1459,6 +1507,7 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
             const int64_t n_walk = W->n_walk, walker_lo = W->walker_lo;
             const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
             const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
+            const float leave = EXIT ? exit_bound(g, p) : 0.f;  // EXIT: the certificate is compiled in
             const float ts = (float)p.ts;
             int64_t hand = claim, hand_end = claim + kClaimWalkers < n_walk ? claim + kClaimWalkers : n_walk;
             bool item_dry = false;
@@ -1576,6 +1625,12 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
                         complete = 1;
                         rna = pcell;
                         rflags = pin;
+                    } else if (EXIT && leave > 0.f && in && cell == a && rj == 3 && tier_exit(t, a, qx, qy, qz, leave)) {
+                        ++n_tier;  // a certified rejected exit (see stream_wave)
+                        ++n_exit;
+                        complete = 1;
+                        rna = a;
+                        rflags = 0;
                     } else {
                         ++n_class;
                         enq = 1;
@@ -1784,6 +1839,10 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
             atomicAdd(&counters[MADI_CNT_WAVE_ITERS], (unsigned long long)wave_iters);
             atomicAdd(&counters[MADI_CNT_LANE_ITERS], (unsigned long long)v7);
         }
+        if (EXIT) {
+            const double v8 = wave_sum_f64((double)n_exit);
+            if (lane == 0) atomicAdd(&counters[MADI_CNT_EXITS], (unsigned long long)v8);
+        }
     }
 }
 
@@ -1792,7 +1851,7 @@ __device__ void split_walker(const WorkDev* __restrict__ work, unsigned long lon
 // geometry slot), answered with the new tier.  Lanes may work on different
 // entries.
 template <int NP, int WW, int CW>
-__device__ void split_classifier(float cap_width, unsigned long long* counters, SplitLds<NP, WW, CW>& L, int cw)
+__device__ __forceinline__ void split_classifier(float cap_width, unsigned long long* counters, SplitLds<NP, WW, CW>& L, int cw)
 {
     const int lane = threadIdx.x & 63;
     // walker lanes wait for this wave's answers: let it issue ahead of the
@@ -1952,8 +2011,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 }
 
 // Streamed walk over a batch (see stream_wave).  The event table is the
-// batch's, so every workgroup builds it once.
-template <int NP, int R>
+// batch's, so every workgroup builds it once.  EXIT compiles in the exit
+// certificate (tier_exit; it costs registers, so launches whose entries never
+// use it go without).
+template <int NP, int R, bool EXIT>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_stream_f32(
     const WorkDev* __restrict__ work, unsigned long long* __restrict__ item_next, const BatchHdr* __restrict__ hdr,
     const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
@@ -1968,12 +2029,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
         for (int e = 0; e < ev.n; ++e)
             if (ev.j[e] < kMaxLdsGrid) lds.evc[ev.j[e]][ev.pair[e]] += (float)ev.sign[e];
     __syncthreads();
-    stream_wave<NP, R>(work, item_next, hdr, *plan, jump_pow, ev, cap_width, counters, lds);
+    stream_wave<NP, R, EXIT>(work, item_next, hdr, *plan, jump_pow, ev, cap_width, counters, lds);
 }
 
 // Split walk over a batch (see split_walker / split_classifier): WW walker
-// waves and CW classifier waves per workgroup.
-template <int NP, int WW, int CW>
+// waves and CW classifier waves per workgroup; EXIT as for k_stream_f32.
+template <int NP, int WW, int CW, bool EXIT>
 __global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_split_f32(
     const WorkDev* __restrict__ work, unsigned long long* __restrict__ item_next, const BatchHdr* __restrict__ hdr,
     const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
@@ -1999,7 +2060,7 @@ __global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(
     // the CU's SIMDs
     const int r = (int)((threadIdx.x / 64 + blockIdx.x) % (WW + CW));
     if (r >= WW) split_classifier<NP, WW, CW>(cap_width, counters, lds, r - WW);
-    else split_walker<NP, WW, CW>(work, item_next, hdr, *plan, jump_pow, ev, counters, lds, r);
+    else split_walker<NP, WW, CW, EXIT>(work, item_next, hdr, *plan, jump_pow, ev, counters, lds, r);
 }
 
 // Column sums of every tile of a streamed batch from its walkers' stored
@@ -2162,24 +2223,35 @@ int launch_batch_f32(const WorkDev* work, BatchHdr* hdr, const PlanDev* plan, co
 }
 
 int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
-                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, int32_t grid,
+                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, bool exits, int32_t grid,
                       unsigned long long* counters, hipStream_t s)
 {
-#define MADI_LAUNCH_STREAM(NPV, RV)                                                                                 \
-    hipLaunchKernelGGL((k_stream_f32<NPV, RV>), dim3(grid), dim3(kBlock), 0, s, work, item_next, hdr, plan, ev,    \
+#define MADI_LAUNCH_STREAM_X(NPV, RV, XV)                                                                           \
+    hipLaunchKernelGGL((k_stream_f32<NPV, RV, XV>), dim3(grid), dim3(kBlock), 0, s, work, item_next, hdr, plan, ev, \
                        jump_pow, cap_width, counters)
+#define MADI_LAUNCH_STREAM(NPV, RV)                                \
+    do {                                                           \
+        if (exits) MADI_LAUNCH_STREAM_X(NPV, RV, true);            \
+        else MADI_LAUNCH_STREAM_X(NPV, RV, false);                 \
+    } while (0)
     MADI_DISPATCH_NP(ev.n_pairs, rounds, MADI_LAUNCH_STREAM);
 #undef MADI_LAUNCH_STREAM
+#undef MADI_LAUNCH_STREAM_X
     return check_launch("k_stream_f32");
 }
 
 int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
-                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, int32_t grid,
+                     const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, bool exits, int32_t grid,
                      unsigned long long* counters, hipStream_t s)
 {
-#define MADI_LAUNCH_SPLIT(NPV, WWV, CWV)                                                                             \
-    hipLaunchKernelGGL((k_split_f32<NPV, WWV, CWV>), dim3(grid), dim3((WWV + CWV) * 64), 0, s, work, item_next, hdr, \
-                       plan, ev, jump_pow, cap_width, counters)
+#define MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, XV)                                                                  \
+    hipLaunchKernelGGL((k_split_f32<NPV, WWV, CWV, XV>), dim3(grid), dim3((WWV + CWV) * 64), 0, s, work,        \
+                       item_next, hdr, plan, ev, jump_pow, cap_width, counters)
+#define MADI_LAUNCH_SPLIT(NPV, WWV, CWV)                           \
+    do {                                                           \
+        if (exits) MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, true);       \
+        else MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, false);            \
+    } while (0)
 #define MADI_SPLIT_SHAPE(NPV)                         \
     do {                                              \
         if (shape <= 1) MADI_LAUNCH_SPLIT(NPV, 3, 1); \
@@ -2191,6 +2263,7 @@ int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const B
     else MADI_SPLIT_SHAPE(MADI_MAX_PAIRS);
 #undef MADI_SPLIT_SHAPE
 #undef MADI_LAUNCH_SPLIT
+#undef MADI_LAUNCH_SPLIT_X
     return check_launch("k_split_f32");
 }
 

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 2
+#define MADI_ABI_VERSION 3
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -51,7 +51,8 @@ extern "C" {
 #define MADI_CNT_LANE_ITERS 9    /* FP32 walk: iterations until each lane's walk ended, summed over lanes */
 #define MADI_CNT_CLASS_ITERS 10  /* split FP32 walk: classifier-wave iterations, summed over waves */
 #define MADI_CNT_CLASS_BUSY 11   /* split FP32 walk: busy classifier lanes, summed over those iterations */
-#define MADI_N_COUNTERS 12
+#define MADI_CNT_EXITS 12        /* streamed FP32 walks: rejected exits certified by the tier (within SAFE_HIT) */
+#define MADI_N_COUNTERS 13
 
 /* Flags summed over walkers (int64 each). */
 #define MADI_FLAG_ESCAPED 0

@@ -186,6 +186,37 @@ def test_tuner_choices_give_identical_results(spec):
         assert np.array_equal(out.cos, ref.cos) and np.array_equal(out.occupancy, ref.occupancy)
 
 
+def test_exit_certificate_changes_no_result(spec):
+    """The streamed walks (modes 3 and 4) answer an inside walker's proposal
+    past its cell without classifying it when the draw rejects both crossing
+    kinds (tier_exit); the per-tile walk (mode 1) classifies every such
+    proposal.  With annuli far below a step deviation (v_i = 0.95) the
+    certificate fires and the sums still equal mode 1's bit for bit, single
+    entry and batched; with annuli above 2.5 deviations (v_i = 0.5) it stays
+    off."""
+    for vi, fires in ((0.95, True), (0.5, False)):
+        req = batch.library_requests(1, seed=41, walkers=(20_000, 20_000), cells=(200, 200), V=(2.0, 2.0),
+                                     vi=(vi, vi), T_max_ms=12.0)[0]
+        prep = batch.prepare_entry(req)
+        geom = DeviceGeometry(prep.arrays)
+        ref = None
+        for cw in (1, 3, 4):
+            p = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed, classify_mode=cw)
+            res = engine.walk_fused(geom, p, spec, precision=32)
+            c = res.counters
+            say(f"v_i {vi} classify_mode {cw}: {c['exits']} certified exits, {c['classify']} classifications")
+            assert res.n_escaped == 0 and res.n_error == 0
+            if ref is None:
+                assert c["exits"] == 0
+                ref = res
+                continue
+            assert (c["exits"] > 0) == fires
+            assert np.array_equal(res.cos, ref.cos) and np.array_equal(res.sin, ref.sin)
+            assert np.array_equal(res.occupancy, ref.occupancy)
+        out = batch.run_batched([prep], spec, classify_mode=4)[0]
+        assert np.array_equal(out.cos, ref.cos) and np.array_equal(out.occupancy, ref.occupancy)
+
+
 def test_autotuner_picks_a_candidate_and_caches_it(spec, tmp_path, monkeypatch):
     from custom_madi_amd import tune
     monkeypatch.setenv("MADI_TUNE_CACHE", str(tmp_path / "tune.json"))

@@ -187,3 +187,63 @@ def test_outside_ball_third_cell_term_is_tight():
         assert bool(np.any(ins)) == expect_inside
         if expect_inside:
             assert set(np.asarray(lab)[np.asarray(ins).astype(bool)].tolist()) == {j_cell}
+
+
+def tier_leaves(t, q, L):
+    """csrc/prod.hip:tier_query's exit certificate (result 2), without rounding slack: with k != a, the smaller
+    margin of planes 1 and 2 puts q outside a's contracted cell (plane 2 without a second edge is (0, cap) and
+    never does), and |q|_1 < 0.49 L keeps a's own seed nearer to q than any periodic image of it."""
+    if t["k"] == t["a"]:
+        return False
+    s = t["h"] - q @ t["u"]
+    sm = min(s, t["h2"] - q @ t["u2"]) if s >= 0 else s  # on k's side plane 1 alone
+    return bool(sm < t["alpha_a"] and np.abs(q).sum() < 0.49 * L)
+
+
+def _exit_cases(ens, n_centres, reach, seed):
+    """Inside centres x of random cells and proposals q = x + d (|d| uniform up to reach): the certified exits
+    and the brute-force labels of those q."""
+    arr = geometry_arrays(ens, with_kd=False)
+    cap = float(arr.cell_annulus.max()) + CAP_WIDTH
+    rng = np.random.default_rng(seed)
+    pts, cells, far, n_exit = [], [], 0, 0
+    for x_world in rng.uniform(0, ens.L, size=(20 * n_centres, 3)):
+        a0 = int(np.argmin(np.linalg.norm(((arr.cell_xyz - x_world + ens.L / 2) % ens.L) - ens.L / 2, axis=1)))
+        a, x = _locate(arr, a0, ((x_world - arr.cell_xyz[a0] + ens.L / 2) % ens.L) - ens.L / 2)
+        t, inside = tier_make(arr, a, x, cap)
+        if not inside:
+            continue
+        d = rng.normal(size=(80, 3))
+        d *= (reach * rng.uniform(0, 1, size=(80, 1))) / np.linalg.norm(d, axis=1, keepdims=True)
+        for q in x + d:
+            m = _margins(arr, a, q)[0]
+            n_exit += bool(m.min() < t["alpha_a"])
+            if tier_leaves(t, q, ens.L):
+                pts.append((arr.cell_xyz[a] + q) % ens.L)
+                cells.append(a)
+                far += bool(np.linalg.norm(q) > 0.3 * ens.L)
+        n_centres -= 1
+        if n_centres == 0:
+            break
+    lab, ins = O.periodic_brute(ens.seeds, ens.annulus, ens.L, np.asarray(pts))
+    return np.asarray(cells), np.asarray(lab), np.asarray(ins).astype(bool), far, n_exit
+
+
+def test_certified_exits_leave_the_contracted_cell():
+    """Every proposal the tier certifies as an exit of an inside walker's cell a lies in another cell or outside
+    a's contracted cell (brute-force labels of the 27-image box), so a step rejected for both crossing kinds
+    needs no classification.  Most geometric exits near the centre are certified."""
+    ens = _geom.periodic(n_cells=80, seed=5)
+    cells, lab, ins, _, n_exit = _exit_cases(ens, 400, 2.0, seed=23)
+    assert len(cells) > 2000 and len(cells) > 0.5 * n_exit
+    assert not np.any((lab == cells) & ins)
+
+
+def test_certified_exits_exclude_periodic_images_of_the_cell():
+    """A 2 x 2 x 2 lattice box (L = 4): proposals reaching a periodic image of the walker's own cell carry its
+    index, so an exit certificate there would skip a move with no crossing.  The |q| < 0.49 L bound excludes
+    them; certified exits still include points well away from the centre."""
+    ens = _geom.lattice(n=2, h=2.0, jitter=1e-3, annulus=0.3, seed=2)
+    cells, lab, ins, far, _ = _exit_cases(ens, 60, 1.6 * ens.L, seed=29)
+    assert len(cells) > 200 and far > 40
+    assert not np.any((lab == cells) & ins)
