@@ -1905,6 +1905,16 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
                 head == (unsigned int)__hip_atomic_load(&L.tail[cw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_WORKGROUP))
                 break;
             __builtin_amdgcn_s_sleep(2);
+            // no lane holds a classification: its state is dead here
+            P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
+            P.na = P.pin = P.moves = 0;
+            P.sa_c = P.sa_c2 = -1;
+            P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+            S.cur = S.end = 0;
+            S.c = S.c2 = -1;
+            S.cell = 0;
+            S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = S.alpha = 0.f;
+            owner = cphase = 0;
             continue;
         }
         ++c_iters;
