@@ -160,7 +160,8 @@ __device__ __forceinline__ Refresh locate_f32(const GeomDev& g, int& a, float& p
 // the tier was built, relative to the anchor.
 struct Tier {
     float ux, uy, uz, h;           // plane of edge a->k, unit normal towards k
-    float ux2, uy2, uz2, h2;       // a's plane of second-smallest margin (never crossed: 0, 0, 0, >= 0)
+    float ux2, uy2, uz2, h2;       // a's planes of second- and third-smallest margin
+    float ux3, uy3, uz3, h3;       // (absent: 0, 0, 0, h >= cap, never crossed)
     float cx, cy, cz;              // centre
     float ra_in2, ra_out2;         // squared radii on a's side: inside / outside answers
     float rk_in2, rk_out2;         // the same on k's side (0 = no k-side answers)
@@ -178,6 +179,8 @@ __device__ __forceinline__ void tier_clear(Tier& t)
     t.h = 0.f;
     t.ux2 = t.uy2 = t.uz2 = 0.f;
     t.h2 = 0.f;
+    t.ux3 = t.uy3 = t.uz3 = 0.f;
+    t.h3 = 0.f;
     t.cx = t.cy = t.cz = 0.f;
     t.ra_in2 = t.ra_out2 = t.rk_in2 = t.rk_out2 = t.rout2 = 0.f;
     t.alpha_a = t.alpha_k = 0.f;
@@ -210,19 +213,20 @@ __device__ __forceinline__ float exit_bound(const GeomDev& g, const WalkDev& p)
 
 // Tier answer for q (relative to the anchor a); returns 1 when certified.
 //
-// Let x be the centre, m1 <= m2 <= m3 the smallest margins of a's planes at x
-// (planes 1 = a->k and 2 kept in the tier; the radii use a lower bound of m3)
-// and n2 the second-smallest margin of k's planes at x (all capped).
-// Margins change by at most |q - x|, so for |q - x| < m3 every plane of a
-// other than planes 1 and 2 keeps a positive margin: q lies in a's Voronoi
-// cell iff s1 = h - q.u >= 0 and s2 = h2 - q.u2 >= 0, and is then outside the
-// contracted cell iff min(s1, s2) < alpha_a; inside needs every other plane
-// at alpha_a or more as well, i.e. |q - x| < m3 - alpha_a.  Across plane 1 the
-// same holds on k's side with n2 and alpha_k (q is in Vor(k) once it is on
-// k's side of the a-k bisector and within n2 of x, whatever a's other planes
-// say); across plane 2 alone there is no answer.  An outside answer has the
-// larger radius m3 (n2): walkers in the extracellular gap, whose next plane
-// is often closer than alpha + a step, keep their tier much longer.
+// Let x be the centre, m1 <= m2 <= m3 <= m4 the smallest margins of a's
+// planes at x (planes 1 = a->k, 2 and 3 kept in the tier; the radii use a
+// lower bound r of m4, of every margin but the three smallest) and n2 the
+// second-smallest margin of k's planes at x (all capped).  Margins change by
+// at most |q - x|, so for |q - x| < r every plane of a other than planes 1-3
+// keeps a positive margin: q lies in a's Voronoi cell iff s_i = h_i - q.u_i
+// >= 0 for i = 1, 2, 3, and is then outside the contracted cell iff
+// min(s1, s2, s3) < alpha_a; inside needs every other plane at alpha_a or
+// more as well, i.e. |q - x| < r - alpha_a.  Across plane 1 the same holds
+// on k's side with n2 and alpha_k (q is in Vor(k) once it is on k's side of
+// the a-k bisector and within n2 of x, whatever a's other planes say); across
+// plane 2 or 3 alone there is no answer.  An outside answer has the larger
+// radius r (n2): walkers in the extracellular gap, whose next plane is often
+// closer than alpha + a step, keep their tier much longer.
 //
 // The outside ball (rout2): every contracted cell j is the Voronoi cell of j
 // eroded by alpha_j, so it lies at least alpha_j + dist(x, Vor(j)) from the
@@ -249,9 +253,11 @@ __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float 
     const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
     if (s >= 0.f) {
         const float s2 = t.h2 - (qx * t.ux2 + qy * t.uy2 + qz * t.uz2);
-        const float sm = fminf(s, s2);
+        const float s3 = t.h3 - (qx * t.ux3 + qy * t.uy3 + qz * t.uz3);
+        const float s23 = fminf(s2, s3);
+        const float sm = fminf(s, s23);
         const int ins = sm >= t.alpha_a;
-        if (s2 < 0.f || fabsf(sm - t.alpha_a) <= t.eps || !(d2 < (ins ? t.ra_in2 : t.ra_out2))) return 0;
+        if (s23 < 0.f || fabsf(sm - t.alpha_a) <= t.eps || !(d2 < (ins ? t.ra_in2 : t.ra_out2))) return 0;
         cell = a;
         inside = ins;
         return 1;
@@ -266,11 +272,11 @@ __device__ __forceinline__ int tier_query(const Tier& t, int a, float qx, float 
 
 // Exit certificate for a proposal q the tier does not label, from a walker
 // inside its anchor cell a whose draw rejects both crossing kinds (rj == 3):
-// q lies outside a's contracted cell when plane 1 (on a's side: plane 1 or
-// 2) has a margin below alpha_a by more than the rounding bound at q (the
-// centre's plus kUlp * |q - x|_1).  Both planes are edges of a when k != a:
-// without a second edge below the cap, plane 2 is (0, h2 = cap) and never
-// passes.  |q|_1 < leave (exit_bound) keeps q nearer to a's seed than to any
+// q lies outside a's contracted cell when plane 1 (on a's side: plane 1, 2
+// or 3) has a margin below alpha_a by more than the rounding bound at q (the
+// centre's plus kUlp * |q - x|_1).  The planes are edges of a when k != a:
+// a missing second or third edge leaves its plane at (0, h >= cap), which
+// never passes.  |q|_1 < leave (exit_bound) keeps q nearer to a's seed than to any
 // periodic image of it, so q lies in another cell or outside a's contracted
 // one: the walker crosses one membrane or two, and rj == 3 rejects either.
 // The caller then completes the step as a rejected crossing, the outcome
@@ -279,7 +285,8 @@ __device__ __forceinline__ bool tier_exit(const Tier& t, int a, float qx, float 
 {
     const float s = t.h - (qx * t.ux + qy * t.uy + qz * t.uz);
     const float s2 = t.h2 - (qx * t.ux2 + qy * t.uy2 + qz * t.uz2);
-    const float sm = s >= 0.f ? fminf(s, s2) : s;
+    const float s3 = t.h3 - (qx * t.ux3 + qy * t.uy3 + qz * t.uz3);
+    const float sm = s >= 0.f ? fminf(s, fminf(s2, s3)) : s;
     const float lim =
         t.alpha_a - fmaf(kUlp * 1.125f, fabsf(qx - t.cx) + fabsf(qy - t.cy) + fabsf(qz - t.cz), t.eps);
     return t.k != a && sm < lim && fabsf(qx) + fabsf(qy) + fabsf(qz) < leave;
@@ -383,8 +390,8 @@ __device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 // `cell`) over the cell's edges in order of half length.  cur < 0: the scan
 // of (cell, s) is set up by scan_begin on its first group.
 struct ScanState {
-    int cur, end, c, c2, cell;
-    float m1, m2, m3, xb, pe, eps;  // xb: lower bound of every unvisited edge's margin
+    int cur, end, c, c2, c3, cell;       // c, c2, c3: edges of the three smallest margins (-1: none)
+    float m1, m2, m3, m4, xb, pe, eps;  // xb: lower bound of every unvisited edge's margin
     float alpha;       // the cell's annulus
     float nx, ny, nz;  // -s
 };
@@ -402,8 +409,8 @@ __device__ __forceinline__ void scan_setup(ScanState& S, int off, int end, float
     S.cur = off;
     S.end = end;
     S.alpha = alpha;
-    S.c = S.c2 = -1;
-    S.m1 = S.m2 = S.m3 = S.xb = __builtin_inff();
+    S.c = S.c2 = S.c3 = -1;
+    S.m1 = S.m2 = S.m3 = S.m4 = S.xb = __builtin_inff();
     S.eps = kUlp * (fabsf(S.nx) + fabsf(S.ny) + fabsf(S.nz) + hmax);
     const float n2 = fmaf(S.nx, S.nx, fmaf(S.ny, S.ny, S.nz * S.nz));
     S.pe = fmaf(__builtin_amdgcn_sqrtf(n2), 1.f + 0x1.0p-20f, S.eps);
@@ -427,74 +434,87 @@ __device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx,
 
 // One group of kEdgePad edges (requires S.cur < S.end; lists are padded with
 // sentinels whose margin is +inf).  All records are fetched before any is
-// used, so a group costs one memory round trip.  Margins of edges past the
-// early-exit point are at least min(m2, cap), so evaluating them cannot
-// change m1, c or the capped m2: the group needs no per-edge branch.  The
-// scan ends when the group's last half length minus pe reaches min(m2, cap):
-// edges are sorted by h, so every later computed margin is at least that,
-// and the scan's m1, c and capped m2 equal those of the whole list.
-__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets)
+// used, so a group costs one memory round trip.  The scan ends when the
+// group's last half length minus pe reaches min(m3, cap) (an anchor scan,
+// third) or min(m2, cap) (k's side): edges are sorted by h, so every later
+// computed margin is at least that bound (xb), the scan's m1, c, m2, c2
+// (and for an anchor scan m3, c3) are those of the whole list up to the
+// cap, and min(m4, xb) bounds every margin but the three smallest from
+// below.  Evaluating the rest of the last group cannot change those values:
+// the group needs no per-edge branch.
+__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets, bool third)
 {
     const float4* __restrict__ rec = g.nb_rec + S.cur;
     float4 r[kEdgePad];
 #pragma unroll
     for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
     // chunk-local codes: cl -1 = the previous c; cl2 -2 = the previous c2,
-    // -1 = the previous c
-    int cl = -1, cl2 = -2;
+    // -1 = the previous c; cl3 -3 = the previous c3, -2 / -1 as for cl2
+    int cl = -1, cl2 = -2, cl3 = -3;
 #pragma unroll
     for (int u = 0; u < kEdgePad; ++u) {
         const float m = fmaf(S.nx, r[u].x, fmaf(S.ny, r[u].y, fmaf(S.nz, r[u].z, r[u].w)));
-        const bool lt1 = m < S.m1, lt2 = m < S.m2;
+        const bool lt1 = m < S.m1, lt2 = m < S.m2, lt3 = m < S.m3;
+        cl3 = lt2 ? cl2 : (lt3 ? u : cl3);
         cl2 = lt1 ? cl : (lt2 ? u : cl2);
         cl = lt1 ? u : cl;
-        S.m3 = fminf(S.m3, fmaxf(S.m2, m));
-        S.m2 = fminf(S.m2, fmaxf(S.m1, m));
+        // m1 <= m2 <= m3 <= m4 throughout, so each update is a median
+        S.m4 = __builtin_amdgcn_fmed3f(S.m3, m, S.m4);
+        S.m3 = __builtin_amdgcn_fmed3f(S.m2, m, S.m3);
+        S.m2 = __builtin_amdgcn_fmed3f(S.m1, m, S.m2);
         S.m1 = fminf(S.m1, m);
     }
-    S.c2 = cl2 == -2 ? S.c2 : (cl2 == -1 ? S.c : S.cur + cl2);
+    const int c0 = S.c, c20 = S.c2;
+    S.c3 = cl3 == -3 ? S.c3 : (cl3 == -2 ? c20 : (cl3 == -1 ? c0 : S.cur + cl3));
+    S.c2 = cl2 == -2 ? c20 : (cl2 == -1 ? c0 : S.cur + cl2);
     if (cl >= 0) S.c = S.cur + cl;
     facets += kEdgePad;
     S.cur += kEdgePad;
+    // an anchor scan (third) runs until the third margin is exact, k's side
+    // until the second
     const float bound = r[kEdgePad - 1].w - S.pe;
-    if (bound >= fminf(S.m2, cap)) {
+    if (bound >= fminf(third ? S.m3 : S.m2, cap)) {
         S.xb = bound;
         S.cur = S.end;
     }
 }
 
 struct Scan {
-    float m1, m2, m3, eps;  // m3: a lower bound of the third-smallest margin
-    int c, c2;              // edges of m1 and m2 (-1: none)
+    float m1, m2, m3, r, eps;  // r: a lower bound of every margin but the three smallest
+    int c, c2, c3;             // edges of m1, m2 and m3 (-1: none)
 };
 
 // Whole scan of one cell.
 __device__ __forceinline__ Scan scan_cell(const GeomDev& g, int cell, float px, float py, float pz, float cap,
-                                          uint32_t& facets)
+                                          uint32_t& facets, bool third)
 {
     ScanState S;
     scan_start(g, cell, px, py, pz, S);
-    while (S.cur < S.end) scan_chunk(g, cap, S, facets);
-    return Scan{S.m1, S.m2, fminf(S.m3, S.xb), S.eps, S.c, S.c2};
+    while (S.cur < S.end) scan_chunk(g, cap, S, facets, third);
+    return Scan{S.m1, S.m2, S.m3, fminf(S.m4, S.xb), S.eps, S.c, S.c2, S.c3};
 }
 
-// Tier from the scans at q: binding edge c of anchor a with smallest margin
-// m1, edge c2 of the second-smallest margin m2a, a lower bound m3a of the
-// third-smallest; k-side bound n2 when the k side was scanned (an outside
-// point).  c < 0 means no edge below the cap (every margin is at least cap:
-// pass m2a = m3a = cap; the tier then answers "inside a" up to cap -
-// alpha_a); m2a = m3a = -inf gives a tier that answers nothing.  One code
-// path for every case, so the walk's update is branch-free.
-__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2, float qx, float qy, float qz,
-                                          float m1, float m2a, float m3a, float epsa, bool kside, float n2, float epsk,
+// Tier from the scans at q: edges c, c2, c3 of anchor a with the three
+// smallest margins (m1 the smallest, m2a the second), a lower bound ra of
+// every other margin of a; k-side bound n2 when the k side was scanned (an
+// outside point).  c < 0 means no edge below the cap (every margin is at
+// least cap: pass m2a = ra = cap; the tier then answers "inside a" up to
+// cap - alpha_a); m2a = ra = -inf gives a tier that answers nothing.  A
+// missing second or third edge leaves that plane at (0, max(ra, 0)) with
+// ra >= cap then (the scan ran past the cap).  One code path for every case,
+// so the walk's update is branch-free.
+__device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2, int c3, float qx, float qy, float qz,
+                                          float m1, float m2a, float ra, float epsa, bool kside, float n2, float epsk,
                                           Tier& t)
 {
-    const bool plane = c >= 0, plane2 = plane && c2 >= 0;
-    const int ci = plane ? c : 0, ci2 = plane2 ? c2 : 0;
-    const float4 e0 = gload(g.nb_rec + ci), e20 = gload(g.nb_rec + ci2);
+    const bool plane = c >= 0, plane2 = plane && c2 >= 0, plane3 = plane2 && c3 >= 0;
+    const int ci = plane ? c : 0, ci2 = plane2 ? c2 : 0, ci3 = plane3 ? c3 : 0;
+    const float4 e0 = gload(g.nb_rec + ci), e20 = gload(g.nb_rec + ci2), e30 = gload(g.nb_rec + ci3);
     const int k = plane ? gload(g.nb_id + ci) : a;
     const float4 e = plane ? e0 : make_float4(0.f, 0.f, 0.f, m2a);
-    const float4 e2 = plane2 ? e20 : make_float4(0.f, 0.f, 0.f, fmaxf(m3a, 0.f));
+    const float hr = fmaxf(ra, 0.f);
+    const float4 e2 = plane2 ? e20 : make_float4(0.f, 0.f, 0.f, hr);
+    const float4 e3 = plane3 ? e30 : make_float4(0.f, 0.f, 0.f, hr);
     const int4 ar = gload(g.cell_rec + a);
     const float aa = __int_as_float(ar.w), ak = gload(g.cell_alpha + k);
     t.a_off = ar.x;
@@ -502,7 +522,7 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2
     t.a_hmax = __int_as_float(ar.z);
     kside = kside && plane;
     const float ep = kside ? fmaxf(epsa, epsk) : epsa;
-    const float ra_in = m3a - fmaxf(aa, 0.f) - 2.f * ep, ra_out = m3a - 2.f * ep;
+    const float ra_in = ra - fmaxf(aa, 0.f) - 2.f * ep, ra_out = ra - 2.f * ep;
     const float rk_in = kside ? n2 - fmaxf(ak, 0.f) - 2.f * ep : 0.f;
     const float rk_out = kside ? n2 - 2.f * ep : 0.f;
     t.ux = e.x;
@@ -513,6 +533,10 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2
     t.uy2 = e2.y;
     t.uz2 = e2.z;
     t.h2 = e2.w;
+    t.ux3 = e3.x;
+    t.uy3 = e3.y;
+    t.uz3 = e3.z;
+    t.h3 = e3.w;
     t.cx = qx;
     t.cy = qy;
     t.cz = qz;
@@ -528,7 +552,8 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2
     const float rout =
         kside ? fminf(fminf(aa - m1, ak + m1), g.alpha_min + fmaxf(fminf(m2a, n2), 0.f)) - 2.f * ep : 0.f;
     t.rout2 = rout > 0.f ? rout * rout : 0.f;
-    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) + fmaxf(fabsf(e.w), fabsf(e2.w)));
+    t.eps = ep + kUlp * (1.8f * fmaxf(fmaxf(ra_out, rk_out), 0.f) +
+                         fmaxf(fmaxf(fabsf(e.w), fabsf(e2.w)), fabsf(e3.w)));
 }
 
 // Full classification of q from anchor a: greedy location, label, FP64
@@ -541,7 +566,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
                              uint32_t& rechecks)
 {
     for (int it = 0; it <= kMaxGraphIters; ++it) {
-        const Scan s = scan_cell(g, a, qx, qy, qz, cap, facets);
+        const Scan s = scan_cell(g, a, qx, qy, qz, cap, facets, true);
         if (s.m1 < -s.eps) {
             // a neighbour is nearer: move the anchor and rescan
             const double ex = g.nb_e64[3 * s.c], ey = g.nb_e64[3 * s.c + 1], ez = g.nb_e64[3 * s.c + 2];
@@ -564,7 +589,7 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             return 1;
         }
         if (s.c < 0 || s.m1 >= cap) {
-            tier_make(g, a, -1, -1, qx, qy, qz, 0.f, cap, cap, s.eps, false, 0.f, 0.f, t);
+            tier_make(g, a, -1, -1, -1, qx, qy, qz, 0.f, cap, cap, s.eps, false, 0.f, 0.f, t);
             return 1;
         }
         const bool kside = !inside;
@@ -576,11 +601,12 @@ __device__ int classify_full(const GeomDev& g, int& a, float& qx, float& qy, flo
             const float kx = (float)((double)qx - g.nb_e64[3 * s.c]);
             const float ky = (float)((double)qy - g.nb_e64[3 * s.c + 1]);
             const float kz = (float)((double)qz - g.nb_e64[3 * s.c + 2]);
-            const Scan sk = scan_cell(g, g.nb_id[s.c], kx, ky, kz, cap, facets);
+            const Scan sk = scan_cell(g, g.nb_id[s.c], kx, ky, kz, cap, facets, false);
             n2 = fminf(sk.m2, cap);
             epsk = sk.eps;
         }
-        tier_make(g, a, s.c, s.c2, qx, qy, qz, s.m1, fminf(s.m2, cap), fminf(s.m3, cap), s.eps, kside, n2, epsk, t);
+        tier_make(g, a, s.c, s.c2, s.c3, qx, qy, qz, s.m1, fminf(s.m2, cap), fminf(s.r, cap), s.eps, kside, n2, epsk,
+                  t);
         return 1;
     }
     return 0;
@@ -606,8 +632,8 @@ struct TileLds {
 // anchor's offset, and the a-side results kept for the tier.
 struct Pending {
     float qx, qy, qz, ox, oy, oz;
-    int na, pin, moves, sa_c, sa_c2;
-    float sa_m1, sa_m2, sa_m3, sa_eps;
+    int na, pin, moves, sa_c, sa_c2, sa_c3;
+    float sa_m1, sa_m2, sa_r, sa_eps;  // sa_r: the tier radius bound (Scan::r, capped)
 };
 
 constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classification
@@ -615,8 +641,8 @@ constexpr int kScanA = 1, kScanK = 2;  // scan phases of a pending classificatio
 __device__ __forceinline__ void pending_start(Pending& P, ScanState& S, int& phase)
 {
     P.moves = 0;
-    P.sa_c = P.sa_c2 = -1;
-    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+    P.sa_c = P.sa_c2 = P.sa_c3 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_r = P.sa_eps = 0.f;
     S.cell = P.na;
     S.nx = -P.qx;
     S.ny = -P.qy;
@@ -635,7 +661,7 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
                                               uint32_t& rechecks)
 {
     if (S.cur < 0) scan_begin(g, S);
-    scan_chunk(g, cap, S, nfac);
+    scan_chunk(g, cap, S, nfac, phase == kScanA);
     if (S.cur < S.end) return 0;
     int next;  // 1: greedy move to the nearer neighbour, 2: scan k's side
     if (phase == kScanK) {
@@ -652,9 +678,10 @@ __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& 
         P.pin = S.m1 >= a1;
         P.sa_c = S.c;
         P.sa_c2 = S.c2;
+        P.sa_c3 = S.c3;
         P.sa_m1 = S.m1;
         P.sa_m2 = fminf(S.m2, cap);
-        P.sa_m3 = fminf(fminf(S.m3, S.xb), cap);
+        P.sa_r = fminf(fminf(S.m4, S.xb), cap);
         P.sa_eps = S.eps;
         if (fabsf(S.m1 - a1) <= S.eps) {
             ++rechecks;
@@ -710,12 +737,12 @@ __device__ __forceinline__ void tier_of(const GeomDev& g, const Pending& P, int 
 {
     const float none = -__builtin_inff();
     const float m2a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m2);
-    const float m3a = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_m3);
+    const float ra = tmode == 2 ? none : (tmode == 3 ? cap : P.sa_r);
     // tmode 2 (FP64 re-decision) may have moved the anchor off the scanned
     // cell, whose edges are then not a's: no planes (the tier answers nothing)
     const bool planes = tmode != 2 && tmode != 3;
-    tier_make(g, P.na, planes ? P.sa_c : -1, planes ? P.sa_c2 : -1, P.qx, P.qy, P.qz, P.sa_m1, m2a, m3a,
-              P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps, t);
+    tier_make(g, P.na, planes ? P.sa_c : -1, planes ? P.sa_c2 : -1, planes ? P.sa_c3 : -1, P.qx, P.qy, P.qz, P.sa_m1,
+              m2a, ra, P.sa_eps, tmode == 1, fminf(S.m2, cap), S.eps, t);
 }
 
 // One tile of kBlock walkers [w_lo, w_lo + kBlock) of an entry: the whole
@@ -811,14 +838,14 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     Pending P;
     P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
     P.na = P.pin = P.moves = 0;
-    P.sa_c = P.sa_c2 = -1;
-    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+    P.sa_c = P.sa_c2 = P.sa_c3 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_r = P.sa_eps = 0.f;
     int pcell = 0, rj = 0;
     ScanState S;
     S.cur = S.end = 0;
-    S.c = S.c2 = -1;
+    S.c = S.c2 = S.c3 = -1;
     S.cell = 0;
-    S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+    S.m1 = S.m2 = S.m3 = S.m4 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
     uint32_t wave_iters = 0, lane_iters = 0;
     while (__any(phase != kDone)) {
         ++wave_iters;
@@ -1105,14 +1132,14 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
             Pending P;
             P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
             P.na = P.pin = P.moves = 0;
-            P.sa_c = P.sa_c2 = -1;
-            P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+            P.sa_c = P.sa_c2 = P.sa_c3 = -1;
+            P.sa_m1 = P.sa_m2 = P.sa_r = P.sa_eps = 0.f;
             int pcell = 0, rj = 0;
             ScanState S;
             S.cur = S.end = 0;
-            S.c = S.c2 = -1;
+            S.c = S.c2 = S.c3 = -1;
             S.cell = 0;
-            S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+            S.m1 = S.m2 = S.m3 = S.m4 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
             for (;;) {
                 // hand walkers to idle lanes
                 uint64_t need = __ballot(phase == kIdle);
@@ -1427,13 +1454,13 @@ struct ReqSlot {
     float qx, qy, qz, ox, oy, oz;
 };
 
-// 128 bytes, the tier first (16-byte aligned in the array)
+// 144 bytes, the tier first (16-byte aligned in the array)
 struct RespSlot {
     Tier t;
     int status, na, flags;  // flags: bit 0 inside, bits 1-3 tmode, bit 4 classifier failure
     float qx, qy, qz, ox, oy, oz;
 };
-static_assert(sizeof(ReqSlot) == 48 && sizeof(RespSlot) == 128, "split ring slots");
+static_assert(sizeof(ReqSlot) == 48 && sizeof(RespSlot) == 144, "split ring slots");
 
 template <int NP, int WW, int CW>
 struct SplitLds {
@@ -1863,13 +1890,13 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
     Pending P;
     P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
     P.na = P.pin = P.moves = 0;
-    P.sa_c = P.sa_c2 = -1;
-    P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+    P.sa_c = P.sa_c2 = P.sa_c3 = -1;
+    P.sa_m1 = P.sa_m2 = P.sa_r = P.sa_eps = 0.f;
     ScanState S;
     S.cur = S.end = 0;
-    S.c = S.c2 = -1;
+    S.c = S.c2 = S.c3 = -1;
     S.cell = 0;
-    S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
+    S.m1 = S.m2 = S.m3 = S.m4 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = 0.f;
     for (;;) {
         const uint64_t need = __ballot(!busy);
         if (need != 0) {
@@ -1908,12 +1935,12 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
             // no lane holds a classification: its state is dead here
             P.qx = P.qy = P.qz = P.ox = P.oy = P.oz = 0.f;
             P.na = P.pin = P.moves = 0;
-            P.sa_c = P.sa_c2 = -1;
-            P.sa_m1 = P.sa_m2 = P.sa_m3 = P.sa_eps = 0.f;
+            P.sa_c = P.sa_c2 = P.sa_c3 = -1;
+            P.sa_m1 = P.sa_m2 = P.sa_r = P.sa_eps = 0.f;
             S.cur = S.end = 0;
-            S.c = S.c2 = -1;
+            S.c = S.c2 = S.c3 = -1;
             S.cell = 0;
-            S.m1 = S.m2 = S.m3 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = S.alpha = 0.f;
+            S.m1 = S.m2 = S.m3 = S.m4 = S.xb = S.pe = S.eps = S.nx = S.ny = S.nz = S.alpha = 0.f;
             owner = cphase = 0;
             continue;
         }

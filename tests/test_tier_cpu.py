@@ -3,12 +3,14 @@
 A tier built by a full classification at x answers later proposals q near x
 without classifying them (csrc/prod.hip: tier_make / tier_query).  This test
 restates both in FP64 numpy (exact arithmetic, so without the FP32 rounding
-slack, and with the exact third margin where the kernel uses a lower bound of
-it) and checks every answer the tier gives against the all-seed SI Eq. S2
+slack, and with the exact fourth margin where the kernel uses a lower bound
+of it) and checks every answer the tier gives against the all-seed SI Eq. S2
 labels of the 27-image tiled periodic box (oracle.periodic_brute).  It pins
-the radii: an inside answer trusted up to m3 instead of m3 - alpha, an outside
-ball not bounded by alpha_min (plus the distance to any third Voronoi cell),
-or a k-side answer without the k scan would each give wrong labels here.
+the planes and radii: an inside answer trusted up to m4 instead of
+m4 - alpha, a third plane left out of the answers, a radius from the fifth
+margin, an outside ball not bounded by alpha_min (plus the distance to any
+third Voronoi cell), or a k-side answer without the k scan would each give
+wrong labels here.
 """
 
 from __future__ import annotations
@@ -49,26 +51,28 @@ def _locate(arr, a, p):
 
 
 def tier_make(arr, a, x, cap):
-    """The tier built at x (relative to anchor a, a the nearest seed): the two
-    planes of smallest margin (m1, m2) and the third margin m3."""
+    """The tier built at x (relative to anchor a, a the nearest seed): the
+    three planes of smallest margin (m1, m2, m3) and the fourth margin m4."""
     m, u, h, lo = _margins(arr, a, x)
     order = np.argsort(m, kind="stable")
     m1, c = m[order[0]], int(order[0])
     m2 = min(m[order[1]], cap) if len(m) > 1 else cap
     c2 = int(order[1]) if len(m) > 1 else None
-    m3 = min(m[order[2]], cap) if len(m) > 2 else cap
+    c3 = int(order[2]) if len(m) > 2 else None
+    m4 = min(m[order[3]], cap) if len(m) > 3 else cap
     aa = arr.cell_annulus[a]
     amin = float(arr.cell_annulus.min())
     inside = m1 >= aa
     t = dict(c=x.copy(), a=a, alpha_a=aa)
     if m1 >= cap:
-        t.update(u=np.zeros(3), h=cap, u2=np.zeros(3), h2=cap, k=a, alpha_k=0.0, ra_in=cap - max(aa, 0),
-                 ra_out=cap - 0.0, rk_in=0.0, rk_out=0.0, rout=0.0)
+        t.update(u=np.zeros(3), h=cap, u2=np.zeros(3), h2=cap, u3=np.zeros(3), h3=cap, k=a, alpha_k=0.0,
+                 ra_in=cap - max(aa, 0), ra_out=cap - 0.0, rk_in=0.0, rk_out=0.0, rout=0.0)
         return t, inside
     k = int(arr.nb_ids[lo + c])
     ak = arr.cell_annulus[k]
     u2, h2 = (u[c2], h[c2]) if c2 is not None else (np.zeros(3), cap)
-    t.update(u=u[c], h=h[c], u2=u2, h2=h2, k=k, alpha_k=ak, ra_in=m3 - max(aa, 0.0), ra_out=m3)
+    u3, h3 = (u[c3], h[c3]) if c3 is not None else (np.zeros(3), cap)
+    t.update(u=u[c], h=h[c], u2=u2, h2=h2, u3=u3, h3=h3, k=k, alpha_k=ak, ra_in=m4 - max(aa, 0.0), ra_out=m4)
     if inside:
         t.update(rk_in=0.0, rk_out=0.0, rout=0.0)
     else:
@@ -85,13 +89,13 @@ def tier_query(t, q):
     if d < max(t["rout"], 0.0):
         return True, None, False
     s = t["h"] - q @ t["u"]
-    s2 = t["h2"] - q @ t["u2"]
-    if s >= 0 and s2 >= 0:
-        ins = min(s, s2) >= t["alpha_a"]
+    s23 = min(t["h2"] - q @ t["u2"], t["h3"] - q @ t["u3"])
+    if s >= 0 and s23 >= 0:
+        ins = min(s, s23) >= t["alpha_a"]
         ok = d < max(t["ra_in"] if ins else t["ra_out"], 0.0)
         return ok, t["a"], ins
     if s >= 0:
-        return False, None, None  # across the second plane: no certificate
+        return False, None, None  # across the second or third plane: no certificate
     r = -s
     ins = r >= t["alpha_k"]
     ok = d < max(t["rk_in"] if ins else t["rk_out"], 0.0)
@@ -190,13 +194,13 @@ def test_outside_ball_third_cell_term_is_tight():
 
 
 def tier_leaves(t, q, L):
-    """csrc/prod.hip:tier_query's exit certificate (result 2), without rounding slack: with k != a, the smaller
-    margin of planes 1 and 2 puts q outside a's contracted cell (plane 2 without a second edge is (0, cap) and
-    never does), and |q|_1 < 0.49 L keeps a's own seed nearer to q than any periodic image of it."""
+    """csrc/prod.hip:tier_exit without rounding slack: with k != a, the smallest margin of planes 1-3 (on k's side
+    plane 1 alone) puts q outside a's contracted cell (a missing plane is (0, cap) and never does), and
+    |q|_1 < 0.49 L keeps a's own seed nearer to q than any periodic image of it."""
     if t["k"] == t["a"]:
         return False
     s = t["h"] - q @ t["u"]
-    sm = min(s, t["h2"] - q @ t["u2"]) if s >= 0 else s  # on k's side plane 1 alone
+    sm = min(s, t["h2"] - q @ t["u2"], t["h3"] - q @ t["u3"]) if s >= 0 else s  # on k's side plane 1 alone
     return bool(sm < t["alpha_a"] and np.abs(q).sum() < 0.49 * L)
 
 
