@@ -442,12 +442,10 @@ __device__ __forceinline__ void scan_start(const GeomDev& g, int cell, float sx,
 // cap, and min(m4, xb) bounds every margin but the three smallest from
 // below.  Evaluating the rest of the last group cannot change those values:
 // the group needs no per-edge branch.
-__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets, bool third)
+// Evaluates the group r (the records at S.cur) into S; see scan_chunk.
+__device__ __forceinline__ void scan_group(const float4 (&r)[kEdgePad], float cap, ScanState& S, uint32_t& facets,
+                                           bool third)
 {
-    const float4* __restrict__ rec = g.nb_rec + S.cur;
-    float4 r[kEdgePad];
-#pragma unroll
-    for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
     // chunk-local codes: cl -1 = the previous c; cl2 -2 = the previous c2,
     // -1 = the previous c; cl3 -3 = the previous c3, -2 / -1 as for cl2
     int cl = -1, cl2 = -2, cl3 = -3;
@@ -477,6 +475,34 @@ __device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanStat
         S.xb = bound;
         S.cur = S.end;
     }
+}
+
+__device__ __forceinline__ void scan_chunk(const GeomDev& g, float cap, ScanState& S, uint32_t& facets, bool third)
+{
+    const float4* __restrict__ rec = g.nb_rec + S.cur;
+    float4 r[kEdgePad];
+#pragma unroll
+    for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
+    scan_group(r, cap, S, facets, third);
+}
+
+// Two groups in one memory round trip: the second group's records are
+// fetched with the first when the list has them, and evaluated only if the
+// scan goes on after the first, so the scan state is that of two
+// scan_chunk calls.
+__device__ __forceinline__ void scan_chunk2(const GeomDev& g, float cap, ScanState& S, uint32_t& facets, bool third)
+{
+    const float4* __restrict__ rec = g.nb_rec + S.cur;
+    const bool has2 = S.cur + kEdgePad < S.end;
+    float4 r[kEdgePad], r2[kEdgePad];
+#pragma unroll
+    for (int u = 0; u < kEdgePad; ++u) r[u] = gload(rec + u);
+    if (has2) {
+#pragma unroll
+        for (int u = 0; u < kEdgePad; ++u) r2[u] = gload(rec + kEdgePad + u);
+    }
+    scan_group(r, cap, S, facets, third);
+    if (has2 && S.cur < S.end) scan_group(r2, cap, S, facets, third);
 }
 
 struct Scan {
@@ -651,17 +677,20 @@ __device__ __forceinline__ void pending_start(Pending& P, ScanState& S, int& pha
     phase = kScanA;
 }
 
-// Advances a pending classification by one group of edges and, when a scan
-// ends, by its end-of-scan step: greedy move to a nearer neighbour, the
+// Advances a pending classification by one group of edges (TWO: by up to
+// two, fetched in one round trip) and, when a scan ends, by its end-of-scan
+// step: greedy move to a nearer neighbour, the
 // label with FP64 re-decision near alpha, the start of k's side.  Returns 1
 // when the label is known (tmode: 1 two-sided tier, 2 none, 3 no plane
 // below the cap, 4 a side only), -1 on a classifier failure, 0 otherwise.
+template <bool TWO = false>
 __device__ __forceinline__ int classify_round(const GeomDev& g, float cap, int& phase, ScanState& S, Pending& P,
                                               int& tmode, uint32_t& nfac, uint32_t& iters, uint32_t& facets,
                                               uint32_t& rechecks)
 {
     if (S.cur < 0) scan_begin(g, S);
-    scan_chunk(g, cap, S, nfac, phase == kScanA);
+    if (TWO) scan_chunk2(g, cap, S, nfac, phase == kScanA);
+    else scan_chunk(g, cap, S, nfac, phase == kScanA);
     if (S.cur < S.end) return 0;
     int next;  // 1: greedy move to the nearer neighbour, 2: scan k's side
     if (phase == kScanK) {
@@ -1442,7 +1471,7 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
 // empty.
 
 constexpr int kReqSlots = 256;  // >= walker lanes per workgroup (power of two)
-constexpr int kSplitRounds = 2;  // classification rounds per classifier iteration
+constexpr int kSplitRounds = 1;  // classification rounds (of up to two edge groups) per classifier iteration
 constexpr uint32_t kSplitStallLimit = 1u << 26;
 
 // 48 bytes: a ring of kReqSlots per classifier wave
@@ -1954,7 +1983,8 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
         int res = 0, tmode = 0;
 #pragma unroll 1
         for (int r = 0; r < kSplitRounds; ++r) {
-            if (busy && res == 0) res = classify_round(g, cap, cphase, S, P, tmode, nfac, iters, facets, rechecks);
+            if (busy && res == 0)
+                res = classify_round<true>(g, cap, cphase, S, P, tmode, nfac, iters, facets, rechecks);
         }
         if (res != 0) {
             RespSlot& rs = L.resp[owner];
