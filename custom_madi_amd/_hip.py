@@ -104,6 +104,7 @@ SIGNATURES = {
     "madi_get_device_info": (C.c_int, [C.POINTER(DeviceInfo)]),
     "madi_geom_create": (C.c_int, [C.POINTER(GeomDesc), C.POINTER(_p)]),
     "madi_geom_destroy": (C.c_int, [_p]),
+    "madi_geom_destroy_async": (C.c_int, [_p, _p]),
     "madi_geom_device_bytes": (C.c_int, [_p, _i64p]),
     "madi_classify": (C.c_int, [_p, _p, C.c_int64, _p, _p, _p, _p]),
     "madi_classify_graph": (C.c_int, [_p, _p, C.c_int64, _p, _p, _p, _p, _p]),

@@ -283,6 +283,10 @@ class DeviceScheduler:
             shape = self.spec.shape
             flags = a.flags.cpu().numpy()
             cos, sin, occ = cs.cpu().numpy().reshape(shape), sn.cpu().numpy().reshape(shape), a.occ.cpu().numpy()
+        # the entry's generations ran on the launch stream and have completed:
+        # free its geometry in that stream's order (a plain destroy would wait
+        # for the generation running now)
+        a.geom.release(_hip.stream_ptr())
         req = a.prep.req
         return EntryResult(
             key=req.key, cos=cos, sin=sin, occupancy=occ, n_walkers=req.n_walkers, n_escaped=int(flags[0]),

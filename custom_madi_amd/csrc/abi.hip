@@ -151,23 +151,50 @@ struct madi_geom {
 
 namespace madi {
 
+// Geometry memory comes from the stream-ordered allocator on a per-device
+// non-blocking stream (default priority): creating a geometry then waits
+// for its own copies only, not for kernels running on the legacy default
+// stream, and madi_geom_destroy_async can release it in stream order.
+static int geom_stream(int dev, hipStream_t* out)
+{
+    static std::mutex mu;
+    static std::vector<hipStream_t> streams;
+    std::lock_guard<std::mutex> lock(mu);
+    if ((int)streams.size() <= dev) streams.resize(dev + 1, nullptr);
+    if (!streams[dev]) HIP_TRY(hipStreamCreateWithFlags(&streams[dev], hipStreamNonBlocking));
+    *out = streams[dev];
+    return MADI_OK;
+}
+
 template <typename T>
 static int upload(madi_geom* g, const T* host, size_t count, const T** out)
 {
     *out = nullptr;
     if (count == 0) return MADI_OK;
+    hipStream_t us = nullptr;
+    int rc = geom_stream(g->device, &us);
+    if (rc) return rc;
     void* d = nullptr;
-    HIP_TRY(hipMalloc(&d, count * sizeof(T)));
+    HIP_TRY(hipMallocAsync(&d, count * sizeof(T), us));
     g->allocs.push_back(d);
     g->bytes += (int64_t)(count * sizeof(T));
-    HIP_TRY(hipMemcpy(d, host, count * sizeof(T), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpyAsync(d, host, count * sizeof(T), hipMemcpyHostToDevice, us));
+    HIP_TRY(hipStreamSynchronize(us));  // the host array may go away after this call
     *out = (const T*)d;
     return MADI_OK;
 }
 
+// Release after all work on the device (a plain destroy: whatever stream
+// used the geometry has finished with it).
 static void free_geom(madi_geom* g)
 {
-    for (void* p : g->allocs) (void)hipFree(p);
+    if (g->allocs.empty()) return;
+    hipStream_t us = nullptr;
+    (void)hipDeviceSynchronize();
+    if (geom_stream(g->device, &us) == MADI_OK) {
+        for (void* p : g->allocs) (void)hipFreeAsync(p, us);
+        (void)hipStreamSynchronize(us);
+    }
     g->allocs.clear();
 }
 
@@ -748,6 +775,23 @@ int madi_geom_destroy(madi_geom* g)
     if (!g) return MADI_OK;
     free_geom(g);
     delete g;
+    return MADI_OK;
+}
+
+int madi_geom_destroy_async(madi_geom* g, void* stream)
+{
+    if (!g) return MADI_OK;
+    hipError_t first = hipSuccess;
+    for (void* p : g->allocs) {
+        const hipError_t e = hipFreeAsync(p, (hipStream_t)stream);
+        if (first == hipSuccess) first = e;
+    }
+    g->allocs.clear();
+    delete g;
+    if (first != hipSuccess) {
+        set_error("madi_geom_destroy_async: %s", hipGetErrorString(first));
+        return MADI_ERR_HIP;
+    }
     return MADI_OK;
 }
 

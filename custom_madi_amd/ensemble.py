@@ -762,6 +762,14 @@ class DeviceGeometry:
                 pass
             self.handle = None
 
+    def release(self, stream: int) -> None:
+        """Free the device copy in the order of `stream` without waiting (every
+        use of the geometry was queued on that stream or has finished)."""
+        h = getattr(self, "handle", None)
+        if h is not None and h.value:
+            self.handle = None
+            _hip.check(self._lib.madi_geom_destroy_async(h, stream), "madi_geom_destroy_async")
+
     @property
     def device_bytes(self) -> int:
         out = C.c_int64()

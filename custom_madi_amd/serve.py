@@ -11,6 +11,7 @@ Latency of an entry = finish time - submit time.
 
 from __future__ import annotations
 
+import heapq
 import json
 import multiprocessing as mp
 import time
@@ -23,10 +24,16 @@ import numpy as np
 from .batch import DeviceScheduler, EntryRequest, EntryResult, PreparedEntry, prepare_entry
 
 
+def _worker_init() -> None:
+    # import the geometry code (numpy, scipy.spatial) when the worker starts,
+    # not inside the first request it prepares
+    from . import batch, ensemble  # noqa: F401
+
+
 def make_pool(workers: int):
     """Spawned worker pool for host geometry; create it before the GPU is touched."""
     ctx = mp.get_context("spawn")
-    return ctx.Pool(processes=max(1, int(workers)))
+    return ctx.Pool(processes=max(1, int(workers)), initializer=_worker_init)
 
 
 class LibraryStream:
@@ -60,34 +67,50 @@ class Server:
     """Submit requests; ``step()`` advances preparation, admission, launches and finalisation."""
 
     def __init__(self, scheduler: DeviceScheduler, pool=None, stream: Optional[LibraryStream] = None,
-                 max_prep_inflight: int = 64):
+                 max_prep_inflight: int = 32):
         self.sch = scheduler
         self.pool = pool
         self.stream = stream or LibraryStream(None)
+        self.queued: list = []            # heap of (cells, seq, request) not yet handed to the pool
         self.preparing: deque = deque()   # AsyncResult or PreparedEntry
         self.ready: deque = deque()       # prepared, waiting for HBM
         self.results: List[EntryResult] = []
-        self.max_prep = max_prep_inflight
+        self.max_prep = max(1, int(max_prep_inflight))
         self.rejected = 0
+        self.seq = 0
 
     def submit(self, req: EntryRequest) -> None:
         req.submitted = time.perf_counter()
         if self.pool is None:
             self.preparing.append(prepare_entry(req))
-        else:
+            return
+        heapq.heappush(self.queued, (req.n_cells, self.seq, req))
+        self.seq += 1
+        self._dispatch()
+
+    def _dispatch(self) -> None:
+        """Hand queued requests to the pool, fewest cells first (the cheapest
+        geometry reaches the GPU soonest), keeping at most max_prep in the
+        pool so later, cheaper arrivals can still go ahead."""
+        while self.queued and len(self.preparing) < self.max_prep:
+            _, _, req = heapq.heappop(self.queued)
             self.preparing.append(self.pool.apply_async(prepare_entry, (req,)))
 
     def _collect_prepared(self) -> None:
+        """Move every finished preparation to `ready` (in the order they finish:
+        a slow geometry does not hold back the ones submitted after it)."""
+        pending: deque = deque()
         while self.preparing:
-            head = self.preparing[0]
+            head = self.preparing.popleft()
             if isinstance(head, PreparedEntry):
-                self.ready.append(self.preparing.popleft())
-                continue
-            if not head.ready():
-                break
-            prep = self.preparing.popleft().get()
-            # keep the submit time of the request object the caller holds
-            self.ready.append(prep)
+                self.ready.append(head)
+            elif head.ready():
+                self.ready.append(head.get())
+            else:
+                pending.append(head)
+        self.preparing = pending
+        if self.pool is not None:
+            self._dispatch()
 
     def step(self, block: bool = False) -> List[EntryResult]:
         self._collect_prepared()
@@ -101,7 +124,7 @@ class Server:
         return done
 
     def idle(self) -> bool:
-        return not (self.preparing or self.ready or self.sch.active or self.sch.inflight)
+        return not (self.queued or self.preparing or self.ready or self.sch.active or self.sch.inflight)
 
     def drain(self) -> None:
         while not self.idle():
@@ -110,9 +133,12 @@ class Server:
                 if not self.sch.admit(self.ready[0]):
                     raise MemoryError("a request does not fit in free HBM even with the device idle")
                 self.ready.popleft()
-            self.step(block=bool(self.sch.inflight))
-            if self.preparing and not self.sch.inflight:
-                time.sleep(0.002)
+            # while geometries are still being prepared, keep admitting and
+            # launching as they arrive instead of blocking on a generation
+            preparing = bool(self.preparing or self.queued)
+            done = self.step(block=bool(self.sch.inflight) and not preparing)
+            if preparing and not done:
+                time.sleep(0.0005)
 
 
 def poisson_arrivals(rate_hz: float, duration_s: float, seed: int = 3) -> np.ndarray:

@@ -120,7 +120,14 @@ typedef struct {
 typedef struct madi_geom madi_geom;
 
 int madi_geom_create(const madi_geom_desc* desc, madi_geom** out);
+/* Waits for all work on the device, then releases the handle. */
 int madi_geom_destroy(madi_geom* g);
+/* Releases the handle in the order of `stream` (a hipStream_t): the memory is
+ * freed once the work queued on that stream before this call has run, and
+ * the call does not wait.  Every use of the geometry must be on that stream
+ * or already finished.  No counterpart in the reference (it re-uploads per
+ * walk); for schedulers that retire entries while other work runs. */
+int madi_geom_destroy_async(madi_geom* g, void* stream);
 /* Device bytes held by the handle (admission control). */
 int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes);
 
