@@ -1451,8 +1451,8 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
 // waves of a workgroup only step (tier hits complete at once) and send their
 // tier misses, with the walker's first classification, through an LDS
 // request ring to the workgroup's classifier wave, which keeps its 64 lanes
-// busy with requests (the same per-lane classification state machine,
-// kSplitRounds rounds per iteration) and writes each answer (label, moved
+// busy with requests (the same per-lane classification state machine, one
+// round of up to two edge groups per iteration) and writes each answer (label, moved
 // point, new tier) into the asking lane's answer slot.  A walker lane waits
 // for its answer; walker lanes are refilled with new walkers as in the
 // streamed walk, and moments are stored for k_emit_tiles, so results are
@@ -1471,7 +1471,6 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
 // empty.
 
 constexpr int kReqSlots = 256;  // >= walker lanes per workgroup (power of two)
-constexpr int kSplitRounds = 1;  // classification rounds (of up to two edge groups) per classifier iteration
 constexpr uint32_t kSplitStallLimit = 1u << 26;
 
 // 48 bytes: a ring of kReqSlots per classifier wave
@@ -1981,11 +1980,7 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
         // keep every access in bounds regardless
         if (busy && (unsigned)P.na >= (unsigned)g.n_cells) P.na = 0;
         int res = 0, tmode = 0;
-#pragma unroll 1
-        for (int r = 0; r < kSplitRounds; ++r) {
-            if (busy && res == 0)
-                res = classify_round<true>(g, cap, cphase, S, P, tmode, nfac, iters, facets, rechecks);
-        }
+        if (busy) res = classify_round<true>(g, cap, cphase, S, P, tmode, nfac, iters, facets, rechecks);
         if (res != 0) {
             RespSlot& rs = L.resp[owner];
             if (res > 0) {
