@@ -16,17 +16,20 @@
 // alpha threshold is re-decided in FP64 for the same represented point.
 //
 // Each cell's edges are sorted by h.  Since m_j >= h_j - |p|, the edge loop
-// stops as soon as h_j - |p| reaches the second-smallest margin seen so far
-// (or a cap): no unvisited edge can change the decision or the bounds below.
+// stops as soon as h_j - |p| reaches the third-smallest margin seen so far
+// (k's side: the second; or a cap): no unvisited edge can change the
+// decision or the bounds below (scan_chunk).
 //
-// Plane tier.  A full classification at x records its binding plane (the
-// edge a->k of smallest margin), the second-smallest margin m2 over a's
-// other edges and, for an outside point, n2 over k's other edges.  Near x
-// the label follows from the one plane: s = h - q.u;  s >= 0 -> cell a,
-// inside iff s >= alpha_a;  s < 0 -> cell k, inside iff -s >= alpha_k, with
-// validity radii m2 (outside answers) and m2 - alpha_a (inside answers) on
-// a's side, n2 and n2 - alpha_k on k's side (tier_query).  Membrane bounces
-// and gap walkers crossing a bisector are answered by this tier.
+// Plane tier.  A full classification at x records a's three planes of
+// smallest margin (the binding plane a->k first), a lower bound r of a's
+// other margins and, for an outside point, n2 over k's other edges.  Near x
+// the label follows from those planes: s = h - q.u on the binding plane;
+// s >= 0 (and the other two planes non-negative) -> cell a, inside iff every
+// plane's margin is at least alpha_a; s < 0 -> cell k, inside iff
+// -s >= alpha_k; with validity radii r (outside answers) and r - alpha_a
+// (inside answers) on a's side, n2 and n2 - alpha_k on k's side
+// (tier_query).  Membrane bounces and gap walkers crossing a bisector are
+// answered by this tier.
 #include "common.hpp"
 #include "phase.hpp"
 #include "rng.hpp"
