@@ -122,6 +122,8 @@ SIGNATURES = {
     "madi_batch_graph_launch": (C.c_int, [_p, C.POINTER(Work), C.c_int32, _p]),
     "madi_batch_graph_destroy": (C.c_int, [_p]),
     "madi_tiles_finalize": (C.c_int, [C.POINTER(PhasePlan), _p, C.c_int64, _p, _p, _p]),
+    "madi_tiles_finalize_workspace": (C.c_int, [C.POINTER(PhasePlan), C.c_int64, _i64p]),
+    "madi_tiles_finalize_ws": (C.c_int, [C.POINTER(PhasePlan), _p, C.c_int64, _p, _p, _p, C.c_int64, _p]),
     "madi_fill_i64": (C.c_int, [_p, C.c_int64, C.c_int64, _p]),
 }
 

@@ -6,8 +6,9 @@ walker count.  Its walkers are cut into tiles of 256 (one workgroup each).
 on the device and launches *generations*: one persistent work-stealing kernel
 (madi_batch_launch) over the pending tiles of every active entry.  New entries
 join the next generation as soon as they are admitted, so entries of uneven
-size share launches and the GPU stays full; an entry is finalised (its tile
-slots summed in tile order) when its last tile's generation completes.
+size share launches and the GPU stays full; an entry's tile slots are summed in
+tile order, and copied to pinned host memory, right behind the generation that
+walks its last tiles, on the same stream.
 Because every tile writes its own slot, an entry's result does not depend on
 which other entries shared its launches.
 
@@ -21,6 +22,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import sys
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -115,7 +117,6 @@ class _Active:
     flags: object
     moments: object
     next_tile: int = 0
-    last_gen: int = -1
     admitted: float = 0.0
     device_bytes: int = 0
 
@@ -142,13 +143,10 @@ class DeviceScheduler:
         self.reserve = int(hbm_reserve_bytes)
         self.max_inflight = int(max_inflight)
         self.active: List[_Active] = []
-        self.inflight: deque = deque()   # (gen id, event, workspace, [entries])
+        self.inflight: deque = deque()   # (gen id, event, workspace, [(entry, pinned host results)])
         self.gen = 0
         self.counters = torch.zeros(_hip.N_COUNTERS, dtype=torch.int64, device="cuda")
         self.hbm_samples: list = []
-        # finalisation runs on its own stream (default priority) so it does not
-        # wait behind generations queued after the one that completed
-        self.side = torch.cuda.Stream()
         # one captured batch launch per in-flight generation slot
         self.graphs: List[C.c_void_p] = []
         self.graph_capacity = int(graph_capacity)
@@ -168,7 +166,6 @@ class DeviceScheduler:
         self.graphs = []
 
     def __del__(self):
-        import sys
         if sys.is_finalizing():
             return  # the HIP runtime may already be gone
         try:
@@ -244,7 +241,6 @@ class DeviceScheduler:
             items.append(w)
             owners.append(a)
             a.next_tile += take
-            a.last_gen = self.gen
             budget -= take
         if not items:
             return False
@@ -265,28 +261,52 @@ class DeviceScheduler:
             rc = self.lib.madi_batch_launch(arr, len(items), C.byref(self.plan), C.byref(self.tune), _hip.dptr(ws),
                                             int(need.value), _hip.dptr(self.counters), s)
             _hip.check(rc, "madi_batch_launch")
+        # entries whose last tiles are in this generation: their reduction and
+        # result copies follow it on the same stream
+        staged = [self._stage(a, s) for a in owners if a.next_tile >= a.n_tiles]
         ev = self.torch.cuda.Event()
         ev.record()
-        self.inflight.append((self.gen, ev, ws, owners))
+        self.inflight.append((self.gen, ev, ws, staged))
         self.gen += 1
         return True
 
-    def _finalize(self, a: _Active) -> EntryResult:
-        """Sum the entry's tile slots and copy its results (its generation has completed)."""
+    def _stage(self, a: _Active, s: int) -> tuple:
+        """Queue, behind the generation that walks the entry's last tiles and on
+        its stream, the sum of the tile slots and the copies of the sums,
+        occupancy and flags into pinned host memory.  poll() only reads host
+        memory once the generation's event has completed: no result waits on a
+        second stream, which the runtime may map onto the hardware queue of the
+        launch stream (and so behind the next generation).
+
+        The sum's scratch and output come from torch's allocator on the launch
+        stream, whose blocks are reused only by later work on that stream; the
+        geometry is released by poll() after the generation has completed (the
+        stream-ordered allocator may hand a freed block to a geometry uploaded
+        on another stream before the launch stream reaches the free)."""
         torch = self.torch
-        with torch.cuda.stream(self.side):
-            cs = torch.empty(self.n_cols, dtype=torch.float64, device="cuda")
-            sn = torch.empty(self.n_cols, dtype=torch.float64, device="cuda")
-            rc = self.lib.madi_tiles_finalize(C.byref(self.plan), _hip.dptr(a.partial), a.n_tiles, _hip.dptr(cs),
-                                              _hip.dptr(sn), _hip.stream_ptr())
-            _hip.check(rc, "madi_tiles_finalize")
-            shape = self.spec.shape
-            flags = a.flags.cpu().numpy()
-            cos, sin, occ = cs.cpu().numpy().reshape(shape), sn.cpu().numpy().reshape(shape), a.occ.cpu().numpy()
-        # the entry's generations ran on the launch stream and have completed:
-        # free its geometry in that stream's order (a plain destroy would wait
-        # for the generation running now)
-        a.geom.release(_hip.stream_ptr())
+        n, g = self.n_cols, a.occ.numel()
+        need = C.c_int64()
+        _hip.check(self.lib.madi_tiles_finalize_workspace(C.byref(self.plan), a.n_tiles, C.byref(need)),
+                   "madi_tiles_finalize_workspace")
+        sums = torch.empty(2 * n + -(-int(need.value) // 8), dtype=torch.float64, device="cuda")
+        base = _hip.dptr(sums)
+        rc = self.lib.madi_tiles_finalize_ws(C.byref(self.plan), _hip.dptr(a.partial), a.n_tiles, base, base + 8 * n,
+                                             base + 16 * n, int(need.value), s)
+        _hip.check(rc, "madi_tiles_finalize_ws")
+        host = torch.empty(2 * n + g + _hip.N_FLAGS, dtype=torch.int64, pin_memory=True)
+        host[:2 * n].view(torch.float64).copy_(sums[:2 * n], non_blocking=True)
+        host[2 * n:2 * n + g].copy_(a.occ, non_blocking=True)
+        host[2 * n + g:].copy_(a.flags, non_blocking=True)
+        return a, host
+
+    def _result(self, a: _Active, host) -> EntryResult:
+        """The entry's results from its staged host copy (its generation has completed)."""
+        n, g = self.n_cols, a.occ.numel()
+        h = host.numpy()
+        shape = self.spec.shape
+        sums = h[:2 * n].view(np.float64)
+        cos, sin = sums[:n].reshape(shape).copy(), sums[n:].reshape(shape).copy()
+        occ, flags = h[2 * n:2 * n + g].copy(), h[2 * n + g:]
         req = a.prep.req
         return EntryResult(
             key=req.key, cos=cos, sin=sin, occupancy=occ, n_walkers=req.n_walkers, n_escaped=int(flags[0]),
@@ -295,26 +315,25 @@ class DeviceScheduler:
         )
 
     def poll(self, block: bool = False) -> List[EntryResult]:
-        """Finalise entries whose last generation has completed."""
-        done_gen = -1
+        """Results of entries whose last generation has completed."""
+        out, finished = [], set()
         while self.inflight:
-            gen, ev, ws, owners = self.inflight[0]
+            gen, ev, ws, staged = self.inflight[0]
             if block:
                 ev.synchronize()
             elif not ev.query():
                 break
-            done_gen = gen
             self.inflight.popleft()
             block = False
-        if done_gen < 0:
+            for a, host in staged:
+                # every kernel reading the geometry has completed; the free is
+                # ordered behind whatever the launch stream runs now
+                a.geom.release(_hip.stream_ptr())
+                out.append(self._result(a, host))
+                finished.add(id(a))
+        if not out:
             return []
-        out, keep = [], []
-        for a in self.active:
-            if a.next_tile >= a.n_tiles and a.last_gen <= done_gen:
-                out.append(self._finalize(a))
-            else:
-                keep.append(a)
-        self.active = keep
+        self.active = [a for a in self.active if id(a) not in finished]
         if out:
             free, total = self.torch.cuda.mem_get_info()
             self.hbm_samples.append((time.perf_counter(), total - free))

@@ -4,6 +4,7 @@
 
 #include <cmath>
 #include <cstdarg>
+#include <cstdint>
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
@@ -154,7 +155,10 @@ namespace madi {
 // Geometry memory comes from the stream-ordered allocator on a per-device
 // non-blocking stream (default priority): creating a geometry then waits
 // for its own copies only, not for kernels running on the legacy default
-// stream, and madi_geom_destroy_async can release it in stream order.
+// stream, and madi_geom_destroy_async can release it without waiting for the
+// device.  A block freed on another stream may back the next geometry before
+// that stream reaches the free, so a geometry is released only once every
+// kernel that reads it has finished.
 static int geom_stream(int dev, hipStream_t* out)
 {
     static std::mutex mu;
@@ -1230,19 +1234,58 @@ int madi_batch_graph_destroy(madi_batch_graph* gr)
     return MADI_OK;
 }
 
-int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
-                        double* d_sin, void* stream)
+static bool finalize_plan_ok(const madi_phase_plan* plan)
 {
-    if (!plan || !d_partial || !d_cos || !d_sin || n_tiles < 0) {
-        set_error("madi_tiles_finalize: invalid arguments");
+    return plan && plan->n_pairs >= 0 && plan->n_pairs <= MADI_MAX_PAIRS && plan->n_b >= 0 && plan->n_b <= MADI_MAX_B &&
+           plan->n_dirs >= 0 && plan->n_dirs <= MADI_MAX_DIRS;
+}
+
+int madi_tiles_finalize_workspace(const madi_phase_plan* plan, int64_t n_tiles, int64_t* bytes)
+{
+    if (!finalize_plan_ok(plan) || !bytes || n_tiles < 0) {
+        set_error("madi_tiles_finalize_workspace: invalid arguments");
         return MADI_ERR_ARG;
     }
     const int32_t n_cols = plan->n_pairs * plan->n_b * plan->n_dirs;
+    *bytes = std::max<int64_t>(1, finalize_scratch_doubles(n_tiles, n_cols)) * (int64_t)sizeof(double);
+    return MADI_OK;
+}
+
+int madi_tiles_finalize_ws(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
+                           double* d_sin, void* d_workspace, int64_t workspace_bytes, void* stream)
+{
+    int64_t need = 0;
+    if (!finalize_plan_ok(plan) || !d_partial || !d_cos || !d_sin || !d_workspace || n_tiles < 0 ||
+        ((uintptr_t)d_workspace & 7) != 0) {
+        set_error("madi_tiles_finalize_ws: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    int rc = madi_tiles_finalize_workspace(plan, n_tiles, &need);
+    if (rc) return rc;
+    if (workspace_bytes < need) {
+        set_error("madi_tiles_finalize_ws: workspace of %lld bytes is smaller than the %lld required",
+                  (long long)workspace_bytes, (long long)need);
+        return MADI_ERR_ARG;
+    }
+    const int32_t n_cols = plan->n_pairs * plan->n_b * plan->n_dirs;
+    return launch_finalize_tiles(d_partial, 0, n_tiles, n_cols, d_cos, d_sin, (double*)d_workspace,
+                                 (hipStream_t)stream);
+}
+
+int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
+                        double* d_sin, void* stream)
+{
+    if (!finalize_plan_ok(plan) || !d_partial || !d_cos || !d_sin || n_tiles < 0) {
+        set_error("madi_tiles_finalize: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    int64_t need = 0;
+    int rc = madi_tiles_finalize_workspace(plan, n_tiles, &need);
+    if (rc) return rc;
     hipStream_t s = (hipStream_t)stream;
     double* scratch = nullptr;
-    HIP_TRY(hipMallocAsync((void**)&scratch,
-                           (size_t)std::max<int64_t>(1, finalize_scratch_doubles(n_tiles, n_cols)) * sizeof(double), s));
-    const int rc = launch_finalize_tiles(d_partial, 0, n_tiles, n_cols, d_cos, d_sin, scratch, s);
+    HIP_TRY(hipMallocAsync((void**)&scratch, (size_t)need, s));
+    rc = madi_tiles_finalize_ws(plan, d_partial, n_tiles, d_cos, d_sin, scratch, need, stream);
     (void)hipFreeAsync(scratch, s);
     return rc;
 }

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 3
+#define MADI_ABI_VERSION 4
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -122,11 +122,13 @@ typedef struct madi_geom madi_geom;
 int madi_geom_create(const madi_geom_desc* desc, madi_geom** out);
 /* Waits for all work on the device, then releases the handle. */
 int madi_geom_destroy(madi_geom* g);
-/* Releases the handle in the order of `stream` (a hipStream_t): the memory is
- * freed once the work queued on that stream before this call has run, and
- * the call does not wait.  Every use of the geometry must be on that stream
- * or already finished.  No counterpart in the reference (it re-uploads per
- * walk); for schedulers that retire entries while other work runs. */
+/* Releases the handle without waiting for the device: the memory is returned
+ * to the stream-ordered allocator in the order of `stream` (a hipStream_t).
+ * Every kernel that reads the geometry must have finished (an event recorded
+ * after its last launch has completed): the freed memory may back a geometry
+ * created afterwards before `stream` reaches this point.  No counterpart in
+ * the reference (it re-uploads per walk); for schedulers that retire entries
+ * while other work runs. */
 int madi_geom_destroy_async(madi_geom* g, void* stream);
 /* Device bytes held by the handle (admission control). */
 int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes);
@@ -300,6 +302,14 @@ int madi_batch_graph_destroy(madi_batch_graph* graph);
  * the chunk sums in chunk order (the order madi_walk_fused uses). */
 int madi_tiles_finalize(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
                         double* d_sin, void* stream);
+/* madi_tiles_finalize with its scratch (madi_tiles_finalize_workspace bytes,
+ * 8-byte aligned) passed in: no stream-ordered allocation, so the sum can be
+ * queued behind work still running on `stream` while other streams allocate.
+ * Same order and results.  No counterpart in the reference; for schedulers
+ * that queue each entry's sum behind the launch that completes it. */
+int madi_tiles_finalize_workspace(const madi_phase_plan* plan, int64_t n_tiles, int64_t* bytes);
+int madi_tiles_finalize_ws(const madi_phase_plan* plan, const double* d_partial, int64_t n_tiles, double* d_cos,
+                           double* d_sin, void* d_workspace, int64_t workspace_bytes, void* stream);
 
 /* d[i] = value for i < n, by a fill kernel (usable where memset nodes are not). */
 int madi_fill_i64(int64_t* d, int64_t n, int64_t value, void* stream);
