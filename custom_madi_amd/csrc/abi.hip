@@ -409,7 +409,7 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
 static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != nullptr; }
 static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
 
-static float cap_width_of(float v) { return v > 0.f ? v : 3.0f; }
+static float cap_width_of(float v) { return v > 0.f ? v : 5.0f; }
 // FP32 walk classification mode -> kernel variant (0 per-lane state machine
 // per tile, kVariantStream per-lane state machine with lane refill,
 // kVariantSplit walker and classifier waves); 0 picks the default, the

@@ -166,7 +166,7 @@ typedef struct {
     int32_t n_steps;
     int32_t steps_per_h;
     int32_t n_grid;
-    float tier_cap_um;      /* FP32 walk: plane-tier cap beyond max alpha (0 = default 3 um) */
+    float tier_cap_um;      /* FP32 walk: plane-tier cap beyond max alpha (0 = default 5 um) */
     double sigma;
     double ts;
     double pp;
