@@ -125,6 +125,8 @@ SIGNATURES = {
     "madi_tiles_finalize_workspace": (C.c_int, [C.POINTER(PhasePlan), C.c_int64, _i64p]),
     "madi_tiles_finalize_ws": (C.c_int, [C.POINTER(PhasePlan), _p, C.c_int64, _p, _p, _p, C.c_int64, _p]),
     "madi_fill_i64": (C.c_int, [_p, C.c_int64, C.c_int64, _p]),
+    "madi_map_workspace": (C.c_int, [C.c_int64, C.c_int64, _i64p]),
+    "madi_map_match": (C.c_int, [_p, C.c_int64, _p, C.c_int64, C.c_int32, C.c_int32, _p, C.c_int64, _p, _p, _p, _p]),
 }
 
 _lib: Optional[C.CDLL] = None

@@ -1,8 +1,9 @@
 """Build libmadi_hip.so for gfx950 in-tree with hipcc.
 
-The FP64 reference-semantics kernels (parity.hip) are compiled without FP
-contraction so their arithmetic order is the reference's; the production
-kernels and the ABI use hipcc's defaults.  Every translation unit disables
+The FP64 reference-semantics kernels (parity.hip) and the FP64 library
+matcher (match.hip) are compiled without FP contraction, so their arithmetic
+order is the source's (the matcher fuses only where it calls fma()); the
+production kernels and the ABI use hipcc's defaults.  Every translation unit disables
 packed FP32 instructions.
 """
 
@@ -34,6 +35,7 @@ UNITS = {
     "parity.hip": ["-ffp-contract=off"],
     "prod.hip": [],
     "abi.hip": [],
+    "match.hip": ["-ffp-contract=off"],
 }
 
 

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 4
+#define MADI_ABI_VERSION 5
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -313,6 +313,28 @@ int madi_tiles_finalize_ws(const madi_phase_plan* plan, const double* d_partial,
 
 /* d[i] = value for i < n, by a fill kernel (usable where memset nodes are not). */
 int madi_fill_i64(int64_t* d, int64_t n, int64_t value, void* stream);
+
+/* MAP library matching, FP64.  d_measured is n_vox x n_feat and d_lib
+ * n_lib x n_feat, both row-major; for each voxel row m the library row r of
+ * smallest score, ties to the lowest row:
+ *   fit_s0 = 0: score |r|^2 - 2 m.r, d_best_resid = |m|^2 + score
+ *     (replaces fitters_gpu.map_match_gpu / _map_kernel_fixed,
+ *      madi/fitters_gpu.py:96-118 and :414-438, the GPU path of
+ *      library.match_voxels_batch, madi/library.py:1402-1450);
+ *   fit_s0 = 1: s0 = m.r / rr, score |m|^2 - (m.r)^2 / rr with
+ *     rr = max(|r|^2, 1e-30) and +inf where s0 <= 0; d_best_resid = score,
+ *     d_best_s0 = s0 (replaces map_match_fits0_gpu / _map_kernel_free_s0,
+ *     madi/fitters_gpu.py:120-154 and :441-463, the GPU path of
+ *     library.match_voxels_batch_fits0, madi/library.py:1457-1522).
+ * Dot products and squared norms accumulate over the features in order with
+ * fused multiply-adds.  A row with no finite score returns row 0 (residual
+ * +inf, or |m|^2 + inf, and s0 0), as the reference's scan does.  The
+ * workspace holds madi_map_workspace bytes (8-byte aligned); d_best_s0 may be
+ * NULL when fit_s0 = 0.  Asynchronous on stream. */
+int madi_map_workspace(int64_t n_vox, int64_t n_lib, int64_t* bytes);
+int madi_map_match(const double* d_measured, int64_t n_vox, const double* d_lib, int64_t n_lib, int32_t n_feat,
+                   int32_t fit_s0, void* d_workspace, int64_t workspace_bytes, int64_t* d_best_idx,
+                   double* d_best_resid, double* d_best_s0, void* stream);
 
 #ifdef __cplusplus
 }

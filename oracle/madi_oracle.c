@@ -17,14 +17,20 @@
  *                                       semantics of madi/walker_gpu.py:698-840
  *                                       for samples after an escape)
  *   og_reduce                           madi/walker_gpu.py:954-968
+ *   og_map_match                        madi/fitters_gpu.py:96-154 (MAP
+ *                                       matchers, fixed and free S0)
  *
  * Arithmetic follows the reference GPU formula order (dx*dx+dy*dy+dz*dz, no
- * fused multiply-add).  Build with -ffp-contract=off (see oracle/Makefile).
+ * fused multiply-add except the explicit fma() of the matchers).  Build with
+ * -ffp-contract=off (see oracle/Makefile).
  *
- * Parity pins (tests/test_oracle_cpu.py): brute-force all-seed S2 labels,
- * the trapezoid Y closed form (bitwise), free-water Stejskal-Tanner, an
- * independent GF(2) derivation of the jump, and the numpy restatement of the
- * reference CPU walk on injected streams.
+ * Parity pins (tests/test_oracle_cpu.py, tests/test_match_cpu.py):
+ * brute-force all-seed S2 labels, the trapezoid Y closed form (bitwise),
+ * free-water Stejskal-Tanner, an independent GF(2) derivation of the jump,
+ * the numpy restatement of the reference CPU walk on injected streams; for
+ * the matchers the reference's known-answer test, exact-copy and scaled-copy
+ * libraries with zero residual, and the numpy form of the reference's CPU
+ * matcher path.
  */
 #include <math.h>
 #include <omp.h>
@@ -408,4 +414,67 @@ void og_reduce(const double* Y, int64_t n, int32_t n_grid, const int32_t* jd, co
         cos_out[c] = cs;
         sin_out[c] = sn;
     }
+}
+
+/* ------------------------------------------------------- MAP matching */
+
+/* The reference's per-voxel GPU matchers (madi/fitters_gpu.py:96-118 fixed
+ * S0, :120-154 free S0, called from madi/library.py:1402-1522): squared norms
+ * and dot products accumulate over the features in order with fused
+ * multiply-adds (the contraction GPU compilers apply to the kernels'
+ * `dot += a * b`), and the scan over library rows keeps the first strict
+ * minimum.  fit_s0 = 0: score = |r|^2 - 2 m.r, resid = |m|^2 + best score.
+ * fit_s0 = 1: rr = max(|r|^2, 1e-30) (NaN kept, as np.maximum),
+ * s0 = m.r / rr, resid = |m|^2 - (m.r)^2 / rr, +inf where s0 <= 0.
+ * The reference's CPU path (BLAS) sums in another order; tests compare the
+ * two within a tolerance. */
+static double og_sumsq(const double* x, int32_t nf)
+{
+    double s = 0.0;
+    for (int32_t k = 0; k < nf; ++k) s = fma(x[k], x[k], s);
+    return s;
+}
+
+void og_map_match(const double* M, int64_t n_vox, const double* R, int64_t n_lib, int32_t nf, int32_t fit_s0,
+                  int64_t* idx, double* resid, double* s0_out)
+{
+    double* rn = (double*)malloc((size_t)(n_lib > 0 ? n_lib : 1) * sizeof(double));
+    if (!rn) return;
+    for (int64_t j = 0; j < n_lib; ++j) {
+        double s = og_sumsq(R + (size_t)j * nf, nf);
+        if (fit_s0 && s < 1e-30) s = 1e-30;
+        rn[j] = s;
+    }
+#pragma omp parallel for schedule(static)
+    for (int64_t v = 0; v < n_vox; ++v) {
+        const double* m = M + (size_t)v * nf;
+        const double mm = og_sumsq(m, nf);
+        double best = INFINITY, bs0 = 0.0;
+        int64_t bj = 0;
+        for (int64_t j = 0; j < n_lib; ++j) {
+            const double* r = R + (size_t)j * nf;
+            double dot = 0.0;
+            for (int32_t k = 0; k < nf; ++k) dot = fma(m[k], r[k], dot);
+            if (fit_s0) {
+                const double s0 = dot / rn[j];
+                double res = mm - (dot * dot) / rn[j];
+                if (s0 <= 0.0) res = INFINITY;
+                if (res < best) {
+                    best = res;
+                    bj = j;
+                    bs0 = s0;
+                }
+            } else {
+                const double score = rn[j] - 2.0 * dot;
+                if (score < best) {
+                    best = score;
+                    bj = j;
+                }
+            }
+        }
+        idx[v] = bj;
+        resid[v] = fit_s0 ? best : mm + best;
+        if (fit_s0 && s0_out) s0_out[v] = bs0;
+    }
+    free(rn);
 }

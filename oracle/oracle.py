@@ -5,13 +5,16 @@ leg, as the checker; the shipped package never imports it.
 
 Contents
   * ctypes binding of oracle/libmadi_oracle.so (madi_oracle.c): the FP64 C
-    restatement of the reference walk, classifier, reduce and RNG.
+    restatement of the reference walk, classifier, reduce, RNG and MAP
+    matchers.
   * numpy restatements used as independent checks:
       classify_brute       all-seed SI Eq. S2 labels
                            (tests/physics_audit/test_geometry_diagnostics.py:119-140)
       walk_cpu_reference   the reference CPU walk with cKDTree classification
                            (madi/walker_gpu.py:698-840 + madi/ensemble.py:213-256)
       reduce_reference     madi/walker_gpu.py:954-968
+      map_match_cpu_reference  the reference's CPU MAP matcher arithmetic
+                           (madi/library.py:1441-1450, :1494-1522)
       Xoroshiro128p        pure-Python integer stream (numba-src:45-125)
 """
 
@@ -88,6 +91,9 @@ def lib() -> C.CDLL:
                               _i8p]
         L.og_reduce.restype = None
         L.og_reduce.argtypes = [_f64p, C.c_int64, C.c_int32, _i32p, _i32p, _i32p, _f64p, C.c_int32, _f64p, _f64p]
+        L.og_map_match.restype = None
+        L.og_map_match.argtypes = [_f64p, C.c_int64, _f64p, C.c_int64, C.c_int32, C.c_int32,
+                                   C.POINTER(C.c_int64), _f64p, _f64p]
         _lib = L
     return _lib
 
@@ -240,6 +246,44 @@ def reduce_reference(Y, jd, jD, js, pc):
         cs[c] = np.cos(ph).sum()
         sn[c] = np.sin(ph).sum()
     return cs, sn
+
+
+def map_match(measured, library, fit_s0=False):
+    """C restatement of the reference's GPU MAP matchers (madi/fitters_gpu.py:96-154):
+    (row index, residual, s0 or None) per voxel."""
+    M = np.ascontiguousarray(measured, dtype=np.float64)
+    R = np.ascontiguousarray(library, dtype=np.float64)
+    nv, nf = M.shape
+    nl = R.shape[0]
+    assert R.shape[1] == nf
+    idx = np.empty(nv, np.int64)
+    res = np.empty(nv)
+    s0 = np.empty(nv) if fit_s0 else None
+    lib().og_map_match(_p(M, C.c_double), nv, _p(R, C.c_double), nl, nf, int(bool(fit_s0)),
+                       idx.ctypes.data_as(C.POINTER(C.c_int64)), _p(res, C.c_double), _p(s0, C.c_double))
+    return idx, res, s0
+
+
+def map_match_cpu_reference(measured, library, fit_s0=False):
+    """numpy restatement of the reference's CPU matcher arithmetic
+    (madi/library.py:1441-1450 fixed S0, :1494-1522 free S0): BLAS products,
+    first minimum of the full residual matrix."""
+    M = np.asarray(measured, dtype=np.float64)
+    R = np.asarray(library, dtype=np.float64)
+    rows = np.arange(len(M))
+    if not fit_s0:
+        m2 = np.sum(M ** 2, axis=1, keepdims=True)
+        l2 = np.sum(R ** 2, axis=1, keepdims=True).T
+        dists = m2 + l2 - 2.0 * M @ R.T
+        best = np.argmin(dists, axis=1)
+        return best, dists[rows, best], None
+    rr = np.maximum(np.sum(R * R, axis=1), 1e-30)
+    mm = np.sum(M * M, axis=1)
+    MR = M @ R.T
+    s0 = MR / rr[None, :]
+    resid = np.where(s0 > 0, mm[:, None] - (MR ** 2) / rr[None, :], np.inf)
+    best = np.argmin(resid, axis=1)
+    return best, resid[rows, best], s0[rows, best]
 
 
 def classify_ckdtree(seeds, annulus, pts, tree=None):
