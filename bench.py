@@ -9,6 +9,11 @@ Delta = 15/25/30/40 ms, b = 0/1000/2500/4000/6000 s/mm², 24 directions:
 timed path; a "step" is one full entry (1e6 walkers x 2,300 steps), inputs
 resident on the device.
 
+Other workloads (--workload): c2 (256 uneven library entries, batched vs one
+entry per launch), c3 (served Poisson arrivals, p50/p99 latency), c4 (a
+seeded sample of the full-range sweep), map (MAP matching of 1e6 voxels
+against a 14,105-entry library, FP64).
+
 Prints one JSON line.  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
@@ -191,6 +196,105 @@ def run_c2(args):
     return 0
 
 
+PEAK_FP64_TFLOPS = 78.6
+
+
+def map_workload(n_vox, n_lib, seed=4):
+    """Synthetic matcher inputs: library rows of direction-averaged signals
+    S = exp(-b D(Delta)) for 4 Delta x 5 b (20 columns, pair-major) with
+    per-entry diffusivities, and voxels = a random entry's row + N(0, 0.02)."""
+    rng = np.random.default_rng(seed)
+    b = np.array([0.0, 1.0, 2.5, 4.0, 6.0])            # ms/µm² (s/mm² x 1e-3)
+    Deltas = np.array([15.0, 25.0, 30.0, 40.0])
+    D0 = rng.uniform(0.2, 3.0, (n_lib, 1, 1))
+    r = rng.uniform(0.0, 0.6, (n_lib, 1, 1))
+    Dd = D0 * (1.0 - r * Deltas[None, :, None] / 40.0)
+    R = np.exp(-b[None, None, :] * Dd).reshape(n_lib, -1)
+    M = R[rng.integers(0, n_lib, n_vox)] + rng.normal(0.0, 0.02, (n_vox, R.shape[1]))
+    return np.ascontiguousarray(M), np.ascontiguousarray(R)
+
+
+def run_map(args):
+    """MAP matching of a voxel batch against a library, inputs resident in HBM."""
+    import torch
+    from custom_madi_amd import _hip
+    from custom_madi_amd.match import DeviceLibrary
+    _hip.require_gpu()
+    n_vox, n_lib = args.voxels, args.library_entries
+    M, R = map_workload(n_vox, n_lib)
+    n_feat = R.shape[1]
+    lib = DeviceLibrary(R)
+    dM = torch.from_numpy(M).to("cuda")
+    ws = lib.workspace(n_vox)
+    outs = {fs: (torch.empty(n_vox, dtype=torch.int64, device="cuda"), torch.empty(n_vox, dtype=torch.float64,
+                                                                                     device="cuda"),
+                 torch.empty(n_vox, dtype=torch.float64, device="cuda") if fs else None) for fs in (False, True)}
+    stream = torch.cuda.current_stream()
+
+    def timed(fit_s0, steps, warmup):
+        for _ in range(warmup):
+            lib.match_device(dM, fit_s0, workspace=ws, out=outs[fit_s0])
+        torch.cuda.synchronize()
+        starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+        t0 = time.perf_counter()
+        for k in range(steps):
+            starts[k].record(stream)
+            lib.match_device(dM, fit_s0, workspace=ws, out=outs[fit_s0])
+            ends[k].record(stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        return wall, [s.elapsed_time(e) for s, e in zip(starts, ends)]
+
+    wall, kern_ms = timed(False, args.steps, args.warmup)
+    wall_f, kern_f = timed(True, max(1, args.steps), 1)
+    flops = 2.0 * n_vox * n_lib * n_feat
+    kavg = float(np.mean(kern_ms)) * 1e-3
+    achieved = flops / kavg / 1e12
+    traffic = None
+    pmc = ROOT / "profiles" / "map_match_traffic.json"
+    if pmc.is_file():
+        traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
+    out = {
+        "metric": "voxels/s", "value": n_vox * args.steps / wall, "unit": "voxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * wall / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"map: MAP matching (S0 fixed) of {n_vox} voxels against {n_lib} library entries "
+                               f"x {n_feat} columns (4 Delta x 5 b), FP64", "voxels": n_vox,
+                   "library_entries": n_lib, "columns": n_feat},
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "kernel_ms": kavg * 1e3,
+                     "algorithmic_tflop_per_launch": flops / 1e12},
+        "free_s0": {"ms_per_step": 1e3 * wall_f / max(1, args.steps), "kernel_ms": float(np.mean(kern_f)),
+                    "tflops": flops / (float(np.mean(kern_f)) * 1e-3) / 1e12},
+    }
+    if not args.no_cpu_baseline:
+        from threadpoolctl import threadpool_limits
+        from oracle import oracle as O
+        try:
+            cores = len(os.sched_getaffinity(0))
+        except AttributeError:
+            cores = os.cpu_count() or 1
+        threads = max(1, min(16, cores))
+        batch = 1000                       # the reference's own matching batch (1,000 voxels per call)
+        with threadpool_limits(limits=threads):
+            t0 = time.perf_counter()
+            O.map_match_cpu_reference(M[:batch], R)
+            one = time.perf_counter() - t0
+            n_b = int(max(1, min(n_vox // batch, args.cpu_seconds / max(one, 1e-6))))
+            t0 = time.perf_counter()
+            for q in range(n_b):
+                O.map_match_cpu_reference(M[q * batch:(q + 1) * batch], R)
+            dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {
+            "value": n_b * batch / dt, "unit": "voxels/s", "cores": threads, "kind": "port",
+            "sample": f"{n_b} batches of {batch} voxels against the same {n_lib}-entry library with the reference's "
+                      f"CPU matcher arithmetic (numpy/BLAS, {threads} threads) in {dt:.1f} s on {cpu_model()}",
+        }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _prepared_lookup(preps):
     table = {p.req.key: p for p in preps}
     return lambda req: table[req.key]
@@ -302,7 +406,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map"], default="c1")
+    ap.add_argument("--voxels", type=int, default=1_000_000, help="map: voxels matched per step")
+    ap.add_argument("--library-entries", type=int, default=14_105, help="map: library entries")
     ap.add_argument("--entries", type=int, default=256)
     ap.add_argument("--plain-entries", type=int, default=256)
     ap.add_argument("--workers", type=int, default=16)
@@ -331,6 +437,8 @@ def main(argv=None):
         return run_c3(args)
     if args.workload == "c4":
         return run_c4(args)
+    if args.workload == "map":
+        return run_map(args)
 
     import torch
     from custom_madi_amd import _hip, engine

@@ -6,8 +6,8 @@ entries keyed by rounded (k_io, rho, V), grouped by (rho, V) so geometry is
 reused across the k_io sweep, one checkpoint per group, and the v5
 Monte-Carlo diagnostics (imaginary signal, between-ensemble variance, the
 declared 200-column per-ensemble subset).  The walks run in the HIP kernels
-through ``signal.compute_signals_multi_kio``.  The post-simulation matchers of
-the reference library are outside this build.
+through ``signal.compute_signals_multi_kio``.  The MAP matchers that read a
+library live in ``match.py``.
 """
 
 from __future__ import annotations
