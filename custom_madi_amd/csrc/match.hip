@@ -11,9 +11,12 @@
 //
 // Layout: both matrices row-major FP64 (voxel / library row, then feature).
 // A workgroup of 256 threads owns 128 voxels and walks the whole library in
-// tiles of 64 rows; features are staged through LDS 8 at a time (voxel tile
-// [8][128], library tile [8][64]) and each thread accumulates an 8 x 4 block
-// of dot products (voxels tv + 16 i, rows tl + 16 l) in registers.  Each
+// tiles of 64 rows; each thread accumulates an 8 x 4 block of dot products
+// (voxels tv + 16 i, rows tl + 16 l) in registers.  Up to 24 columns
+// (k_map_match_res) the voxel tile stays in LDS for all columns and the
+// library tiles are double-buffered with a register prefetch, one barrier per
+// tile; wider acquisitions (k_map_match) stage both tiles 8 columns at a
+// time.  Each
 // thread keeps its own best row per voxel in increasing row order; the 16
 // threads of a voxel then combine (score, row) lexicographically, which
 // returns the lowest row among the equal minima exactly as the sequential
@@ -51,7 +54,7 @@ __global__ __launch_bounds__(256) void k_row_sumsq(const double* __restrict__ x,
     out[r] = s;
 }
 
-__device__ __forceinline__ void take_better(double& s, long long& j, double& z, double os, long long oj, double oz)
+__device__ __forceinline__ void take_better(double& s, int& j, double& z, double os, int oj, double oz)
 {
     if (os < s || (os == s && oj < j)) {
         s = os;
@@ -60,6 +63,197 @@ __device__ __forceinline__ void take_better(double& s, long long& j, double& z, 
     }
 }
 
+// One thread's best rows per voxel (rows in increasing order within the thread).
+struct Best {
+    double s[kRV], z[kRV], mn[kRV];
+    int j[kRV];
+};
+
+// Score the thread's kRV x kRL dot products of library tile j0 and keep the
+// first strict minimum per voxel.  Fixed S0: fma(-2, d, rn) equals the
+// reference's rn - 2 d exactly (doubling is exact).
+template <bool FREE>
+__device__ __forceinline__ void score_tile(const double (&acc)[kRV][kRL], int64_t j0, int tl, int64_t n_lib,
+                                           const double* __restrict__ rnorm, Best& B)
+{
+#pragma unroll
+    for (int l = 0; l < kRL; ++l) {
+        const int64_t j = j0 + tl + 16 * l;
+        if (j < n_lib) {
+            const double rn = rnorm[j];
+#pragma unroll
+            for (int i = 0; i < kRV; ++i) {
+                const double d = acc[i][l];
+                if (FREE) {
+                    const double s0 = d / rn;
+                    double res = B.mn[i] - (d * d) / rn;
+                    if (s0 <= 0.0) res = INFINITY;
+                    if (res < B.s[i]) {
+                        B.s[i] = res;
+                        B.j[i] = (int)j;
+                        B.z[i] = s0;
+                    }
+                } else {
+                    const double score = fma(-2.0, d, rn);
+                    if (score < B.s[i]) {
+                        B.s[i] = score;
+                        B.j[i] = (int)j;
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Combine the 16 threads of each voxel (lanes tv, tv+16, tv+32, tv+48 of a
+// wave, then the four waves) and write the block's voxels.  Ws/Wj may alias
+// LDS the caller no longer reads (the caller synchronises before the call).
+template <bool FREE>
+__device__ __forceinline__ void finish(Best& B, int64_t v0, int64_t n_vox, const double* __restrict__ mnorm,
+                                       int64_t* __restrict__ best_idx, double* __restrict__ best_resid,
+                                       double* __restrict__ best_s0, double (*Ws)[4][kTV], int (*Wj)[kTV])
+{
+    const int t = threadIdx.x, tv = t & 15, w = t >> 6;
+#pragma unroll
+    for (int i = 0; i < kRV; ++i) {
+#pragma unroll
+        for (int m = 16; m < 64; m <<= 1) {
+            const double os = __shfl_xor(B.s[i], m, 64);
+            const int oj = __shfl_xor(B.j[i], m, 64);
+            const double oz = FREE ? __shfl_xor(B.z[i], m, 64) : 0.0;
+            take_better(B.s[i], B.j[i], B.z[i], os, oj, oz);
+        }
+    }
+    if ((t & 63) < 16) {
+#pragma unroll
+        for (int i = 0; i < kRV; ++i) {
+            const int v = tv + 16 * i;
+            Ws[0][w][v] = B.s[i];
+            Ws[1][w][v] = B.z[i];
+            Wj[w][v] = B.j[i];
+        }
+    }
+    __syncthreads();
+    if (t < kTV) {
+        double s = Ws[0][0][t], z = Ws[1][0][t];
+        int j = Wj[0][t];
+        for (int q = 1; q < 4; ++q) take_better(s, j, z, Ws[0][q][t], Wj[q][t], Ws[1][q][t]);
+        const int64_t v = v0 + t;
+        if (v < n_vox) {
+            best_idx[v] = j;
+            if (FREE) {
+                best_resid[v] = s;
+                best_s0[v] = z;
+            } else {
+                best_resid[v] = mnorm[v] + s;
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void best_init(Best& B, int64_t v0, int64_t n_vox, const double* __restrict__ mnorm)
+{
+    const int tv = threadIdx.x & 15;
+#pragma unroll
+    for (int i = 0; i < kRV; ++i) {
+        B.s[i] = INFINITY;
+        B.z[i] = 0.0;
+        B.j[i] = 0;
+        const int64_t v = v0 + tv + 16 * i;
+        B.mn[i] = v < n_vox ? mnorm[v] : 0.0;
+    }
+}
+
+// Up to kResF columns: the block's voxel tile stays in LDS for every column,
+// library tiles are double-buffered (the next tile is fetched into registers
+// while the current one is used), one barrier per library tile.
+constexpr int kResF = 24;
+constexpr int kPre = (kTL * kResF + kMatchThreads - 1) / kMatchThreads;
+
+template <bool FREE>
+__global__ __launch_bounds__(kMatchThreads) __attribute__((amdgpu_waves_per_eu(FREE ? 2 : 3))) void k_map_match_res(const double* __restrict__ M, int64_t n_vox,
+                                                                 const double* __restrict__ R, int64_t n_lib,
+                                                                 int32_t nf, const double* __restrict__ mnorm,
+                                                                 const double* __restrict__ rnorm,
+                                                                 int64_t* __restrict__ best_idx,
+                                                                 double* __restrict__ best_resid,
+                                                                 double* __restrict__ best_s0)
+{
+    __shared__ double As[kResF * kTV];      // [k][v]; reused for the final combine
+    __shared__ double Bs[2][kResF * kTL];   // [k][l]
+    const int t = threadIdx.x, tv = t & 15, tl = t >> 4;
+    const int64_t v0 = (int64_t)blockIdx.x * kTV;
+    Best B;
+    best_init(B, v0, n_vox, mnorm);
+
+    // the voxel tile, all columns (row-major reads, consecutive threads)
+    for (int e = t; e < kTV * nf; e += kMatchThreads) {
+        const int v = e / nf, k = e - v * nf;
+        const int64_t gv = v0 + v;
+        As[k * kTV + v] = gv < n_vox ? M[gv * nf + k] : 0.0;
+    }
+    // this thread's slots of a library tile, (row l << 16) | column k per
+    // prefetch register, -1 when unused
+    int slot[kPre];
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+        const int e = t + kMatchThreads * q;
+        const int l = e / nf;
+        slot[q] = e < kTL * nf ? (l << 16) | (e - l * nf) : -1;
+    }
+    double pre[kPre];
+#pragma unroll
+    for (int q = 0; q < kPre; ++q) {
+        const int l = slot[q] >> 16, k = slot[q] & 0xffff;
+        pre[q] = (slot[q] >= 0 && l < n_lib) ? R[(int64_t)l * nf + k] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kPre; ++q)
+        if (slot[q] >= 0) Bs[0][(slot[q] & 0xffff) * kTL + (slot[q] >> 16)] = pre[q];
+    __syncthreads();
+
+    int cur = 0;
+    for (int64_t j0 = 0; j0 < n_lib; j0 += kTL) {
+        const int64_t jn = j0 + kTL;
+        if (jn < n_lib) {
+#pragma unroll
+            for (int q = 0; q < kPre; ++q) {
+                const int l = slot[q] >> 16, k = slot[q] & 0xffff;
+                pre[q] = (slot[q] >= 0 && jn + l < n_lib) ? R[(jn + l) * nf + k] : 0.0;
+            }
+        }
+        double acc[kRV][kRL];
+#pragma unroll
+        for (int i = 0; i < kRV; ++i)
+#pragma unroll
+            for (int l = 0; l < kRL; ++l) acc[i][l] = 0.0;
+        const double* Bc = Bs[cur];
+        for (int k = 0; k < nf; ++k) {
+            double a[kRV], b[kRL];
+#pragma unroll
+            for (int i = 0; i < kRV; ++i) a[i] = As[k * kTV + tv + 16 * i];
+#pragma unroll
+            for (int l = 0; l < kRL; ++l) b[l] = Bc[k * kTL + tl + 16 * l];
+#pragma unroll
+            for (int i = 0; i < kRV; ++i)
+#pragma unroll
+                for (int l = 0; l < kRL; ++l) acc[i][l] = fma(a[i], b[l], acc[i][l]);
+        }
+        score_tile<FREE>(acc, j0, tl, n_lib, rnorm, B);
+        if (jn < n_lib) {
+            // the other buffer was last read before the previous barrier
+#pragma unroll
+            for (int q = 0; q < kPre; ++q)
+                if (slot[q] >= 0) Bs[cur ^ 1][(slot[q] & 0xffff) * kTL + (slot[q] >> 16)] = pre[q];
+        }
+        __syncthreads();
+        cur ^= 1;
+    }
+    finish<FREE>(B, v0, n_vox, mnorm, best_idx, best_resid, best_s0, reinterpret_cast<double (*)[4][kTV]>(As),
+                 reinterpret_cast<int (*)[kTV]>(As + 2 * 4 * kTV));
+}
+
+// Any column count: features staged 8 at a time for both tiles.
 template <bool FREE>
 __global__ __launch_bounds__(kMatchThreads) void k_map_match(const double* __restrict__ M, int64_t n_vox,
                                                              const double* __restrict__ R, int64_t n_lib, int32_t nf,
@@ -71,23 +265,14 @@ __global__ __launch_bounds__(kMatchThreads) void k_map_match(const double* __res
 {
     __shared__ double As[kKC][kTV];
     __shared__ double Bs[kKC][kTL];
-    __shared__ double Ws[2][4][kTV];  // per-wave winners: score, s0
-    __shared__ long long Wj[4][kTV];  // and row
+    __shared__ double Ws[2][4][kTV];
+    __shared__ int Wj[4][kTV];
 
     const int t = threadIdx.x;
     const int tv = t & 15, tl = t >> 4;
     const int64_t v0 = (int64_t)blockIdx.x * kTV;
-
-    double best[kRV], bz[kRV], mn[kRV];
-    long long bj[kRV];
-#pragma unroll
-    for (int i = 0; i < kRV; ++i) {
-        best[i] = INFINITY;
-        bz[i] = 0.0;
-        bj[i] = 0;
-        const int64_t v = v0 + tv + 16 * i;
-        mn[i] = v < n_vox ? mnorm[v] : 0.0;
-    }
+    Best B;
+    best_init(B, v0, n_vox, mnorm);
 
     for (int64_t j0 = 0; j0 < n_lib; j0 += kTL) {
         double acc[kRV][kRL];
@@ -98,7 +283,6 @@ __global__ __launch_bounds__(kMatchThreads) void k_map_match(const double* __res
 
         for (int f0 = 0; f0 < nf; f0 += kKC) {
             const int kc = nf - f0 < kKC ? nf - f0 : kKC;
-            // stage the voxel tile (kTV x kKC) and the library tile (kTL x kKC)
             for (int e = t; e < kTV * kKC; e += kMatchThreads) {
                 const int v = e / kKC, k = e % kKC;
                 const int64_t gv = v0 + v;
@@ -123,75 +307,9 @@ __global__ __launch_bounds__(kMatchThreads) void k_map_match(const double* __res
             }
             __syncthreads();
         }
-
-        // this thread's rows in increasing order: first strict minimum wins
-#pragma unroll
-        for (int l = 0; l < kRL; ++l) {
-            const int64_t j = j0 + tl + 16 * l;
-            if (j < n_lib) {
-                const double rn = rnorm[j];
-#pragma unroll
-                for (int i = 0; i < kRV; ++i) {
-                    const double d = acc[i][l];
-                    if (FREE) {
-                        const double s0 = d / rn;
-                        double res = mn[i] - (d * d) / rn;
-                        if (s0 <= 0.0) res = INFINITY;
-                        if (res < best[i]) {
-                            best[i] = res;
-                            bj[i] = j;
-                            bz[i] = s0;
-                        }
-                    } else {
-                        const double score = rn - 2.0 * d;
-                        if (score < best[i]) {
-                            best[i] = score;
-                            bj[i] = j;
-                        }
-                    }
-                }
-            }
-        }
+        score_tile<FREE>(acc, j0, tl, n_lib, rnorm, B);
     }
-
-    // combine the 16 threads of each voxel: lanes tv, tv+16, tv+32, tv+48 of a
-    // wave, then the four waves
-    const int w = t >> 6;
-#pragma unroll
-    for (int i = 0; i < kRV; ++i) {
-#pragma unroll
-        for (int m = 16; m < 64; m <<= 1) {
-            const double os = __shfl_xor(best[i], m, 64);
-            const long long oj = __shfl_xor(bj[i], m, 64);
-            const double oz = __shfl_xor(bz[i], m, 64);
-            take_better(best[i], bj[i], bz[i], os, oj, oz);
-        }
-    }
-    if ((t & 63) < 16) {
-#pragma unroll
-        for (int i = 0; i < kRV; ++i) {
-            const int v = tv + 16 * i;
-            Ws[0][w][v] = best[i];
-            Ws[1][w][v] = bz[i];
-            Wj[w][v] = bj[i];
-        }
-    }
-    __syncthreads();
-    if (t < kTV) {
-        double s = Ws[0][0][t], z = Ws[1][0][t];
-        long long j = Wj[0][t];
-        for (int q = 1; q < 4; ++q) take_better(s, j, z, Ws[0][q][t], Wj[q][t], Ws[1][q][t]);
-        const int64_t v = v0 + t;
-        if (v < n_vox) {
-            best_idx[v] = j;
-            if (FREE) {
-                best_resid[v] = s;
-                best_s0[v] = z;
-            } else {
-                best_resid[v] = mnorm[v] + s;
-            }
-        }
-    }
+    finish<FREE>(B, v0, n_vox, mnorm, best_idx, best_resid, best_s0, Ws, Wj);
 }
 
 int64_t map_workspace_doubles(int64_t n_vox, int64_t n_lib) { return n_vox + n_lib; }
@@ -232,7 +350,7 @@ int madi_map_match(const double* d_measured, int64_t n_vox, const double* d_lib,
         return MADI_ERR_ARG;
     }
     const int64_t blocks = (n_vox + kTV - 1) / kTV;
-    if (blocks > 0x7fffffffLL || (n_lib + 255) / 256 > 0x7fffffffLL) {
+    if (blocks > 0x7fffffffLL || n_lib > 0x7fffffffLL - kTL) {
         set_error("madi_map_match: %lld voxels or %lld library rows exceed one launch", (long long)n_vox,
                   (long long)n_lib);
         return MADI_ERR_ARG;
@@ -246,12 +364,20 @@ int madi_map_match(const double* d_measured, int64_t n_vox, const double* d_lib,
     hipLaunchKernelGGL(k_row_sumsq, dim3((unsigned)((n_lib + 255) / 256)), dim3(256), 0, s, d_lib, n_lib, n_feat,
                        fit_s0 ? 1e-30 : 0.0, rnorm);
     if ((rc = check_launch("k_row_sumsq"))) return rc;
-    if (fit_s0)
-        hipLaunchKernelGGL(k_map_match<true>, dim3((unsigned)blocks), dim3(kMatchThreads), 0, s, d_measured, n_vox,
-                           d_lib, n_lib, n_feat, mnorm, rnorm, d_best_idx, d_best_resid, d_best_s0);
+    const dim3 grid((unsigned)blocks), block(kMatchThreads);
+    const bool res = n_feat <= kResF;
+    if (fit_s0 && res)
+        hipLaunchKernelGGL(k_map_match_res<true>, grid, block, 0, s, d_measured, n_vox, d_lib, n_lib, n_feat, mnorm,
+                           rnorm, d_best_idx, d_best_resid, d_best_s0);
+    else if (res)
+        hipLaunchKernelGGL(k_map_match_res<false>, grid, block, 0, s, d_measured, n_vox, d_lib, n_lib, n_feat, mnorm,
+                           rnorm, d_best_idx, d_best_resid, d_best_s0);
+    else if (fit_s0)
+        hipLaunchKernelGGL(k_map_match<true>, grid, block, 0, s, d_measured, n_vox, d_lib, n_lib, n_feat, mnorm, rnorm,
+                           d_best_idx, d_best_resid, d_best_s0);
     else
-        hipLaunchKernelGGL(k_map_match<false>, dim3((unsigned)blocks), dim3(kMatchThreads), 0, s, d_measured, n_vox,
-                           d_lib, n_lib, n_feat, mnorm, rnorm, d_best_idx, d_best_resid, d_best_s0);
+        hipLaunchKernelGGL(k_map_match<false>, grid, block, 0, s, d_measured, n_vox, d_lib, n_lib, n_feat, mnorm, rnorm,
+                           d_best_idx, d_best_resid, d_best_s0);
     return check_launch("k_map_match");
 }
 

@@ -28,7 +28,8 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("fit_s0", [False, True])
-@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 3, 3), (129, 65, 9), (300, 200, 20), (257, 130, 0),
+@pytest.mark.parametrize("shape", [(1, 1, 1), (7, 3, 3), (129, 65, 9), (300, 200, 20), (257, 130, 0), (130, 70, 37),
+                                   (200, 129, 24), (150, 64, 25),
                                    (64, 1000, 480), (2000, 3001, 20)])
 def test_kernel_equals_oracle_bitwise(shape, fit_s0):
     nv, nl, nf = shape
