@@ -382,9 +382,18 @@ __device__ __forceinline__ float u32_to_f32(uint32_t v)
     return f;
 }
 
+// max(v, 0.5) as one instruction for a converted integer v (never NaN; the
+// compiler's fmaxf would first canonicalise v with a second v_max).
+__device__ __forceinline__ float max_half(float v)
+{
+    float m;
+    asm("v_max_f32 %0, 0.5, %1" : "=v"(m) : "v"(v));
+    return m;
+}
+
 __device__ __forceinline__ float normal_scaled_f32(Rng& r, float c)
 {
-    const float h1 = fmaxf(u32_to_f32((uint32_t)(rng_next(r) >> 32)), 0.5f);
+    const float h1 = max_half(u32_to_f32((uint32_t)(rng_next(r) >> 32)));
     const float u2 = u32_to_f32((uint32_t)(rng_next(r) >> 32)) * 0x1.0p-32f;
     return __builtin_amdgcn_sqrtf(fmaf(c, __builtin_amdgcn_logf(h1), -32.f * c)) * __builtin_amdgcn_cosf(u2);
 }
@@ -853,7 +862,7 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
     const float lox = (float)(-x0), loy = (float)(-y0), loz = (float)(-z0);
     const float hix = (float)(g.L - x0), hiy = (float)(g.L - y0), hiz = (float)(g.L - z0);
 
-    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+    float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
     float dM[NP][3];
 #pragma unroll
     for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
@@ -955,19 +964,24 @@ __device__ __forceinline__ void walk_tile_f32(const GeomDev& g, const WalkDev& p
                 oz = P.oz;
             }
             const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
-            const float yx = Dx - Cx, tx = Sx + yx;
-            Cx = (tx - Sx) - yx;
-            Sx = tx;
-            const float yy = Dy - Cy, ty = Sy + yy;
-            Cy = (ty - Sy) - yy;
-            Sy = ty;
-            const float yz = Dz - Cz, tz = Sz + yz;
-            Cz = (tz - Sz) - yz;
-            Sz = tz;
+            hx += Dx;
+            hy += Dy;
+            hz += Dz;
             ++step;
             if (--until_sample == 0) {
                 until_sample = p.steps_per_h;
                 const int j = ++j_sample;
+                // fold the interval's plain sum into the compensated total
+                const float yx = hx - Cx, tx = Sx + yx;
+                Cx = (tx - Sx) - yx;
+                Sx = tx;
+                const float yy = hy - Cy, ty = Sy + yy;
+                Cy = (ty - Sy) - yy;
+                Sy = ty;
+                const float yz = hz - Cz, tz = Sz + yz;
+                Cz = (tz - Sz) - yz;
+                Sz = tz;
+                hx = hy = hz = 0.f;
                 if (in) {
                     if (occ_in_lds) atomicAdd(&L.occ[j], 1);
                     else atomicAdd((unsigned long long*)&out.occupancy[j], 1ull);
@@ -1156,7 +1170,7 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
             float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
             Tier t;
             tier_clear(t);
-            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
             float dM[NP][3];
 #pragma unroll
             for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
@@ -1212,7 +1226,7 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
                         hix = (float)(g.L - x0);
                         hiy = (float)(g.L - y0);
                         hiz = (float)(g.L - z0);
-                        Sx = Sy = Sz = Cx = Cy = Cz = 0.f;
+                        Sx = Sy = Sz = Cx = Cy = Cz = hx = hy = hz = 0.f;
 #pragma unroll
                         for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
                         step = 0;
@@ -1339,19 +1353,24 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
                         oz = P.oz;
                     }
                     const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
-                    const float yx = Dx - Cx, tx = Sx + yx;
-                    Cx = (tx - Sx) - yx;
-                    Sx = tx;
-                    const float yy = Dy - Cy, ty = Sy + yy;
-                    Cy = (ty - Sy) - yy;
-                    Sy = ty;
-                    const float yz = Dz - Cz, tz = Sz + yz;
-                    Cz = (tz - Sz) - yz;
-                    Sz = tz;
+                    hx += Dx;
+                    hy += Dy;
+                    hz += Dz;
                     ++step;
                     if (--until_sample == 0) {
                         until_sample = p.steps_per_h;
                         const int j = ++j_sample;
+                        // fold the interval's plain sum into the compensated total
+                        const float yx = hx - Cx, tx = Sx + yx;
+                        Cx = (tx - Sx) - yx;
+                        Sx = tx;
+                        const float yy = hy - Cy, ty = Sy + yy;
+                        Cy = (ty - Sy) - yy;
+                        Sy = ty;
+                        const float yz = hz - Cz, tz = Sz + yz;
+                        Cz = (tz - Sz) - yz;
+                        Sz = tz;
+                        hx = hy = hz = 0.f;
                         if (in) {
                             if (occ_in_lds) atomicAdd(&occ[j], 1);
                             else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
@@ -1580,7 +1599,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
             float lox = 0.f, loy = 0.f, loz = 0.f, hix = 0.f, hiy = 0.f, hiz = 0.f;
             Tier t;
             tier_clear(t);
-            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f;
+            float Sx = 0.f, Sy = 0.f, Sz = 0.f, Cx = 0.f, Cy = 0.f, Cz = 0.f, hx = 0.f, hy = 0.f, hz = 0.f;
             float dM[NP][3];
 #pragma unroll
             for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
@@ -1625,7 +1644,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         hix = (float)(g.L - x0);
                         hiy = (float)(g.L - y0);
                         hiz = (float)(g.L - z0);
-                        Sx = Sy = Sz = Cx = Cy = Cz = 0.f;
+                        Sx = Sy = Sz = Cx = Cy = Cz = hx = hy = hz = 0.f;
 #pragma unroll
                         for (int q = 0; q < NP; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.f;
                         step = 0;
@@ -1787,19 +1806,24 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         oz = qoz;
                     }
                     const float Dx = ox + px, Dy = oy + py, Dz = oz + pz;
-                    const float yx = Dx - Cx, tx = Sx + yx;
-                    Cx = (tx - Sx) - yx;
-                    Sx = tx;
-                    const float yy = Dy - Cy, ty = Sy + yy;
-                    Cy = (ty - Sy) - yy;
-                    Sy = ty;
-                    const float yz = Dz - Cz, tz = Sz + yz;
-                    Cz = (tz - Sz) - yz;
-                    Sz = tz;
+                    hx += Dx;
+                    hy += Dy;
+                    hz += Dz;
                     ++step;
                     if (--until_sample == 0) {
                         until_sample = p.steps_per_h;
                         const int j = ++j_sample;
+                        // fold the interval's plain sum into the compensated total
+                        const float yx = hx - Cx, tx = Sx + yx;
+                        Cx = (tx - Sx) - yx;
+                        Sx = tx;
+                        const float yy = hy - Cy, ty = Sy + yy;
+                        Cy = (ty - Sy) - yy;
+                        Sy = ty;
+                        const float yz = hz - Cz, tz = Sz + yz;
+                        Cz = (tz - Sz) - yz;
+                        Sz = tz;
+                        hx = hy = hz = 0.f;
                         if (in) {
                             if (occ_in_lds) atomicAdd(&occ[j], 1);
                             else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
