@@ -295,6 +295,45 @@ def test_hip_library_exports_every_declared_symbol():
     assert set(_hip.SIGNATURES) >= declared
 
 
+def test_matcher_and_finalize_abi_argument_checks():
+    """Workspace sizes and argument errors of the matcher and tile-sum entry
+    points, which return before any device call."""
+    import ctypes as C
+    from custom_madi_amd import _hip
+    lib = _hip.load_library()
+    b = C.c_int64()
+    assert lib.madi_map_workspace(10, 3, C.byref(b)) == _hip.MADI_OK and b.value == 13 * 8
+    assert lib.madi_map_workspace(0, 0, C.byref(b)) == _hip.MADI_OK and b.value == 8
+    assert lib.madi_map_workspace(-1, 3, C.byref(b)) == _hip.MADI_ERR_ARG
+    fake = 4096  # never dereferenced: each call below returns before touching memory
+    args = dict(meas=fake, n_vox=10, lib=fake, n_lib=3, nf=4, fit=0, ws=fake, wsb=13 * 8, idx=fake, res=fake, s0=None)
+
+    def call(**kw):
+        a = {**args, **kw}
+        return lib.madi_map_match(a["meas"], a["n_vox"], a["lib"], a["n_lib"], a["nf"], a["fit"], a["ws"], a["wsb"],
+                                  a["idx"], a["res"], a["s0"], None)
+
+    assert call(n_lib=0) == _hip.MADI_ERR_ARG
+    assert call(fit=2) == _hip.MADI_ERR_ARG
+    assert call(fit=1) == _hip.MADI_ERR_ARG            # free S0 needs the S0 output
+    assert call(ws=None) == _hip.MADI_ERR_ARG
+    assert call(ws=fake + 4) == _hip.MADI_ERR_ARG      # 8-byte alignment
+    assert call(nf=-1) == _hip.MADI_ERR_ARG
+    assert call(wsb=12 * 8) == _hip.MADI_ERR_ARG
+    assert "smaller than" in _hip.last_error()
+    assert call(n_vox=0, idx=None, res=None, wsb=8) == _hip.MADI_ERR_ARG  # the library norms still need 3 x 8 bytes
+    assert call(n_vox=0, idx=None, res=None, wsb=3 * 8) == _hip.MADI_OK  # nothing to match
+    plan = _hip.PhasePlan()
+    plan.n_pairs, plan.n_b, plan.n_dirs = 4, 5, 24
+    assert lib.madi_tiles_finalize_workspace(C.byref(plan), 100, C.byref(b)) == _hip.MADI_OK
+    assert b.value == 4 * 480 * 2 * 8                  # ceil(100 / 32) chunks x 480 columns x (cos, sin)
+    plan.n_pairs = 9
+    assert lib.madi_tiles_finalize_workspace(C.byref(plan), 100, C.byref(b)) == _hip.MADI_ERR_ARG
+    plan.n_pairs = 4
+    assert lib.madi_tiles_finalize_ws(C.byref(plan), fake, 100, fake, fake, fake, 4 * 480 * 2 * 8 - 8,
+                                      None) == _hip.MADI_ERR_ARG
+
+
 def test_ctypes_structs_match_the_header(tmp_path):
     """Every ctypes mirror of a header struct has the C compiler's size and field offsets."""
     import ctypes
