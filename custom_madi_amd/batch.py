@@ -21,12 +21,11 @@ prepare, upload, one launch, synchronise, next entry.
 from __future__ import annotations
 
 import ctypes as C
-import math
 import sys
 import time
 from collections import deque
-from dataclasses import dataclass, field
-from typing import Callable, Iterable, List, Optional
+from dataclasses import dataclass
+from typing import Callable, Iterable, List
 
 import numpy as np
 
