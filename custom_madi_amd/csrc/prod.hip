@@ -554,7 +554,10 @@ __device__ __forceinline__ void tier_make(const GeomDev& g, int a, int c, int c2
     const float4 e2 = plane2 ? e20 : make_float4(0.f, 0.f, 0.f, hr);
     const float4 e3 = plane3 ? e30 : make_float4(0.f, 0.f, 0.f, hr);
     const int4 ar = gload(g.cell_rec + a);
-    const float aa = __int_as_float(ar.w), ak = gload(g.cell_alpha + k);
+    // k's annulus from the edge's own copy (nb_kalpha[c] = cell_alpha[nb_id[c]]),
+    // so every load here depends on the scan's edges only: one round trip
+    const float ka = gload(g.nb_kalpha + ci);
+    const float aa = __int_as_float(ar.w), ak = plane ? ka : aa;
     t.a_off = ar.x;
     t.a_end = ar.y;
     t.a_hmax = __int_as_float(ar.z);
