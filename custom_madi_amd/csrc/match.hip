@@ -94,11 +94,11 @@ __device__ __forceinline__ void score_tile(const double (&acc)[kRV][kRL], int64_
                         B.z[i] = s0;
                     }
                 } else {
+                    // scores are never -0 (rn >= +0), and a NaN score never
+                    // wins: fmin keeps the running minimum as the strict scan does
                     const double score = fma(-2.0, d, rn);
-                    if (score < B.s[i]) {
-                        B.s[i] = score;
-                        B.j[i] = (int)j;
-                    }
+                    B.j[i] = score < B.s[i] ? (int)j : B.j[i];
+                    B.s[i] = fmin(B.s[i], score);
                 }
             }
         }
@@ -222,13 +222,22 @@ __global__ __launch_bounds__(kMatchThreads) __attribute__((amdgpu_waves_per_eu(F
                 pre[q] = (slot[q] >= 0 && jn + l < n_lib) ? R[(jn + l) * nf + k] : 0.0;
             }
         }
+        // the first column's products start the sums: a*b equals fma(a, b, +0)
+        // but for the sign of a zero, which no score or S0 test can see
         double acc[kRV][kRL];
-#pragma unroll
-        for (int i = 0; i < kRV; ++i)
-#pragma unroll
-            for (int l = 0; l < kRL; ++l) acc[i][l] = 0.0;
         const double* Bc = Bs[cur];
-        for (int k = 0; k < nf; ++k) {
+        if (nf > 0) {
+#pragma unroll
+            for (int i = 0; i < kRV; ++i)
+#pragma unroll
+                for (int l = 0; l < kRL; ++l) acc[i][l] = As[tv + 16 * i] * Bc[tl + 16 * l];
+        } else {
+#pragma unroll
+            for (int i = 0; i < kRV; ++i)
+#pragma unroll
+                for (int l = 0; l < kRL; ++l) acc[i][l] = 0.0;
+        }
+        for (int k = 1; k < nf; ++k) {
             double a[kRV], b[kRL];
 #pragma unroll
             for (int i = 0; i < kRV; ++i) a[i] = As[k * kTV + tv + 16 * i];
