@@ -211,3 +211,50 @@ def test_fp32_walk_matches_fp64_statistically(which, request):
     assert dS.max() <= 2e-3
     occ_rel = np.abs(f32.occupancy - f64.occupancy) / n
     assert occ_rel.max() < 0.01
+
+
+def test_config0_entry_fp64_matches_oracle():
+    """The CPU-runnable configuration end to end: periodic cube of 64 cells
+    (seeds from default_rng(0)), rho = 6e5 / uL, V = 1 pL (v_i = 0.6),
+    k_io = 20 / s (p_p by Eq. 5), D = 2.0, dt = 0.05 ms x 920 steps, 20,000
+    walkers, FP64; 4 Delta x 5 b x 24 directions.  The fused FP64 entry
+    (walk + phase sums in one launch) against the oracle's walk with the
+    directional sums formed in numpy from its Y."""
+    from custom_madi_amd.config import SimConfig
+    from custom_madi_amd.ensemble import create_periodic_ensemble
+    from custom_madi_amd.walker_gpu import kio_to_pp
+    cfg = SimConfig(D0=2.0, ts=0.05, T_max_ms=46.0, h_ms=1.0, n_walkers=20_000, small_deltas=[6.0],
+                    big_deltas=[15.0, 25.0, 30.0, 40.0], b_values=[0.0, 1000.0, 2500.0, 4000.0, 6000.0],
+                    periodic=True, precision="fp64")
+    assert cfg.n_steps == 920 and cfg.steps_per_h == 20
+    ens = create_periodic_ensemble(64, 6.0e5, 0.6, cfg, seed=0)
+    pp = kio_to_pp(20.0, ens.mean_AV, cfg)
+    dirs = np.random.default_rng(1).normal(size=(24, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    Deltas = [15.0, 25.0, 30.0, 40.0]
+    pc = np.array([[engine.phase_coef(b, 6.0, D) for b in cfg.b_values] for D in Deltas])
+    spec = engine.PhaseSpec(j_delta=np.array([6] * 4), j_Delta=np.array([int(D) for D in Deltas]),
+                            j_sum=np.array([int(D + 6.0) for D in Deltas]), pc=pc, dirs=dirs)
+    N = cfg.n_walkers
+    say(f"config 0: L={ens.L:.3f} um, pp={pp:.6g}, {N} walkers x {cfg.n_steps} steps")
+    p = engine.walk_params(N, cfg, pp, 0)
+    fused = engine.walk_fused(ens.device_geometry(), p, spec, precision=64)
+    arr = geometry_arrays(ens)
+    Y, itr, esc, err = O.walk(O.Geometry.from_arrays(arr), N, cfg.n_steps, cfg.steps_per_h, cfg.n_grid, cfg.sigma,
+                              cfg.ts, pp, arr.source_lo, arr.source_hi, seed=0)
+    assert fused.n_escaped == 0 and fused.n_error == 0 and not esc.any() and not err.any()
+    assert np.array_equal(fused.occupancy, itr.sum(axis=0))
+    ocos = np.empty((4, 5, 24))
+    osin = np.empty((4, 5, 24))
+    for k in range(4):
+        dM = Y[:, spec.j_delta[k]] + Y[:, spec.j_Delta[k]] - Y[:, spec.j_sum[k]]
+        proj = dM @ dirs.T
+        for b in range(5):
+            ph = pc[k, b] * proj
+            ocos[k, b] = np.cos(ph).sum(axis=0)
+            osin[k, b] = np.sin(ph).sum(axis=0)
+    assert np.allclose(fused.cos, ocos, rtol=1e-9, atol=1e-7)
+    assert np.allclose(fused.sin, osin, rtol=1e-9, atol=1e-7)
+    S = fused.cos / N
+    say(f"config 0: S(b=6000, Delta=40) mean over directions {S[3, 4].mean():.5f}")
+    assert 0.0 < S[3, 4].mean() < S[0, 1].mean() < 1.0
