@@ -247,3 +247,21 @@ def test_graph_replays_equal_eager_launches(small_requests, spec):
     assert len(out) == len(preps)
     for r in out:
         assert _same(r, eager[r.key]), r.key
+
+
+def test_entries_retire_while_others_are_admitted(spec):
+    """One-tile generations with entries admitted between them: every entry's
+    geometry is uploaded while earlier entries finish and release theirs.  A
+    geometry released before the generation that reads it has completed could
+    be overwritten by the next upload; every result must equal the batched
+    run's bit for bit."""
+    from custom_madi_amd import serve
+    reqs = batch.library_requests(12, seed=57, walkers=(700, 2500), cells=(60, 160), T_max_ms=12.0)
+    sch = batch.DeviceScheduler(spec, max_tiles_per_generation=1)
+    srv = serve.Server(sch)
+    rep = serve.serve_poisson(srv, reqs, np.linspace(0.0, 0.2, len(reqs)))
+    assert rep["finished"] == len(reqs) and sch.gen >= sum(-(-r.n_walkers // 256) for r in reqs)
+    ref = {r.key: r for r in batch.run_batched([batch.prepare_entry(r) for r in reqs], spec)}
+    for r in srv.results:
+        assert r.n_escaped == 0 and r.n_error == 0
+        assert _same(r, ref[r.key])
