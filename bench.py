@@ -164,6 +164,7 @@ def run_c2(args):
     res_pg = batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps), **walk_knobs)
     t_plain_dev = time.perf_counter() - t0
     pool.close()
+    pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
 
     same = sum(int(np.array_equal(res_b[r.key].cos, r.cos) and np.array_equal(res_b[r.key].sin, r.sin)
                    and np.array_equal(res_b[r.key].occupancy, r.occupancy)) for r in res_p)
@@ -172,7 +173,7 @@ def run_c2(args):
     eph_p = n_plain / t_plain * 3600.0
     eph_pd = n_plain / t_plain_dev * 3600.0
     lat = [r.latency_s for r in srv.results]
-    esc = sum(r.n_escaped + r.n_error for r in srv.results)
+    esc = len(srv.failed)  # failed entries (escape or classifier failure)
     out = {
         "metric": "library entries/h", "value": eph_b, "unit": "entries/h", "n_gpus": 1, "higher_is_better": True,
         "workload": "c2: 256 uneven entries (k_io log-U 1-130/s, V log-U 0.1-20 pL, v_i U 0.5-0.95, 50-4000 cells, "
@@ -322,6 +323,7 @@ def run_c3(args):
     sch.close()
     stream.close()
     pool.close()
+    pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
     hbm = rep.pop("hbm_used_bytes_samples")
     out = {"metric": "p50/p99 entry latency", "unit": "s", "n_gpus": 1, "higher_is_better": False,
            "workload": f"c3: Poisson {args.rate}/s for {args.duration:.0f} s, entries as c2 with 1e5-1e6 walkers, "
@@ -361,6 +363,7 @@ def run_c4(args):
     wall = time.perf_counter() - t0
     sch.close()
     pool.close()
+    pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
     res = {r.key: r for r in srv.results}
     log(f"batched: {wall:.1f} s")
     errs = []
@@ -377,7 +380,7 @@ def run_c4(args):
                      "max_abs_dS_over_3se": float((dS / (3 * se + 1e-300)).max()),
                      "per_Delta_b_max_abs_dS": dS.max(axis=2).tolist(), "fp64_escaped": f64.n_escaped})
         log(f"fp64 check {r.key}: max |dS| {dS.max():.2e}")
-    esc = sum(r.n_escaped + r.n_error for r in srv.results)
+    esc = len(srv.failed)  # failed entries (escape or classifier failure)
     lat = [r.latency_s for r in srv.results]
     out = {"metric": "library entries/h", "value": len(reqs) / wall * 3600.0, "unit": "entries/h", "n_gpus": 1,
            "higher_is_better": True,
@@ -481,8 +484,8 @@ def main(argv=None):
     if res.n_escaped or res.n_error:
         raise RuntimeError(f"walk reported escapes={res.n_escaped} errors={res.n_error}")
 
-    # per launch counters are accumulated over every launch
-    launches = args.warmup + args.steps
+    # madi_walk_fused zeroes the counters: they hold the last launch's values
+    launches = 1
     facets = res.counters["facets"] / launches
     flops = walker_steps * F_STEP + facets * F_FACET + N * F_WALKER
     kavg = float(np.mean(kern_ms)) * 1e-3

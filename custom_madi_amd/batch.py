@@ -25,7 +25,7 @@ import sys
 import time
 from collections import deque
 from dataclasses import dataclass
-from typing import Callable, Iterable, List
+from typing import Callable, Iterable, List, Optional
 
 import numpy as np
 
@@ -95,6 +95,11 @@ class EntryResult:
     admitted: float
     finished: float
     prep_s: float
+    error: Optional[str] = None  # set when the entry failed (escape or classifier failure)
+
+    @property
+    def failed(self) -> bool:
+        return self.error is not None
 
     @property
     def signal(self) -> np.ndarray:
@@ -307,10 +312,11 @@ class DeviceScheduler:
         cos, sin = sums[:n].reshape(shape).copy(), sums[n:].reshape(shape).copy()
         occ, flags = h[2 * n:2 * n + g].copy(), h[2 * n + g:]
         req = a.prep.req
+        n_esc, n_err = int(flags[0]), int(flags[1])
         return EntryResult(
-            key=req.key, cos=cos, sin=sin, occupancy=occ, n_walkers=req.n_walkers, n_escaped=int(flags[0]),
-            n_error=int(flags[1]), pp=a.prep.pp, submitted=req.submitted, admitted=a.admitted,
-            finished=time.perf_counter(), prep_s=a.prep.prep_s,
+            key=req.key, cos=cos, sin=sin, occupancy=occ, n_walkers=req.n_walkers, n_escaped=n_esc,
+            n_error=n_err, pp=a.prep.pp, submitted=req.submitted, admitted=a.admitted,
+            finished=time.perf_counter(), prep_s=a.prep.prep_s, error=entry_error(n_esc, n_err, req.n_walkers),
         )
 
     def poll(self, block: bool = False) -> List[EntryResult]:
@@ -353,8 +359,37 @@ def torch_uint8(torch):
     return torch.uint8
 
 
-def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, **kw) -> List[EntryResult]:
-    """Admit every prepared entry as it arrives and run generations until all finish."""
+def entry_error(n_escaped: int, n_error: int, n_walkers: int) -> Optional[str]:
+    """The reference's fatal outcome of a walk, or None: a classifier failure
+    (madi/walker_gpu.py:938-940) or walkers leaving the simulation domain
+    (madi/walker_gpu.py:1023-1028).  A failed entry has no valid signal."""
+    if n_error:
+        return f"GPU full-facet classification failed for {n_error}/{n_walkers} walkers"
+    if n_escaped:
+        return f"SI §S.III fatal boundary condition: {n_escaped}/{n_walkers} walkers exited Ω_sim."
+    return None
+
+
+def split_failed(results: List[EntryResult], failed: Optional[list]) -> List[EntryResult]:
+    """Good results; failed ones go to `failed`, or raise RuntimeError (after
+    every entry has finished) when the caller gave no list for them."""
+    good = [r for r in results if not r.failed]
+    bad = [r for r in results if r.failed]
+    if failed is not None:
+        failed.extend(bad)
+    elif bad:
+        raise RuntimeError("; ".join(f"entry {r.key}: {r.error}" for r in bad))
+    return good
+
+
+def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, failed: Optional[list] = None,
+                **kw) -> List[EntryResult]:
+    """Admit every prepared entry as it arrives and run generations until all finish.
+
+    An entry whose walk failed (escape or classifier failure) fails alone: it
+    is left out of the returned results and appended to `failed` (without a
+    list, RuntimeError once every entry has finished); the other entries'
+    results do not depend on it."""
     sch = DeviceScheduler(spec, **kw)
     waiting: deque = deque()
     results: List[EntryResult] = []
@@ -373,13 +408,14 @@ def run_batched(preps: Iterable[PreparedEntry], spec: engine.PhaseSpec, **kw) ->
             raise MemoryError("an entry does not fit in free HBM even with the device idle")
         results += sch.poll(block=True)
     sch.close()
-    return results
+    return split_failed(results, failed)
 
 
 def run_plain(requests: Iterable[EntryRequest], spec: engine.PhaseSpec,
               prepare: Callable[[EntryRequest], PreparedEntry] = prepare_entry, classify_mode: int = 0,
-              scan_rounds: int = 0) -> List[EntryResult]:
-    """One entry per launch: prepare, upload, launch, synchronise, in sequence."""
+              scan_rounds: int = 0, failed: Optional[list] = None) -> List[EntryResult]:
+    """One entry per launch: prepare, upload, launch, synchronise, in sequence
+    (failed entries as in run_batched)."""
     results = []
     for req in requests:
         t_admit = time.perf_counter()
@@ -392,9 +428,10 @@ def run_plain(requests: Iterable[EntryRequest], spec: engine.PhaseSpec,
             key=req.key, cos=res.cos, sin=res.sin, occupancy=res.occupancy, n_walkers=req.n_walkers,
             n_escaped=res.n_escaped, n_error=res.n_error, pp=prep.pp, submitted=req.submitted, admitted=t_admit,
             finished=time.perf_counter(), prep_s=prep.prep_s,
+            error=entry_error(res.n_escaped, res.n_error, req.n_walkers),
         ))
         del geom
-    return results
+    return split_failed(results, failed)
 
 
 def library_requests(n: int, seed: int = 2, walkers=(200_000, 1_000_000), cells=(50, 4000), kio=(1.0, 130.0),

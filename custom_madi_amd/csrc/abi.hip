@@ -959,6 +959,7 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
     if ((rc = zero_i64((long long*)d_occupancy, W.n_grid, s))) return rc;
     if ((rc = zero_i64((long long*)d_flags, MADI_N_FLAGS, s))) return rc;
+    if (d_counters && (rc = zero_i64((long long*)d_counters, MADI_N_COUNTERS, s))) return rc;
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
     const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;

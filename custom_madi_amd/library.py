@@ -264,8 +264,11 @@ def _entry_from_result(kio, rho, V, result, columns, cfg, entry_weights):
             "definition": "MADI I Eq. 5 using SI §S.IV untrimmed governing-process arithmetic <A/V>; "
                           "no residence-time proxy is used.",
         },
-        "boundary": {"mode": cfg.boundary_mode, "n_escaped": int(result["n_escaped"]),
-                     "n_walkers_total": int(result["n_eff"] // 3), "fatal_on_escape": True,
+        # a periodic box (this build's synthetic torus; the reference has no
+        # periodic mode) has no exit, so nothing can escape and be fatal
+        "boundary": {"mode": "periodic_torus" if cfg.periodic else cfg.boundary_mode,
+                     "n_escaped": int(result["n_escaped"]),
+                     "n_walkers_total": int(result["n_eff"] // 3), "fatal_on_escape": not cfg.periodic,
                      "occupancy_fraction": np.asarray(result["occupancy_fraction"]).tolist()},
         "weight": None if weight is None else float(weight),
         "imaginary_signal_check": check,

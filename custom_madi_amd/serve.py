@@ -53,7 +53,7 @@ class LibraryStream:
             "signal": (res.cos / res.n_walkers).ravel().tolist(),
             "signal_imag": (res.sin / res.n_walkers).ravel().tolist(), "shape": list(spec_shape),
             "occupancy_fraction": (res.occupancy / res.n_walkers).tolist(),
-            "n_escaped": res.n_escaped, "latency_s": res.latency_s,
+            "n_escaped": res.n_escaped, "n_error": res.n_error, "latency_s": res.latency_s,
         }
         self.fh.write(json.dumps(rec) + "\n")
         self.fh.flush()
@@ -75,6 +75,7 @@ class Server:
         self.preparing: deque = deque()   # AsyncResult or PreparedEntry
         self.ready: deque = deque()       # prepared, waiting for HBM
         self.results: List[EntryResult] = []
+        self.failed: List[EntryResult] = []  # entries whose walk failed: not in results, not in the stream
         self.max_prep = max(1, int(max_prep_inflight))
         self.rejected = 0
         self.seq = 0
@@ -118,9 +119,16 @@ class Server:
             self.ready.popleft()
         self.sch.launch_generation()
         done = self.sch.poll(block=block)
+        good = []
         for r in done:
+            if r.failed:
+                # the reference aborts such an entry (fatal escape / classifier
+                # failure); here it fails alone and is never written
+                self.failed.append(r)
+                continue
             self.stream.append(r, self.sch.spec.shape)
-        self.results += done
+            good.append(r)
+        self.results += good
         return done
 
     def idle(self) -> bool:
@@ -181,7 +189,7 @@ def serve_poisson(server: Server, requests: List[EntryRequest], arrivals: np.nda
     wall = time.perf_counter() - t0
     lat = np.array([r.latency_s for r in server.results])
     return {
-        "requests": len(requests), "finished": len(server.results), "wall_s": wall,
+        "requests": len(requests), "finished": len(server.results), "failed": len(server.failed), "wall_s": wall,
         "entries_per_hour": len(server.results) / wall * 3600.0,
         "latency_s_p50": float(np.percentile(lat, 50)) if len(lat) else None,
         "latency_s_p99": float(np.percentile(lat, 99)) if len(lat) else None,
