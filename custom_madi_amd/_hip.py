@@ -114,6 +114,9 @@ SIGNATURES = {
     "madi_walk_fused_workspace": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_walk_fused": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, C.c_int64,
                                   _p, _p, _p, _p, _p, _p]),
+    "madi_walk_fused_static_max_bytes": (C.c_int, [_i64p]),
+    "madi_walk_fused_static": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, _p, _p,
+                                         _p, _p, _p]),
     "madi_moments_bytes": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_batch_workspace": (C.c_int, [C.c_int32, _i64p]),
     "madi_batch_launch": (C.c_int, [C.POINTER(Work), C.c_int32, C.POINTER(PhasePlan), C.POINTER(Tune), _p, C.c_int64,

@@ -93,6 +93,11 @@ static HostState mat_apply(const Gf2Col* M, HostState v)
     return HostState{lo, hi};
 }
 
+// The jump matrices live in a file-scope device array (each GPU holds its
+// own copy), filled once per device: no entry point allocates device memory
+// for them.
+static __device__ __align__(256) Gf2Col jump_table[kJumpPowers * 128];
+
 struct DeviceTables {
     Gf2Col* jump_pow = nullptr;
 };
@@ -131,7 +136,7 @@ static int get_jump_pow(const Gf2Col** out)
             return MADI_ERR_HIP;
         }
         Gf2Col* d = nullptr;
-        HIP_TRY(hipMalloc(&d, P.size() * sizeof(Gf2Col)));
+        HIP_TRY(hipGetSymbolAddress((void**)&d, HIP_SYMBOL(jump_table)));
         HIP_TRY(hipMemcpy(d, P.data(), P.size() * sizeof(Gf2Col), hipMemcpyHostToDevice));
         g_tables[dev].jump_pow = d;
     }
@@ -719,6 +724,153 @@ static int launch_batch(const WorkDev* table, unsigned long long* item_next, Bat
 }
 
 
+
+// Scratch of madi_walk_fused_static: a file-scope device array, so that
+// entry point allocates nothing (each GPU holds its own copy until the
+// process ends).  Host code never passes the array's name to a kernel: its
+// per-device address comes from hipGetSymbolAddress (static_scratch).
+constexpr int64_t kStaticScratchBytes = 128ll << 20;
+static __device__ __align__(256) char scratch[kStaticScratchBytes];
+
+// Address of this device's scratch, looked up once per device with the
+// device current.
+static int static_scratch(char** out)
+{
+    static std::mutex mu;
+    static char* ptrs[64] = {};
+    int dev = 0;
+    HIP_TRY(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 64) {
+        set_error("device index %d out of range", dev);
+        return MADI_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptrs[dev]) {
+        char* ptr = nullptr;
+        HIP_TRY(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        ptrs[dev] = ptr;
+    }
+    *out = ptrs[dev];
+    return MADI_OK;
+}
+
+// Largest staging image of a one-item fused walk (batch header, one work
+// item, its walker counter), kept on the host stack.
+constexpr int64_t kFusedImgBytes = 4096;
+
+// The fused walk of one entry with its workspace at `base` (workspace_bytes
+// long): argument checks, plan and table staging, launches.  Allocates no
+// device or host memory.
+static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
+                      char* base, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
+                      int64_t* d_flags, int64_t* d_counters, hipStream_t s, const char* who)
+{
+    WalkDev W;
+    int rc = make_walk(p, W);
+    if (rc) return rc;
+    if (!g || !plan || !d_cos || !d_sin || !d_occupancy || !d_flags || !base) {
+        set_error("%s: invalid arguments", who);
+        return MADI_ERR_ARG;
+    }
+    if (precision != 32 && precision != 64) {
+        set_error("%s: precision must be 32 or 64", who);
+        return MADI_ERR_ARG;
+    }
+    if (precision == 32 && !graph_ready(g->dev)) {
+        set_error("%s: the FP32 walk needs the geometry's Delaunay graph", who);
+        return MADI_ERR_ARG;
+    }
+    if (precision == 64 && !kd_ready(g->dev)) {
+        set_error("%s: the FP64 walk needs the geometry's KD tree", who);
+        return MADI_ERR_ARG;
+    }
+    if (plan->n_pairs < 0 || plan->n_pairs > MADI_MAX_PAIRS) {
+        set_error("%s: invalid phase plan", who);
+        return MADI_ERR_ARG;
+    }
+    const FusedLayout lay = fused_layout(W.n_walkers, plan);
+    if (workspace_bytes < lay.total) {
+        set_error("%s: workspace of %lld bytes is smaller than the %lld required", who, (long long)workspace_bytes,
+                  (long long)lay.total);
+        return MADI_ERR_ARG;
+    }
+    if (lay.moments - lay.hdr > kFusedImgBytes) {
+        set_error("%s: work table image exceeds %lld bytes", who, (long long)kFusedImgBytes);
+        return MADI_ERR_ARG;
+    }
+    PlanDev P;
+    Events ev;
+    if ((rc = make_plan(plan, W.n_grid, P, ev))) return rc;
+    PlanDev* dplan = (PlanDev*)base;
+    double* partial = (double*)(base + lay.partial);
+    HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
+    if ((rc = zero_i64((long long*)d_occupancy, W.n_grid, s))) return rc;
+    if ((rc = zero_i64((long long*)d_flags, MADI_N_FLAGS, s))) return rc;
+    if (d_counters && (rc = zero_i64((long long*)d_counters, MADI_N_COUNTERS, s))) return rc;
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) return rc;
+    const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
+    if (tiles == 0) {
+        // no walkers: every sum is +0.0 (all bits zero)
+        if ((rc = zero_i64((long long*)d_cos, P.n_cols, s))) return rc;
+        return zero_i64((long long*)d_sin, P.n_cols, s);
+    }
+    const int variant = variant_of(p->classify_mode);
+    if (precision == 32 && streamed(variant)) {
+        // one work item covering the entry
+        madi_work item;
+        memset(&item, 0, sizeof item);
+        item.geom = g;
+        item.walk = *p;
+        item.tile_lo = 0;
+        item.n_tiles = tiles;
+        item.d_partial = partial;
+        item.d_occupancy = d_occupancy;
+        item.d_flags = d_flags;
+        item.d_moments = base + lay.moments;
+        alignas(256) char img[kFusedImgBytes];
+        const int64_t img_bytes = lay.moments - lay.hdr;
+        memset(img, 0, (size_t)img_bytes);
+        WorkDev* table = (WorkDev*)(img + (lay.table - lay.hdr));
+        int64_t total = 0, walkers = 0;
+        if ((rc = batch_table(&item, 1, plan, table, total, who, true, &walkers))) return rc;
+        BatchHdr* h = (BatchHdr*)img;
+        h->queue = 0;
+        h->total = total;
+        h->n_work = 1;
+        h->total_walkers = walkers;
+        // the copy zeroes the item's walker counter too (lay.next lies inside the image)
+        HIP_TRY(hipMemcpyAsync(base + lay.hdr, img, (size_t)img_bytes, hipMemcpyHostToDevice, s));
+        madi_tune tn;
+        memset(&tn, 0, sizeof tn);
+        tn.scan_rounds = rounds_of(p->scan_rounds, variant);
+        tn.classify_mode = variant == kVariantSplit ? 4 : 3;  // the resolved variant
+        tn.tier_cap_um = p->tier_cap_um;
+        BatchKnobs k;
+        if ((rc = batch_knobs(&tn, p, k))) return rc;
+        k.exits = table_exits(table, 1);
+        const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
+        rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next),
+                          (BatchHdr*)(base + lay.hdr), dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
+        if (rc) return rc;
+        return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
+    }
+    FusedOut fo;
+    fo.plan = dplan;
+    fo.ev = ev;
+    fo.partial = partial;
+    fo.tile_base = 0;
+    fo.occupancy = (long long*)d_occupancy;
+    fo.flags = (long long*)d_flags;
+    if (precision == 32)
+        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds, variant),
+                                   (unsigned long long*)d_counters, s);
+    else
+        rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
+    if (rc) return rc;
+    return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
+}
+
 }  // namespace
 
 extern "C" {
@@ -922,106 +1074,33 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
                     void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
                     int64_t* d_flags, int64_t* d_counters, void* stream)
 {
-    WalkDev W;
-    int rc = make_walk(p, W);
-    if (rc) return rc;
-    if (!g || !d_cos || !d_sin || !d_occupancy || !d_flags || !d_workspace) {
+    if (!d_workspace) {
         set_error("madi_walk_fused: invalid arguments");
         return MADI_ERR_ARG;
     }
-    if (precision != 32 && precision != 64) {
-        set_error("madi_walk_fused: precision must be 32 or 64");
+    return fused_walk(g, p, plan, precision, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
+                      d_counters, (hipStream_t)stream, "madi_walk_fused");
+}
+
+int madi_walk_fused_static_max_bytes(int64_t* bytes)
+{
+    if (!bytes) {
+        set_error("madi_walk_fused_static_max_bytes: bytes is NULL");
         return MADI_ERR_ARG;
     }
-    if (precision == 32 && !graph_ready(g->dev)) {
-        set_error("madi_walk_fused: the FP32 walk needs the geometry's Delaunay graph");
-        return MADI_ERR_ARG;
-    }
-    if (precision == 64 && !kd_ready(g->dev)) {
-        set_error("madi_walk_fused: the FP64 walk needs the geometry's KD tree");
-        return MADI_ERR_ARG;
-    }
-    int64_t need = 0;
-    if ((rc = madi_walk_fused_workspace(W.n_walkers, plan, &need))) return rc;
-    if (workspace_bytes < need) {
-        set_error("madi_walk_fused: workspace of %lld bytes is smaller than the %lld required",
-                  (long long)workspace_bytes, (long long)need);
-        return MADI_ERR_ARG;
-    }
-    PlanDev P;
-    Events ev;
-    if ((rc = make_plan(plan, W.n_grid, P, ev))) return rc;
-    hipStream_t s = (hipStream_t)stream;
-    const FusedLayout lay = fused_layout(W.n_walkers, plan);
-    char* base = (char*)d_workspace;
-    PlanDev* dplan = (PlanDev*)base;
-    double* partial = (double*)(base + lay.partial);
-    HIP_TRY(hipMemcpyAsync(dplan, &P, sizeof P, hipMemcpyHostToDevice, s));
-    if ((rc = zero_i64((long long*)d_occupancy, W.n_grid, s))) return rc;
-    if ((rc = zero_i64((long long*)d_flags, MADI_N_FLAGS, s))) return rc;
-    if (d_counters && (rc = zero_i64((long long*)d_counters, MADI_N_COUNTERS, s))) return rc;
-    const Gf2Col* jp = nullptr;
-    if ((rc = get_jump_pow(&jp))) return rc;
-    const int64_t tiles = (W.n_walkers + kBlock - 1) / kBlock;
-    const int variant = variant_of(p->classify_mode);
-    if (precision == 32 && streamed(variant) && tiles > 0) {
-        // one work item covering the entry
-        madi_work item;
-        memset(&item, 0, sizeof item);
-        item.geom = g;
-        item.walk = *p;
-        item.tile_lo = 0;
-        item.n_tiles = tiles;
-        item.d_partial = partial;
-        item.d_occupancy = d_occupancy;
-        item.d_flags = d_flags;
-        item.d_moments = base + lay.moments;
-        std::vector<char> img((size_t)(lay.moments - lay.hdr), 0);
-        int64_t total = 0, walkers = 0;
-        if ((rc = batch_table(&item, 1, plan, (WorkDev*)(img.data() + (lay.table - lay.hdr)), total, "madi_walk_fused",
-                              true, &walkers)))
-            return rc;
-        BatchHdr* h = (BatchHdr*)img.data();
-        h->queue = 0;
-        h->total = total;
-        h->n_work = 1;
-        h->total_walkers = walkers;
-        HIP_TRY(hipMemcpyAsync(base + lay.hdr, img.data(), img.size(), hipMemcpyHostToDevice, s));
-        madi_tune tn;
-        memset(&tn, 0, sizeof tn);
-        tn.scan_rounds = rounds_of(p->scan_rounds, variant);
-        tn.classify_mode = variant == kVariantSplit ? 4 : 3;  // the resolved variant
-        tn.tier_cap_um = p->tier_cap_um;
-        BatchKnobs k;
-        if ((rc = batch_knobs(&tn, p, k))) return rc;
-        k.exits = table_exits((const WorkDev*)(img.data() + (lay.table - lay.hdr)), 1);
-        const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
-        rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next), (BatchHdr*)(base + lay.hdr),
-                          dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
-        if (rc) return rc;
-        return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
-    }
-    FusedOut fo;
-    fo.plan = dplan;
-    fo.ev = ev;
-    fo.partial = partial;
-    fo.tile_base = 0;
-    fo.occupancy = (long long*)d_occupancy;
-    fo.flags = (long long*)d_flags;
-    if (precision == 32)
-        rc = launch_walk_f32_fused(g->dev, W, jp, fo, cap_width_of(p->tier_cap_um), rounds_of(p->scan_rounds, variant),
-                                   (unsigned long long*)d_counters, s);
-    else
-        rc = launch_walk_f64_fused(g->dev, W, jp, fo, (unsigned long long*)d_counters, s);
+    *bytes = kStaticScratchBytes;
+    return MADI_OK;
+}
+
+int madi_walk_fused_static(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,
+                           int32_t precision, double* d_cos, double* d_sin, int64_t* d_occupancy, int64_t* d_flags,
+                           int64_t* d_counters, void* stream)
+{
+    char* base = nullptr;
+    int rc = static_scratch(&base);
     if (rc) return rc;
-    if (tiles == 0) {
-        std::vector<double> z(P.n_cols, 0.0);
-        HIP_TRY(hipMemcpyAsync(d_cos, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(d_sin, z.data(), z.size() * sizeof(double), hipMemcpyHostToDevice, s));
-        HIP_TRY(hipStreamSynchronize(s));
-        return MADI_OK;
-    }
-    return launch_finalize_tiles(partial, 0, tiles, P.n_cols, d_cos, d_sin, (double*)(base + lay.final), s);
+    return fused_walk(g, p, plan, precision, base, kStaticScratchBytes, d_cos, d_sin, d_occupancy, d_flags,
+                      d_counters, (hipStream_t)stream, "madi_walk_fused_static");
 }
 
 }  // extern "C"

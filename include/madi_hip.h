@@ -238,6 +238,23 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
                     void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
                     int64_t* d_flags, int64_t* d_counters, void* stream);
 
+/* madi_walk_fused without a workspace argument, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  No counterpart in the reference.  The
+ * scratch is a file-scope device array of madi_walk_fused_static_max_bytes
+ * bytes (128 MiB; each GPU the library runs on holds it until the process
+ * ends); an entry whose madi_walk_fused_workspace exceeds it is rejected with
+ * MADI_ERR_ARG.  The call allocates nothing (no hipMalloc, hipFree or
+ * stream-ordered allocation) and gives the same results as madi_walk_fused,
+ * bit for bit.  Every call on a GPU shares that GPU's scratch and the scratch
+ * keeps the last call's contents (each call clears what must start at zero):
+ * calls that overlap on one GPU race, so serialise them (one stream, or
+ * events). */
+int madi_walk_fused_static_max_bytes(int64_t* bytes);
+int madi_walk_fused_static(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,
+                           int32_t precision, double* d_cos, double* d_sin, int64_t* d_occupancy, int64_t* d_flags,
+                           int64_t* d_counters, void* stream);
+
 /* ------------------------------------------------------------- batching */
 
 #define MADI_TILE_WALKERS 256
