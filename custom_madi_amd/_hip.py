@@ -30,6 +30,9 @@ MADI_MAX_B = 16
 MADI_MAX_DIRS = 64
 N_COUNTERS = 13
 N_FLAGS = 2
+N_VERIFY = 7
+VERIFY_NAMES = ("tier_checked", "tier_mismatch", "exit_checked", "exit_mismatch", "classifier_checked",
+                "classifier_mismatch", "exact_failed")
 COUNTER_NAMES = (
     "classify", "safe_hit", "graph_iters", "fp64_recheck", "facets", "kd_nodes", "crossings", "rejects",
     "wave_iters", "lane_iters", "class_iters", "class_busy", "exits",
@@ -114,6 +117,8 @@ SIGNATURES = {
     "madi_walk_fused_workspace": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_walk_fused": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, C.c_int64,
                                   _p, _p, _p, _p, _p, _p]),
+    "madi_walk_fused_verify": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), _p, C.c_int64, _p, _p,
+                                         _p, _p, _p, _p, _p]),
     "madi_walk_fused_static_max_bytes": (C.c_int, [_i64p]),
     "madi_walk_fused_static": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, _p, _p,
                                          _p, _p, _p]),

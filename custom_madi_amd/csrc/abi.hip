@@ -585,6 +585,7 @@ struct BatchKnobs {
     float cap_width;
     int rounds, variant, grid, emit_grid;
     bool exits;  // compile the exit certificate into the streamed walks
+    unsigned long long* verify = nullptr;  // split walk: the verification build's counters
 };
 
 // Whether any item of a batch table can use the exit certificate.
@@ -716,7 +717,7 @@ static int launch_batch(const WorkDev* table, unsigned long long* item_next, Bat
         return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
     int rc = k.variant == kVariantSplit
                  ? launch_split_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, k.exits, grid, counters,
-                                    s)
+                                    k.verify, s)
                  : launch_stream_f32(table, item_next, hdr, plan, ev, jp, k.cap_width, k.rounds, k.exits, grid,
                                      counters, s);
     if (rc) return rc;
@@ -763,7 +764,8 @@ constexpr int64_t kFusedImgBytes = 4096;
 // device or host memory.
 static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
                       char* base, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
-                      int64_t* d_flags, int64_t* d_counters, hipStream_t s, const char* who)
+                      int64_t* d_flags, int64_t* d_counters, hipStream_t s, const char* who,
+                      int64_t* d_verify = nullptr)
 {
     WalkDev W;
     int rc = make_walk(p, W);
@@ -816,6 +818,17 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         return zero_i64((long long*)d_sin, P.n_cols, s);
     }
     const int variant = variant_of(p->classify_mode);
+    if (d_verify) {
+        if (precision != 32 || variant != kVariantSplit) {
+            set_error("%s: the verification build is the FP32 split walk (precision 32, classify_mode 0 or 4)", who);
+            return MADI_ERR_ARG;
+        }
+        if (!kd_ready(g->dev) || !graph_ready(g->dev)) {
+            set_error("%s: verification needs the geometry's KD tree and Delaunay graph", who);
+            return MADI_ERR_ARG;
+        }
+        if ((rc = zero_i64((long long*)d_verify, MADI_N_VERIFY, s))) return rc;
+    }
     if (precision == 32 && streamed(variant)) {
         // one work item covering the entry
         madi_work item;
@@ -849,6 +862,7 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         BatchKnobs k;
         if ((rc = batch_knobs(&tn, p, k))) return rc;
         k.exits = table_exits(table, 1);
+        k.verify = (unsigned long long*)d_verify;
         const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
         rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next),
                           (BatchHdr*)(base + lay.hdr), dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
@@ -1080,6 +1094,19 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     }
     return fused_walk(g, p, plan, precision, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
                       d_counters, (hipStream_t)stream, "madi_walk_fused");
+}
+
+int madi_walk_fused_verify(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,
+                           void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin,
+                           int64_t* d_occupancy, int64_t* d_flags, int64_t* d_counters, int64_t* d_verify,
+                           void* stream)
+{
+    if (!d_workspace || !d_verify) {
+        set_error("madi_walk_fused_verify: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    return fused_walk(g, p, plan, 32, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
+                      d_counters, (hipStream_t)stream, "madi_walk_fused_verify", d_verify);
 }
 
 int madi_walk_fused_static_max_bytes(int64_t* bytes)

@@ -109,9 +109,11 @@ int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const 
                       const Events& ev, const Gf2Col* jump_pow, float cap_width, int rounds, bool exits, int32_t grid,
                       unsigned long long* counters, hipStream_t s);
 // shape 1: 3 walker waves + 1 classifier wave per workgroup; 2: 2 + 2
+// verify (MADI_N_VERIFY counters, or NULL): the verification build, which
+// checks every label against the exact FP64 KD classifier
 int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, bool exits, int32_t grid,
-                     unsigned long long* counters, hipStream_t s);
+                     unsigned long long* counters, unsigned long long* verify, hipStream_t s);
 int launch_emit_tiles(const WorkDev* work, const BatchHdr* hdr, const PlanDev* plan, int n_pairs, int32_t grid,
                       hipStream_t s);
 // Column sums of an entry's tile slots (see k_finalize_chunks for the order);

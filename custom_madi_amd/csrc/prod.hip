@@ -35,6 +35,13 @@
 #include "rng.hpp"
 #include "launch.hpp"
 
+// The exact FP64 KD classifier, for the verification build of the split
+// walk: compiled without FP contraction, so its arithmetic keeps the
+// operation order of madi_classify's (parity.hip is built that way too).
+#pragma clang fp contract(off)
+#include "kd_exact.hpp"
+#pragma clang fp contract(fast)
+
 namespace madi {
 
 constexpr float kUlp = 4.8e-7f;  // 8 * 2^-24: FP32 margin error per unit of |p|_1 + h
@@ -1537,14 +1544,48 @@ __device__ __forceinline__ void lds_store_release(int* p, int v)
     __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Verification build (VERIFY): the label the walk takes for a proposal,
+// from the plane tier (kind 0), the exit certificate (1) or the classifier
+// wave (2), against the exact FP64 KD classifier of madi_classify on the same
+// represented point: the anchor's seed plus the FP32 offset q, summed in
+// FP64.  A tier or classifier label (cell, inside) agrees when the inside
+// flags are equal and, for an inside label, the cells are; a certified exit
+// claims only that q is not inside the walker's cell a.  Counts per kind:
+// checked, disagreeing; plus exact-classifier failures.
+constexpr int kVerifyKinds = 3;
+
+__device__ __forceinline__ void verify_label(const GeomDev& g, const KdView& v, int anchor, float qx, float qy,
+                                             float qz, int kind, int cell, int inside, int a, uint32_t (&checked)[3],
+                                             uint32_t (&bad)[3], uint32_t& failed)
+{
+    const double* c = g.cell_xyz + 3 * anchor;
+    const double wx = gload(c) + (double)qx, wy = gload(c + 1) + (double)qy, wz = gload(c + 2) + (double)qz;
+    int ecell = 0, ein = 0;
+    long long nodes = 0;
+    const int ok = classify_point_f64(g, v, wx, wy, wz, ecell, ein, nodes);
+    const bool agree =
+        kind == 1 ? !(ein && ecell == a) : (ein == inside && (!inside || ecell == cell));
+    failed += !ok;
+#pragma unroll
+    for (int k = 0; k < kVerifyKinds; ++k)
+        if (k == kind) {
+            ++checked[k];
+            bad[k] += ok && !agree;
+        }
+}
+
 // One walker wave (wv < WW): the streamed walk's walker life cycle with the
-// classification replaced by a request to the classifier wave.
-template <int NP, int WW, int CW, bool EXIT>
+// classification replaced by a request to the classifier wave.  VERIFY
+// checks every label against the exact classifier (verify_label) and adds
+// the counts to `verify` (MADI_N_VERIFY).
+template <int NP, int WW, int CW, bool EXIT, bool VERIFY>
 __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, unsigned long long* item_next, const BatchHdr* hdr,
                              const PlanDev& plan, const Gf2Col* __restrict__ jump_pow, const Events& ev,
-                             unsigned long long* counters, SplitLds<NP, WW, CW>& L, int wv)
+                             unsigned long long* counters, unsigned long long* verify, SplitLds<NP, WW, CW>& L,
+                             int wv)
 {
     enum : int { kStep = 0, kWait = 1, kDone = 3, kIdle = 4 };
+    uint32_t v_checked[kVerifyKinds] = {0, 0, 0}, v_bad[kVerifyKinds] = {0, 0, 0}, v_failed = 0;
     const int lane = threadIdx.x & 63;
     const int32_t n_work = hdr->n_work;
     const int64_t total_w = hdr->total_walkers;
@@ -1580,6 +1621,12 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
             dry = 0;
             const GeomDev& g = W->g;
             const WalkDev& p = W->w;
+            KdView kdv;  // the exact classifier reads every node from memory
+            kdv.gf = g.kd_f;
+            kdv.gi = g.kd_i;
+            kdv.sf = nullptr;
+            kdv.si = nullptr;
+            kdv.n_lds = 0;
             // the classifier reads this wave's entry from LDS; no request of
             // this wave is outstanding while it changes
             if (lane == 0) L.geo[wv] = g;
@@ -1676,7 +1723,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                 if (__ballot(phase != kIdle) == 0) break;
                 ++wave_iters;
                 lane_iters += phase != kIdle;
-                int complete = 0, fromq = 0, rflags = 0, rna = 0;
+                int complete = 0, fromq = 0, rflags = 0, rna = 0, vkind = 2;
                 if (phase == kStep) {
                     const float dx = normal_scaled_f32(rng, nscale);
                     const float dy = normal_scaled_f32(rng, nscale);
@@ -1702,12 +1749,14 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                             }
                     } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
                         ++n_tier;
+                        vkind = 0;
                         complete = 1;
                         rna = pcell;
                         rflags = pin;
                     } else if (EXIT && leave > 0.f && in && cell == a && rj == 3 && tier_exit(t, a, qx, qy, qz, leave)) {
                         ++n_tier;  // a certified rejected exit (see stream_wave)
                         ++n_exit;
+                        vkind = 1;
                         complete = 1;
                         rna = a;
                         rflags = 0;
@@ -1765,6 +1814,11 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         phase = kStep;
                     }
                 }
+                // a tier label or a certified exit is for q relative to a; a
+                // classifier answer moved q to its anchor rna
+                if (VERIFY && complete)
+                    verify_label(g, kdv, vkind == 2 ? rna : a, qx, qy, qz, vkind, rna, rflags & 1, a, v_checked,
+                                 v_bad, v_failed);
                 if (complete && init) {
                     // the walker's first label: no step, no acceptance test
                     t = my_resp.t;
@@ -1912,6 +1966,18 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
         }
     }
     if (lane == 0) __hip_atomic_fetch_add(&L.done, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (VERIFY && verify) {
+#pragma unroll
+        for (int k = 0; k < kVerifyKinds; ++k) {
+            const double c = wave_sum_f64((double)v_checked[k]), b = wave_sum_f64((double)v_bad[k]);
+            if (lane == 0) {
+                atomicAdd(&verify[2 * k], (unsigned long long)c);
+                atomicAdd(&verify[2 * k + 1], (unsigned long long)b);
+            }
+        }
+        const double f = wave_sum_f64((double)v_failed);
+        if (lane == 0) atomicAdd(&verify[MADI_VERIFY_EXACT_FAILED], (unsigned long long)f);
+    }
     if (counters) {
         const double v0 = wave_sum_f64((double)n_class), v1 = wave_sum_f64((double)n_tier),
                      v5 = wave_sum_f64((double)crossings), v6 = wave_sum_f64((double)rejects),
@@ -2126,11 +2192,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(3))) voi
 
 // Split walk over a batch (see split_walker / split_classifier): WW walker
 // waves and CW classifier waves per workgroup; EXIT as for k_stream_f32.
-template <int NP, int WW, int CW, bool EXIT>
+template <int NP, int WW, int CW, bool EXIT, bool VERIFY>
 __global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(3))) void k_split_f32(
     const WorkDev* __restrict__ work, unsigned long long* __restrict__ item_next, const BatchHdr* __restrict__ hdr,
     const PlanDev* __restrict__ plan, Events ev, const Gf2Col* __restrict__ jump_pow, float cap_width,
-    unsigned long long* counters)
+    unsigned long long* counters, unsigned long long* verify)
 {
     static_assert(kReqSlots >= 64 * ((WW + CW - 1) / CW), "a ring needs a slot per walker lane it serves");
     static_assert(WW <= 4, "a request names its lane in 8 bits");
@@ -2152,7 +2218,7 @@ __global__ __launch_bounds__((WW + CW) * 64) __attribute__((amdgpu_waves_per_eu(
     // the CU's SIMDs
     const int r = (int)((threadIdx.x / 64 + blockIdx.x) % (WW + CW));
     if (r >= WW) split_classifier<NP, WW, CW>(cap_width, counters, lds, r - WW);
-    else split_walker<NP, WW, CW, EXIT>(work, item_next, hdr, *plan, jump_pow, ev, counters, lds, r);
+    else split_walker<NP, WW, CW, EXIT, VERIFY>(work, item_next, hdr, *plan, jump_pow, ev, counters, verify, lds, r);
 }
 
 // Column sums of every tile of a streamed batch from its walkers' stored
@@ -2334,20 +2400,25 @@ int launch_stream_f32(const WorkDev* work, unsigned long long* item_next, const 
 
 int launch_split_f32(const WorkDev* work, unsigned long long* item_next, const BatchHdr* hdr, const PlanDev* plan,
                      const Events& ev, const Gf2Col* jump_pow, float cap_width, int shape, bool exits, int32_t grid,
-                     unsigned long long* counters, hipStream_t s)
+                     unsigned long long* counters, unsigned long long* verify, hipStream_t s)
 {
-#define MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, XV)                                                                  \
-    hipLaunchKernelGGL((k_split_f32<NPV, WWV, CWV, XV>), dim3(grid), dim3((WWV + CWV) * 64), 0, s, work,        \
-                       item_next, hdr, plan, ev, jump_pow, cap_width, counters)
-#define MADI_LAUNCH_SPLIT(NPV, WWV, CWV)                           \
-    do {                                                           \
-        if (exits) MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, true);       \
-        else MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, false);            \
+#define MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, XV, VV)                                                              \
+    hipLaunchKernelGGL((k_split_f32<NPV, WWV, CWV, XV, VV>), dim3(grid), dim3((WWV + CWV) * 64), 0, s, work,    \
+                       item_next, hdr, plan, ev, jump_pow, cap_width, counters, verify)
+#define MADI_LAUNCH_SPLIT(NPV, WWV, CWV)                                 \
+    do {                                                                 \
+        if (exits) MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, true, false);      \
+        else MADI_LAUNCH_SPLIT_X(NPV, WWV, CWV, false, false);           \
     } while (0)
-#define MADI_SPLIT_SHAPE(NPV)                         \
-    do {                                              \
-        if (shape <= 1) MADI_LAUNCH_SPLIT(NPV, 3, 1); \
-        else MADI_LAUNCH_SPLIT(NPV, 2, 2);            \
+    // the verification build exists in the three-walker shape only (results
+    // are the same bits in every shape)
+#define MADI_SPLIT_SHAPE(NPV)                                                 \
+    do {                                                                      \
+        if (verify) {                                                         \
+            if (exits) MADI_LAUNCH_SPLIT_X(NPV, 3, 1, true, true);            \
+            else MADI_LAUNCH_SPLIT_X(NPV, 3, 1, false, true);                 \
+        } else if (shape <= 1) MADI_LAUNCH_SPLIT(NPV, 3, 1);                  \
+        else MADI_LAUNCH_SPLIT(NPV, 2, 2);                                    \
     } while (0)
     const int np = ev.n_pairs;
     if (np <= 2) MADI_SPLIT_SHAPE(2);

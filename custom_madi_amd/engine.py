@@ -241,6 +241,26 @@ class FusedRunner:
         )
 
 
+def walk_fused_verify(geom: DeviceGeometry, params: _hip.WalkParams, spec: PhaseSpec):
+    """The FP32 split walk's verification build: (FusedResult, verify counts).
+
+    Every label the walk takes (plane tier, certified exit, classifier wave)
+    is checked against the exact FP64 KD classifier on the same represented
+    point; the counts name how many of each kind were checked and how many
+    disagree.  The geometry needs its KD tree and its Delaunay graph."""
+    torch = _hip.require_gpu()
+    runner = FusedRunner(int(params.n_walkers), spec, int(params.n_grid))
+    ver = torch.empty(_hip.N_VERIFY, dtype=torch.int64, device="cuda")
+    lib = runner.lib
+    rc = lib.madi_walk_fused_verify(
+        geom.handle, C.byref(params), C.byref(runner.plan), _hip.dptr(runner.ws), runner.ws_bytes,
+        _hip.dptr(runner.cos), _hip.dptr(runner.sin), _hip.dptr(runner.occ), _hip.dptr(runner.flags),
+        _hip.dptr(runner.cnt), _hip.dptr(ver), _hip.stream_ptr())
+    _hip.check(rc, "madi_walk_fused_verify")
+    res = runner.result(int(params.n_walkers))
+    return res, dict(zip(_hip.VERIFY_NAMES, ver.cpu().numpy().tolist()))
+
+
 def walk_fused(geom: DeviceGeometry, params: _hip.WalkParams, spec: PhaseSpec, precision: int = 32) -> FusedResult:
     runner = FusedRunner(int(params.n_walkers), spec, int(params.n_grid))
     runner.launch(geom, params, precision)

@@ -17,7 +17,7 @@ import multiprocessing as mp
 import time
 from collections import deque
 from pathlib import Path
-from typing import List, Optional
+from typing import Callable, List, Optional
 
 import numpy as np
 
@@ -67,9 +67,12 @@ class Server:
     """Submit requests; ``step()`` advances preparation, admission, launches and finalisation."""
 
     def __init__(self, scheduler: DeviceScheduler, pool=None, stream: Optional[LibraryStream] = None,
-                 max_prep_inflight: int = 32):
+                 max_prep_inflight: int = 32, prepare: Callable[[EntryRequest], PreparedEntry] = prepare_entry):
         self.sch = scheduler
         self.pool = pool
+        # without a pool, requests are prepared in the calling process by
+        # `prepare` (for example a lookup of geometry built beforehand)
+        self.prepare = prepare
         self.stream = stream or LibraryStream(None)
         self.queued: list = []            # heap of (cells, seq, request) not yet handed to the pool
         self.preparing: deque = deque()   # AsyncResult or PreparedEntry
@@ -83,7 +86,7 @@ class Server:
     def submit(self, req: EntryRequest) -> None:
         req.submitted = time.perf_counter()
         if self.pool is None:
-            self.preparing.append(prepare_entry(req))
+            self.preparing.append(self.prepare(req))
             return
         heapq.heappush(self.queued, (req.n_cells, self.seq, req))
         self.seq += 1

@@ -59,6 +59,17 @@ extern "C" {
 #define MADI_FLAG_CLASSIFIER_ERR 1
 #define MADI_N_FLAGS 2
 
+/* Counters of madi_walk_fused_verify: labels checked and labels that
+ * disagree with the exact classifier, per source of the label. */
+#define MADI_VERIFY_TIER_CHECKED 0        /* labels from the plane tier */
+#define MADI_VERIFY_TIER_MISMATCH 1
+#define MADI_VERIFY_EXIT_CHECKED 2        /* rejected exits certified without a classification */
+#define MADI_VERIFY_EXIT_MISMATCH 3
+#define MADI_VERIFY_CLASSIFIER_CHECKED 4  /* labels from a full FP32 classification (FP64 near a boundary) */
+#define MADI_VERIFY_CLASSIFIER_MISMATCH 5
+#define MADI_VERIFY_EXACT_FAILED 6        /* the exact classifier itself failed (coincident seeds) */
+#define MADI_N_VERIFY 7
+
 int madi_abi_version(void);
 const char* madi_last_error(void);
 
@@ -237,6 +248,23 @@ int madi_moments_bytes(int64_t n_walkers, const madi_phase_plan* plan, int64_t* 
 int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
                     void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
                     int64_t* d_flags, int64_t* d_counters, void* stream);
+
+/* Verification build of the FP32 split walk (precision 32, classify_mode 0
+ * or 4): the same walk as madi_walk_fused, bit for bit, in which every label
+ * the walk takes -- from its plane tier, from an exit certificate, or from its
+ * classifier wave -- is checked against the exact FP64 KD classifier of
+ * madi_classify on the same represented point (the anchor's seed plus the
+ * walker's FP32 offset, summed in FP64).  d_verify receives MADI_N_VERIFY
+ * int64 counts (zeroed first).  The geometry needs both its KD tree and its
+ * Delaunay graph.  Slower than madi_walk_fused (one exact classification per
+ * step).  Replaces the reference's cached-vs-exact classifier equivalence
+ * replay (scripts/validate_exact_classifier_cache.py:73-140) for this
+ * build's tier, which has no bitwise twin: a tier answer keeps the walker's
+ * anchor where a full classification moves it. */
+int madi_walk_fused_verify(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,
+                           void* d_workspace, int64_t workspace_bytes, double* d_cos, double* d_sin,
+                           int64_t* d_occupancy, int64_t* d_flags, int64_t* d_counters, int64_t* d_verify,
+                           void* stream);
 
 /* madi_walk_fused without a workspace argument, for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops,

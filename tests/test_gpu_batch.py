@@ -68,10 +68,17 @@ def test_batched_fp32_matches_fp64_exact_walk(spec):
     ens = create_periodic_ensemble(req.n_cells, req.rho, req.rho * req.V * 1e-6, req.config(), seed=req.seed)
     dg = DeviceGeometry(geometry_arrays(ens))
     p = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed)
-    res64 = engine.walk_fused(dg, p, spec, precision=64)
-    dS = np.abs(res32.cos - res64.cos) / req.n_walkers
-    say(f"batched fp32 vs fp64: max |dS| = {dS.max():.3e}")
+    # the FP64 run adds b = 16000 (4 x 4000): Var cos(phi) = (1 + S(4b)) / 2 - S(b)^2
+    specv = batch.acquisition_spec(n_dirs=6, delta=2.0, Deltas=(5.0, 8.0, 10.0),
+                                   b_values=(0.0, 1000.0, 4000.0, 16000.0))
+    res64 = engine.walk_fused(dg, p, specv, precision=64)
+    from custom_madi_amd import workloads
+    S64, se = workloads.signal_se(res64.cos, req.n_walkers, np.array([0, 1, 2]), np.array([0, 2, 3]))
+    dS = np.abs(res32.cos / req.n_walkers - S64)
+    say(f"batched fp32 vs fp64: max |dS| = {dS.max():.3e}, max |dS| / SE = "
+        f"{(dS / np.maximum(se, 1e-300)).max():.3f}")
     assert dS.max() <= 2e-3
+    assert np.all(dS <= 3.0 * se)
     assert res64.n_escaped == 0
 
 

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from custom_madi_amd import _hip, engine
+from custom_madi_amd import _hip, engine, workloads
 from custom_madi_amd.ensemble import DeviceGeometry, geometry_arrays
 from tests import _geom
 
@@ -162,11 +162,18 @@ def test_reduce_matches_oracle():
     assert np.allclose(sn, osn, rtol=1e-11, atol=1e-9)
 
 
-def _spec(G, sph_ms=1.0, dirs=engine.AXIS_DIRECTIONS):
-    delta, Deltas, bs = 2.0, (5.0, 8.0, 10.0), (0.0, 1000.0, 2500.0, 6000.0)
+def _spec(G, sph_ms=1.0, dirs=engine.AXIS_DIRECTIONS, bs=(0.0, 1000.0, 2500.0, 6000.0)):
+    delta, Deltas = 2.0, (5.0, 8.0, 10.0)
     pc = np.array([[engine.phase_coef(b, delta, D) for b in bs] for D in Deltas])
     return engine.PhaseSpec(
         j_delta=np.array([2, 2, 2]), j_Delta=np.array([5, 8, 10]), j_sum=np.array([7, 10, 12]), pc=pc, dirs=dirs)
+
+
+# _spec's b values followed by the 4b values that give each signal's Monte
+# Carlo variance (Var cos(phi) = (1 + S(4b)) / 2 - S(b)^2, since the phase
+# grows as sqrt(b)); idx / idx4: positions of b and of 4b
+_B_VAR = (0.0, 1000.0, 2500.0, 6000.0, 4000.0, 10000.0, 24000.0)
+_IDX, _IDX4 = np.array([0, 1, 2, 3]), np.array([0, 4, 5, 6])
 
 
 def test_fused_fp64_equals_walk_then_reduce(big_geom):
@@ -200,15 +207,16 @@ def test_fp32_walk_matches_fp64_statistically(which, request):
     cfg = _Cfg(steps, sph, G, sigma, ts)
     spec = _spec(G)
     p = engine.walk_params(n, cfg, pp, 987)
-    f64 = engine.walk_fused(dg, p, spec, precision=64)
+    f64 = engine.walk_fused(dg, p, _spec(G, bs=_B_VAR), precision=64)
     f32 = engine.walk_fused(dg, p, spec, precision=32)
     say(f"fp32 counters: {f32.counters}")
     assert f32.n_escaped == 0 and f32.n_error == 0
-    S64 = f64.cos / n
+    S64, se = workloads.signal_se(f64.cos, n, _IDX, _IDX4)
     S32 = f32.cos / n
     dS = np.abs(S64 - S32)
-    say(f"max |dS| fp32 vs fp64 = {dS.max():.3e}")
+    say(f"max |dS| fp32 vs fp64 = {dS.max():.3e}, max |dS| / SE = {(dS / np.maximum(se, 1e-300)).max():.3f}")
     assert dS.max() <= 2e-3
+    assert np.all(dS <= 3.0 * se)
     occ_rel = np.abs(f32.occupancy - f64.occupancy) / n
     assert occ_rel.max() < 0.01
 
