@@ -370,6 +370,14 @@ def entry_error(n_escaped: int, n_error: int, n_walkers: int) -> Optional[str]:
     return None
 
 
+def failed_entry(req: EntryRequest, error: str, shape, admitted: float = 0.0) -> EntryResult:
+    """Result of an entry that failed before or during its walk (no signal)."""
+    now = time.perf_counter()
+    return EntryResult(key=req.key, cos=np.zeros(shape), sin=np.zeros(shape), occupancy=np.zeros(0, np.int64),
+                       n_walkers=req.n_walkers, n_escaped=0, n_error=0, pp=float("nan"), submitted=req.submitted,
+                       admitted=admitted or now, finished=now, prep_s=0.0, error=error)
+
+
 def split_failed(results: List[EntryResult], failed: Optional[list]) -> List[EntryResult]:
     """Good results; failed ones go to `failed`, or raise RuntimeError (after
     every entry has finished) when the caller gave no list for them."""
@@ -419,7 +427,11 @@ def run_plain(requests: Iterable[EntryRequest], spec: engine.PhaseSpec,
     results = []
     for req in requests:
         t_admit = time.perf_counter()
-        prep = prepare(req)
+        try:
+            prep = prepare(req)
+        except RuntimeError as e:  # a geometry with no defined labels fails its entry alone
+            results.append(failed_entry(req, str(e), spec.shape, t_admit))
+            continue
         geom = DeviceGeometry(prep.arrays)
         params = engine.walk_params(req.n_walkers, req.config(), prep.pp, req.walk_seed, scan_rounds=scan_rounds,
                                     classify_mode=classify_mode)

@@ -581,10 +581,22 @@ def _circumspheres(points: np.ndarray, simplices: np.ndarray) -> Tuple[np.ndarra
 
 
 def delaunay_graph(points: np.ndarray, owned: int) -> Tuple[np.ndarray, np.ndarray, Delaunay]:
-    """CSR Delaunay neighbours (offsets over all points, ids) of a point set."""
+    """CSR Delaunay neighbours (offsets over all points, ids) of a point set.
+
+    A seed that coincides with another (or that the triangulation leaves
+    out) has no neighbours, so no label is defined near it: that is the
+    reference's classifier failure (the exact classifier divides by the
+    seeds' separation, madi/walker_gpu.py:272-335, 938-940), raised here as
+    RuntimeError before any walk starts."""
     tri = Delaunay(points)
     indptr, indices = tri.vertex_neighbor_vertices
-    return np.asarray(indptr, dtype=np.int64), np.asarray(indices, dtype=np.int64), tri
+    indptr = np.asarray(indptr, dtype=np.int64)
+    isolated = np.flatnonzero(np.diff(indptr[: owned + 1]) == 0)
+    if len(isolated):
+        raise RuntimeError(
+            f"GPU full-facet classification failed: {len(isolated)} seed(s) (first: {int(isolated[0])}) coincide "
+            "with another seed or fall out of the Delaunay triangulation, so their cells have no facets")
+    return indptr, np.asarray(indices, dtype=np.int64), tri
 
 
 def _hint_grid(tree: cKDTree, L: float, n_hint: int) -> np.ndarray:

@@ -21,7 +21,7 @@ from typing import Callable, List, Optional
 
 import numpy as np
 
-from .batch import DeviceScheduler, EntryRequest, EntryResult, PreparedEntry, prepare_entry
+from .batch import DeviceScheduler, EntryRequest, EntryResult, PreparedEntry, failed_entry, prepare_entry
 
 
 def _worker_init() -> None:
@@ -75,7 +75,7 @@ class Server:
         self.prepare = prepare
         self.stream = stream or LibraryStream(None)
         self.queued: list = []            # heap of (cells, seq, request) not yet handed to the pool
-        self.preparing: deque = deque()   # AsyncResult or PreparedEntry
+        self.preparing: deque = deque()   # (request, AsyncResult) or PreparedEntry
         self.ready: deque = deque()       # prepared, waiting for HBM
         self.results: List[EntryResult] = []
         self.failed: List[EntryResult] = []  # entries whose walk failed: not in results, not in the stream
@@ -86,7 +86,10 @@ class Server:
     def submit(self, req: EntryRequest) -> None:
         req.submitted = time.perf_counter()
         if self.pool is None:
-            self.preparing.append(self.prepare(req))
+            try:
+                self.preparing.append(self.prepare(req))
+            except RuntimeError as e:
+                self.failed.append(failed_entry(req, str(e), self.sch.spec.shape))
             return
         heapq.heappush(self.queued, (req.n_cells, self.seq, req))
         self.seq += 1
@@ -98,7 +101,7 @@ class Server:
         pool so later, cheaper arrivals can still go ahead."""
         while self.queued and len(self.preparing) < self.max_prep:
             _, _, req = heapq.heappop(self.queued)
-            self.preparing.append(self.pool.apply_async(prepare_entry, (req,)))
+            self.preparing.append((req, self.pool.apply_async(prepare_entry, (req,))))
 
     def _collect_prepared(self) -> None:
         """Move every finished preparation to `ready` (in the order they finish:
@@ -108,10 +111,15 @@ class Server:
             head = self.preparing.popleft()
             if isinstance(head, PreparedEntry):
                 self.ready.append(head)
-            elif head.ready():
-                self.ready.append(head.get())
-            else:
+                continue
+            req, job = head
+            if not job.ready():
                 pending.append(head)
+                continue
+            try:
+                self.ready.append(job.get())
+            except RuntimeError as e:  # the geometry has no defined labels: this entry fails alone
+                self.failed.append(failed_entry(req, str(e), self.sch.spec.shape))
         self.preparing = pending
         if self.pool is not None:
             self._dispatch()

@@ -363,3 +363,38 @@ def test_ctypes_structs_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_coincident_seeds_are_a_classifier_failure_before_the_walk():
+    """A seed on top of another has no Delaunay facets, so no label is defined
+    near it: the reference's classifier failure (madi/walker_gpu.py:938-940),
+    raised when the device geometry is prepared, in periodic and
+    non-periodic boxes alike; the failing entry fails alone in the plain and
+    served forms."""
+    from custom_madi_amd import batch, serve
+    from custom_madi_amd.ensemble import build_kdtree, geometry_arrays
+    for ens in (_geom.periodic(n_cells=64, seed=0), _geom.cellular(L=40.0, seed=2)):
+        ens.seeds[5] = ens.seeds[4]
+        (ens.kd_node_seed, ens.kd_node_axis, ens.kd_node_left, ens.kd_node_right,
+         ens.kd_node_parent) = build_kdtree(ens.seeds)
+        with pytest.raises(RuntimeError, match="classification failed"):
+            geometry_arrays(ens)
+
+    req = batch.library_requests(1, seed=3, walkers=(1000, 1000), cells=(60, 60), T_max_ms=12.0)[0]
+
+    def bad_prepare(r):
+        raise RuntimeError("GPU full-facet classification failed: seeds coincide")
+
+    spec = batch.acquisition_spec(n_dirs=3, delta=2.0, Deltas=(5.0,), b_values=(0.0, 1000.0))
+    failed = []
+    assert batch.run_plain([req], spec, prepare=bad_prepare, failed=failed) == []
+    assert len(failed) == 1 and failed[0].key == req.key and "classification failed" in failed[0].error
+    with pytest.raises(RuntimeError, match="classification failed"):
+        batch.run_plain([req], spec, prepare=bad_prepare)
+
+    class _Sch:  # a scheduler stand-in: the request never reaches a device
+        spec = batch.acquisition_spec(n_dirs=3, delta=2.0, Deltas=(5.0,), b_values=(0.0, 1000.0))
+
+    srv = serve.Server(_Sch(), prepare=bad_prepare)
+    srv.submit(req)
+    assert len(srv.failed) == 1 and not srv.results and not srv.preparing
