@@ -1,20 +1,28 @@
 """Benchmark of the MADI walk on one MI355X.
 
-Default workload (one library entry of the production single-entry config):
-periodic cube of 1,000 contracted Voronoi cells, rho = 5e5 cells/µL,
+Default run: one JSON line whose headline is the production single entry
+(C1): a periodic cube of 1,000 contracted Voronoi cells, rho = 5e5 cells/µL,
 V = 1.2 pL (v_i = 0.6), k_io = 22 s^-1 (p_p by Eq. 5), D = 2.0 µm²/ms,
 dt = 0.02 ms for 2,300 steps (46 ms), 1e6 walkers, PGSE delta = 6 ms,
 Delta = 15/25/30/40 ms, b = 0/1000/2500/4000/6000 s/mm², 24 directions:
 480 signals.  The FP32 production walk with the fused phase reduction is the
 timed path; a "step" is one full entry (1e6 walkers x 2,300 steps), inputs
-resident on the device.
+resident on the device.  The same line carries:
 
-Other workloads (--workload): c2 (256 uneven library entries, batched vs one
-entry per launch), c3 (served Poisson arrivals, p50/p99 latency), c4 (a
-seeded sample of the full-range sweep), map (MAP matching of 1e6 voxels
-against a 14,105-entry library, FP64).
+* ``c2``: 256 uneven library entries, the batched/served form against the
+  plain one-entry-per-launch form (all 256 entries each), entries/h, their
+  ratio, the bitwise count and the batched entries' latency;
+* ``c3``: 60 s of Poisson arrivals at 20 requests/s served with continuous
+  batching, p50/p99 latency, entries/h and HBM in use;
+* ``fp64_check``: the FP32 entry against the FP64 exact-classifier walk on the
+  same seeds, per (Delta, b), with each signal's Monte Carlo standard error;
+* ``cpu_baseline``: the reference's CPU walk algorithm (numpy/scipy
+  restatement) on the host cores, single process and across the worker
+  processes, and the FP64 C oracle with OpenMP.
 
-Prints one JSON line.  --gpus N > 1 exits: this benchmark uses one GPU.
+Other workloads (--workload): c2, c3 on their own, c4 (a seeded sample of the
+full-range sweep), map (MAP matching of 1e6 voxels against a 14,105-entry
+library, FP64).  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
 from __future__ import annotations
@@ -43,35 +51,10 @@ PEAK_FP32_NONPACKED_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
 
 
-def c1_workload(n_walkers=1_000_000):
-    from custom_madi_amd.config import SimConfig
-    from custom_madi_amd.ensemble import create_periodic_ensemble
-    from custom_madi_amd.walker_gpu import kio_to_pp
-    from custom_madi_amd import engine
-
-    rho, V, kio = 5.0e5, 1.2, 22.0
-    vi = rho * V * 1e-6
-    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=46.0, h_ms=1.0, n_walkers=n_walkers, small_deltas=[6.0],
-                    big_deltas=[15.0, 25.0, 30.0, 40.0], b_values=[0.0, 1000.0, 2500.0, 4000.0, 6000.0],
-                    periodic=True, precision="fp32")
-    ens = create_periodic_ensemble(1000, rho, vi, cfg, seed=0)
-    pp = kio_to_pp(kio, ens.mean_AV, cfg)
-    dirs = np.random.default_rng(1).normal(size=(24, 3))
-    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
-    delta = 6.0
-    Deltas = [15.0, 25.0, 30.0, 40.0]
-    pc = np.array([[engine.phase_coef(b, delta, D) for b in cfg.b_values] for D in Deltas])
-    spec = engine.PhaseSpec(j_delta=np.array([6] * 4), j_Delta=np.array([int(D) for D in Deltas]),
-                            j_sum=np.array([int(D + delta) for D in Deltas]), pc=pc, dirs=dirs)
-    return cfg, ens, pp, spec
-
-
-def cpu_baseline(ens, cfg, pp, threads, target_s=15.0, max_walkers=1_000_000):
-    """The CPU restatement of the reference walk (FP64 exact classifier), timed on host cores.
-
-    A 2,048-walker calibration run sizes the timed sample to about target_s
-    seconds (at most the entry's own walker count).
-    """
+def cpu_baseline_c_oracle(ens, cfg, pp, threads, target_s=10.0, max_walkers=1_000_000):
+    """The FP64 C oracle (the reference walk's semantics, exact classifier) with
+    OpenMP on host cores; a 2,048-walker calibration run sizes the timed
+    sample to about target_s seconds.  Returns (walker-steps/s, s, walkers)."""
     from oracle import oracle as O
     from custom_madi_amd.ensemble import geometry_arrays
     O.set_threads(threads)
@@ -87,6 +70,47 @@ def cpu_baseline(ens, cfg, pp, threads, target_s=15.0, max_walkers=1_000_000):
     n = int(min(max_walkers, max(2048, target_s / dt * 2048)))
     dt = run(n)
     return n * cfg.n_steps / dt, dt, n
+
+
+def _reference_cpu_walk(job):
+    """One process's share of the reference CPU walk algorithm: the numpy /
+    scipy restatement of _walk_cpu + classify_cpu (madi/walker_gpu.py:698-840,
+    madi/ensemble.py:213-256) over the image-padded periodic seed set."""
+    from oracle import oracle as O
+    kd_xyz, kd_ann, kd_label, L, n, steps, sph, n_grid, sigma, ts, pp, seed = job
+    t0 = time.perf_counter()
+    O.walk_cpu_reference(kd_xyz, kd_ann, L, 0.0, L, n, steps, sph, n_grid, sigma, ts, pp, seed=seed,
+                         label=kd_label)
+    return time.perf_counter() - t0
+
+
+def cpu_baseline_reference(ens, cfg, pp, pool, processes, target_s, log):
+    """The reference's CPU walk algorithm timed single-process and split by
+    walker ranges over `processes` worker processes (each its own seed).
+    A 64-walker x 230-step calibration sizes each process's walker count for
+    about target_s seconds of the full 2,300-step walk."""
+    from custom_madi_amd.ensemble import geometry_arrays
+    arr = geometry_arrays(ens)
+    base = (arr.kd_xyz, arr.kd_annulus, arr.kd_label, ens.L)
+    tail = (cfg.steps_per_h, cfg.n_grid, cfg.sigma, cfg.ts, pp)
+    cal = _reference_cpu_walk((*base, 64, 230, *tail, 1))
+    n = int(max(16, min(20_000, target_s / (cal * 10.0) * 64)))
+    log(f"reference CPU walk: {n} walkers x {cfg.n_steps} steps per process (calibration {cal:.2f} s)")
+    dt1 = _reference_cpu_walk((*base, n, cfg.n_steps, *tail, 2))
+    single = {"value": n * cfg.n_steps / dt1, "cores": 1, "walkers": n, "wall_s": dt1}
+    t0 = time.perf_counter()
+    pool.map(_reference_cpu_walk, [(*base, n, cfg.n_steps, *tail, 100 + k) for k in range(processes)], chunksize=1)
+    dtn = time.perf_counter() - t0
+    multi = {"value": processes * n * cfg.n_steps / dtn, "cores": processes, "walkers": processes * n,
+             "wall_s": dtn}
+    return single, multi
+
+
+def visible_cpus() -> int:
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
 
 
 def cpu_model() -> str:
@@ -118,17 +142,18 @@ def _tuning(args, spec, log, sample=None, tag=""):
     return choice.kwargs(), src
 
 
-def run_c2(args):
-    """256 uneven library entries: plain one-entry-per-launch form vs the batched form."""
-    from custom_madi_amd import batch, serve
-    reqs = batch.library_requests(args.entries, seed=2)
-    spec = batch.acquisition_spec(24)
-    pool = serve.make_pool(args.workers)           # spawned before the GPU is initialised
-    pool.map(_noop, range(args.workers))
-    from custom_madi_amd import _hip
+def measure_c2(args, pool, log):
+    """256 uneven library entries: plain one-entry-per-launch form vs the batched form.
+
+    Both forms run every entry.  The plain form restates the reference's
+    library loop (madi/library.py:546-629): host geometry, upload, one
+    launch, synchronise, per entry.  The batched form prepares geometry in
+    the worker pool and streams entries through DeviceScheduler."""
+    from custom_madi_amd import _hip, batch, serve, workloads
+    reqs = workloads.c2_requests(args.entries)
+    spec = workloads.acquisition()
     _hip.require_gpu()
     info = _hip.device_info()
-    log = lambda m: print(f"[bench c2] {m}", file=sys.stderr, flush=True)  # noqa: E731
     walker_steps = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs)
     log(f"{len(reqs)} entries, {walker_steps:.3e} walker-steps, device {info['name']} CUs={info['cu_count']}")
 
@@ -161,10 +186,8 @@ def run_c2(args):
     # plain form with host geometry prepared beforehand (device-only comparison)
     preps = pool.map(batch.prepare_entry, reqs[:n_plain])
     t0 = time.perf_counter()
-    res_pg = batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps), **walk_knobs)
+    batch.run_plain([p.req for p in preps], spec, prepare=_prepared_lookup(preps), **walk_knobs)
     t_plain_dev = time.perf_counter() - t0
-    pool.close()
-    pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
 
     same = sum(int(np.array_equal(res_b[r.key].cos, r.cos) and np.array_equal(res_b[r.key].sin, r.sin)
                    and np.array_equal(res_b[r.key].occupancy, r.occupancy)) for r in res_p)
@@ -175,7 +198,6 @@ def run_c2(args):
     lat = [r.latency_s for r in srv.results]
     esc = len(srv.failed)  # failed entries (escape or classifier failure)
     out = {
-        "metric": "library entries/h", "value": eph_b, "unit": "entries/h", "n_gpus": 1, "higher_is_better": True,
         "workload": "c2: 256 uneven entries (k_io log-U 1-130/s, V log-U 0.1-20 pL, v_i U 0.5-0.95, 50-4000 cells, "
                     "2e5-1e6 walkers, dt 0.02 ms x 2300 steps, 480 signals), FP32 walk, FP64 sums",
         "batched": {"entries": len(reqs), "wall_s": t_batch, "entries_per_hour": eph_b,
@@ -192,7 +214,19 @@ def run_c2(args):
         "tuning": {**knobs, "source": knob_src},
         "classifier": sch.counter_dict(),
     }
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def run_c2(args):
+    from custom_madi_amd import serve
+    pool = serve.make_pool(args.workers)           # spawned before the GPU is initialised
+    pool.map(_noop, range(args.workers))
+    log = lambda m: print(f"[bench c2] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    out = measure_c2(args, pool, log)
+    pool.close()
+    pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
+    print(json.dumps({"metric": "library entries/h", "value": out["batched"]["entries_per_hour"],
+                      "unit": "entries/h", "n_gpus": 1, "higher_is_better": True, **out}), flush=True)
     return 0
 
 
@@ -300,38 +334,41 @@ def _prepared_lookup(preps):
     return lambda req: table[req.key]
 
 
-def run_c3(args):
+def measure_c3(args, pool, log, duration):
     """Serving: Poisson arrivals of entry requests, continuous batching, p50/p99 latency."""
-    from custom_madi_amd import batch, serve
-    arrivals = serve.poisson_arrivals(args.rate, args.duration, seed=3)
-    reqs = batch.library_requests(len(arrivals), seed=3, walkers=(100_000, 1_000_000))
-    spec = batch.acquisition_spec(24)
-    pool = serve.make_pool(args.workers)
-    pool.map(_noop, range(args.workers))
-    from custom_madi_amd import _hip
+    from custom_madi_amd import _hip, batch, serve, workloads
+    reqs, arrivals = workloads.c3_requests(args.rate, duration)
+    spec = workloads.acquisition()
     _hip.require_gpu()
-    log = lambda m: print(f"[bench c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
     warm = batch.library_requests(2, seed=99, walkers=(20_000, 20_000), cells=(60, 80))
     batch.run_plain(warm, spec)
     knobs, knob_src = _tuning(args, spec, log)
     sch = batch.DeviceScheduler(spec, tier_cap_um=args.tier_cap, max_tiles_per_generation=args.gen_tiles, **knobs)
     stream = serve.LibraryStream(args.library_out)
     srv = serve.Server(sch, pool=pool, stream=stream)
-    offered = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs) / args.duration
-    log(f"{len(reqs)} requests over {args.duration:.0f} s, offered {offered:.3e} walker-steps/s")
+    offered = sum(r.n_walkers * int(round(r.T_max_ms / r.ts)) for r in reqs) / duration
+    log(f"{len(reqs)} requests over {duration:.0f} s, offered {offered:.3e} walker-steps/s")
     rep = serve.serve_poisson(srv, reqs, arrivals, progress=log)
     sch.close()
     stream.close()
+    hbm = rep.pop("hbm_used_bytes_samples")
+    return {"workload": f"c3: Poisson {args.rate}/s for {duration:.0f} s, entries as c2 with 1e5-1e6 walkers, "
+                        "480 signals", "offered_walker_steps_per_s": offered,
+            "hbm_used_gib_max": max((b for _, b in hbm), default=0) / 2 ** 30,
+            "hbm_used_gib_samples": [[round(t, 1), round(b / 2 ** 30, 2)] for t, b in hbm], **rep,
+            "tuning": {**knobs, "source": knob_src}, "classifier": sch.counter_dict()}
+
+
+def run_c3(args):
+    from custom_madi_amd import serve
+    pool = serve.make_pool(args.workers)
+    pool.map(_noop, range(args.workers))
+    log = lambda m: print(f"[bench c3] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    out = measure_c3(args, pool, log, args.duration)
     pool.close()
     pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
-    hbm = rep.pop("hbm_used_bytes_samples")
-    out = {"metric": "p50/p99 entry latency", "unit": "s", "n_gpus": 1, "higher_is_better": False,
-           "workload": f"c3: Poisson {args.rate}/s for {args.duration:.0f} s, entries as c2 with 1e5-1e6 walkers, "
-                       "480 signals", "offered_walker_steps_per_s": offered,
-           "hbm_used_gib_max": max((b for _, b in hbm), default=0) / 2 ** 30,
-           "hbm_used_gib_samples": [[round(t, 1), round(b / 2 ** 30, 2)] for t, b in hbm], **rep,
-           "tuning": {**knobs, "source": knob_src}, "classifier": sch.counter_dict()}
-    print(json.dumps(out), flush=True)
+    print(json.dumps({"metric": "p50/p99 entry latency", "unit": "s", "n_gpus": 1, "higher_is_better": False,
+                      **out}), flush=True)
     return 0
 
 
@@ -405,7 +442,7 @@ def main(argv=None):
     ap.add_argument("--walkers", type=int, default=1_000_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64-check", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target seconds per CPU baseline sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map"], default="c1")
@@ -425,7 +462,10 @@ def main(argv=None):
     ap.add_argument("--no-tune", action="store_true", help="c2/c3/c4: default knobs instead of the autotuner")
     ap.add_argument("--fp64-entries", type=int, default=16, help="c4: entries checked against the FP64 walk")
     ap.add_argument("--rate", type=float, default=20.0)
-    ap.add_argument("--duration", type=float, default=60.0)
+    ap.add_argument("--duration", type=float, default=60.0, help="c3 workload: seconds of arrivals")
+    ap.add_argument("--c3-duration", type=float, default=60.0, help="default line: seconds of c3 arrivals")
+    ap.add_argument("--no-c2", action="store_true", help="default line without the c2 object")
+    ap.add_argument("--no-c3", action="store_true", help="default line without the c3 object")
     ap.add_argument("--library-out", default=None)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -442,25 +482,67 @@ def main(argv=None):
     if args.workload == "map":
         return run_map(args)
 
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    pool = None
+    if not (args.no_c2 and args.no_c3 and args.no_cpu_baseline):
+        from custom_madi_amd import serve
+        pool = serve.make_pool(args.workers)       # spawned before the GPU is initialised
+        pool.map(_noop, range(args.workers))
+    out, (ens, cfg, pp) = measure_c1(args, log)
+    if not args.no_c2:
+        log("c2: 256 uneven entries, plain and batched")
+        out["c2"] = measure_c2(args, pool, lambda m: log(f"c2: {m}"))
+    if not args.no_c3:
+        log(f"c3: Poisson arrivals at {args.rate}/s for {args.c3_duration:.0f} s")
+        out["c3"] = measure_c3(args, pool, lambda m: log(f"c3: {m}"), args.c3_duration)
+    if not args.no_cpu_baseline:
+        procs, vis = max(1, int(args.workers)), visible_cpus()
+        single, multi = cpu_baseline_reference(ens, cfg, pp, pool, procs, args.cpu_seconds, log)
+        threads = max(1, min(procs, vis))
+        rate_c, dt_c, n_c = cpu_baseline_c_oracle(ens, cfg, pp, threads, target_s=args.cpu_seconds,
+                                                  max_walkers=args.walkers)
+        cpu = cpu_model()
+        out["cpu_baseline"] = {
+            "value": multi["value"], "unit": "walker-steps/s", "cores": multi["cores"], "kind": "port",
+            "sample": f"the reference CPU walk algorithm (numpy/scipy restatement of _walk_cpu + classify_cpu: "
+                      f"cKDTree nearest and ball queries, per-point facet loop; FP64) on the same entry's "
+                      f"image-padded seeds, {multi['cores']} processes x {multi['walkers'] // multi['cores']} "
+                      f"walkers x {cfg.n_steps} steps in {multi['wall_s']:.1f} s on {cpu} ({vis} CPUs visible; "
+                      f"walker-steps/s is linear in walkers, so the sample rate is the entry's)",
+            "single_process": {**single, "unit": "walker-steps/s",
+                               "sample": f"1 process x {single['walkers']} walkers x {cfg.n_steps} steps in "
+                                         f"{single['wall_s']:.1f} s"},
+            "c_oracle": {"value": rate_c, "unit": "walker-steps/s", "cores": threads,
+                         "sample": f"{n_c} walkers x {cfg.n_steps} steps, FP64 C restatement "
+                                   f"(oracle/madi_oracle.c, OpenMP, {threads} threads) in {dt_c:.1f} s"},
+        }
+    if pool is not None:
+        pool.close()
+        pool.join()      # workers exit on their own (no SIGTERM at interpreter exit)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def measure_c1(args, log):
+    """The C1 entry: headline walker-steps/s, roofline and the FP64 check."""
     import torch
-    from custom_madi_amd import _hip, engine
+    from custom_madi_amd import _hip, engine, workloads
     _hip.require_gpu()
     info = _hip.device_info()
-
     if args.copy_probe:
         src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    cfg, ens, pp, spec = c1_workload(args.walkers)
+    cfg, ens, pp, spec = workloads.c1_entry(args.walkers)
     dg = ens.device_geometry()
     N = args.walkers
     runner = engine.FusedRunner(N, spec, cfg.n_grid)
     params = engine.walk_params(N, cfg, pp, seed=0, tier_cap_um=args.tier_cap, scan_rounds=args.scan_rounds,
                                 classify_mode=args.classify_mode)
-    print(f"[bench] device {info['name']} {info['arch']} CUs={info['cu_count']}; workload c1 walkers={N} "
-          f"steps={cfg.n_steps} pp={pp:.6g}", file=sys.stderr, flush=True)
+    log(f"device {info['name']} {info['arch']} CUs={info['cu_count']}; workload c1 walkers={N} "
+        f"steps={cfg.n_steps} pp={pp:.6g}")
 
     for _ in range(args.warmup):
         runner.launch(dg, params, 32)
@@ -485,8 +567,7 @@ def main(argv=None):
         raise RuntimeError(f"walk reported escapes={res.n_escaped} errors={res.n_error}")
 
     # madi_walk_fused zeroes the counters: they hold the last launch's values
-    launches = 1
-    facets = res.counters["facets"] / launches
+    facets = res.counters["facets"]
     flops = walker_steps * F_STEP + facets * F_FACET + N * F_WALKER
     kavg = float(np.mean(kern_ms)) * 1e-3
     achieved = flops / kavg / 1e12
@@ -502,45 +583,34 @@ def main(argv=None):
         "config": {"workload": "c1: periodic 1000-cell box, rho 5e5/uL, V 1.2 pL, k_io 22/s, D 2.0, dt 0.02 ms "
                                "x 2300 steps, 1e6 walkers, 4 Delta x 5 b x 24 dirs = 480 signals",
                    "walkers": N, "steps_per_walk": cfg.n_steps, "cells": 1000, "signals": 480},
-        "entries_per_hour": 3600.0 / (ms_per_step / 1e3),
-        "entry_latency_ms_p50": float(np.percentile(kern_ms, 50)),
-        "entry_latency_ms_p99": float(np.percentile(kern_ms, 99)),
+        "c1_kernel_ms_p50": float(np.percentile(kern_ms, 50)),
+        "c1_kernel_ms_p99": float(np.percentile(kern_ms, 99)),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "frac_nonpacked_peak": achieved / PEAK_FP32_NONPACKED_TFLOPS,
                      "kernel_ms": kavg * 1e3, "algorithmic_tflop_per_launch": flops / 1e12},
-        "classifier": {k: v / launches for k, v in res.counters.items()},
+        "classifier": dict(res.counters),
     }
 
     if not args.no_fp64_check:
+        # the FP64 exact-classifier walk on the same seeds, with the 4b
+        # columns that give each signal's standard error
+        specv, idx, idx4 = workloads.acquisition_with_variance()
         t64 = time.perf_counter()
-        f64 = engine.walk_fused(dg, params, spec, precision=64)
+        f64 = engine.walk_fused(dg, params, specv, precision=64)
         t64 = time.perf_counter() - t64
-        S32 = res.cos / N
-        S64 = f64.cos / N
-        dS = np.abs(S32 - S64)
-        se = np.sqrt(np.maximum(1.0 - S64 ** 2, 0.0) / N)
-        per = dS.max(axis=2)  # (Delta, b)
+        S64, se = workloads.signal_se(f64.cos, N, idx, idx4)
+        dS = np.abs(res.cos / N - S64)
+        ratio = dS / np.maximum(se, 1e-300)
         out["fp64_check"] = {
-            "max_abs_dS": float(dS.max()), "max_abs_dS_over_3se": float((dS / (3 * se + 1e-300)).max()),
-            "per_Delta_b_max_abs_dS": per.tolist(), "fp64_escaped": f64.n_escaped,
+            "max_abs_dS": float(dS.max()), "max_abs_dS_over_se": float(ratio.max()),
+            "signals_within_3se": f"{int(np.count_nonzero(dS <= 3.0 * se))}/{dS.size}",
+            "per_Delta_b_max_abs_dS": dS.max(axis=2).tolist(), "fp64_escaped": f64.n_escaped,
+            "occupancy_max_abs_d": float(np.abs(res.occupancy - f64.occupancy).max() / N),
             "fp64_walk_s": t64, "fp64_kd_nodes": f64.counters["kd_nodes"],
         }
-
-    if not args.no_cpu_baseline:
-        try:
-            cores = len(os.sched_getaffinity(0))
-        except AttributeError:
-            cores = os.cpu_count() or 1
-        threads = max(1, min(16, cores))
-        rate, dt, n_cpu = cpu_baseline(ens, cfg, pp, threads, target_s=args.cpu_seconds, max_walkers=N)
-        out["cpu_baseline"] = {
-            "value": rate, "unit": "walker-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{n_cpu} walkers x {cfg.n_steps} steps of the same entry, FP64 exact-classifier "
-                      f"walk (oracle/madi_oracle.c, OpenMP) in {dt:.1f} s on {cpu_model()}",
-        }
-    print(json.dumps(out), flush=True)
-    return 0
+        log(f"fp64 check: max |dS| {dS.max():.3e}, max |dS|/SE {ratio.max():.3f}")
+    return out, (ens, cfg, pp)
 
 
 if __name__ == "__main__":

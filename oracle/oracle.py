@@ -309,22 +309,33 @@ def classify_ckdtree(seeds, annulus, pts, tree=None):
 
 
 def walk_cpu_reference(seeds, annulus, L, source_lo, source_hi, n_walkers, n_steps, steps_per_h, n_grid, sigma, ts,
-                       pp, seed=0, stream=None, free_water=False):
+                       pp, seed=0, stream=None, free_water=False, label=None):
     """numpy restatement of the reference CPU walk (madi/walker_gpu.py:698-840), exact classifier.
 
     This is the CPU algorithm the benchmark times as the reference's CPU
     fallback: vectorised draws and proposals, cKDTree nearest and ball
     queries, and the per-point facet loop.
+
+    label (periodic box): seeds / annulus are the box's seeds padded with
+    their periodic images and label maps each point to its cell; positions
+    stay unwrapped (Y is unchanged), each proposal is wrapped into [0, L)
+    for its label, and nothing escapes (the C oracle's periodic walk).
     """
     rng = np.random.default_rng(seed) if stream is None else None
     n = int(n_walkers)
     tree = None if free_water else cKDTree(seeds)
+    periodic = label is not None
     pos = (rng.uniform(source_lo, source_hi, size=(n, 3)) if stream is None
            else np.asarray(stream[0], dtype=np.float64).copy())
 
     def classify(p):
         if free_water:
             return np.zeros(len(p), np.int32), np.zeros(len(p), bool)
+        if periodic:
+            w = p - L * np.floor(p / L)
+            w = np.where(w >= L, w - L, w)
+            s1, ins = classify_ckdtree(seeds, annulus, w, tree)
+            return np.asarray(label)[s1], ins
         return classify_ckdtree(seeds, annulus, p, tree)
 
     cell, inside = classify(pos)
@@ -345,7 +356,7 @@ def walk_cpu_reference(seeds, annulus, L, source_lo, source_hi, n_walkers, n_ste
             inc = np.asarray(stream[1][step])
             acc = np.asarray(stream[2][step])
         prop = pos + inc
-        out = ((prop < 0.0).any(axis=1) | (prop >= L).any(axis=1)) & active
+        out = ((prop < 0.0).any(axis=1) | (prop >= L).any(axis=1)) & active & (not periodic)
         escaped[out] = True
         prop[out] = pos[out]
         active = ~escaped

@@ -238,6 +238,29 @@ def test_oracle_walk_matches_numpy_reference_cpu_walk_on_injected_stream():
     assert 0 < itr.mean() < 1
 
 
+def test_oracle_periodic_walk_matches_numpy_reference_walk_on_the_image_tiling():
+    """The numpy walk over the image-padded seeds (the reference's CPU
+    algorithm seeing the periodic tiling) equals the C oracle's periodic walk."""
+    ens = _geom.periodic(n_cells=64, seed=3)
+    arr = geometry_arrays(ens)
+    g = O.Geometry.from_arrays(arr)
+    n, steps, sph, G = 64, 500, 50, 11
+    sigma, ts, pp = np.sqrt(2 * 2.0 * 0.02), 0.02, 0.05
+    rng = np.random.default_rng(8)
+    x0 = rng.uniform(0.0, ens.L, size=(n, 3))
+    inc = rng.normal(0.0, sigma, size=(steps, n, 3))
+    u = rng.uniform(0.0, 1.0, size=(steps, n))
+    Y, itr, esc, err = O.walk(g, n, steps, sph, G, sigma, ts, pp, 0.0, ens.L, stream=(x0, inc, u))
+    Yr, itr_r, esc_r = O.walk_cpu_reference(arr.kd_xyz, arr.kd_annulus, ens.L, 0.0, ens.L, n, steps, sph, G, sigma,
+                                            ts, pp, stream=(x0, inc, u), label=arr.kd_label)
+    assert not err.any() and not esc.any() and not esc_r.any()
+    assert np.array_equal(itr, itr_r)
+    assert np.allclose(Y, Yr, rtol=5e-11, atol=5e-12)
+    # proposals cross the box faces, so the wrap is exercised
+    path = x0[None] + np.cumsum(inc, axis=0)
+    assert np.any((path < 0.0) | (path >= ens.L)) and 0 < itr.mean() < 1
+
+
 def test_oracle_reduce_matches_numpy_restatement():
     rng = np.random.default_rng(2)
     Y = rng.normal(0.0, 30.0, size=(300, 9, 3))
