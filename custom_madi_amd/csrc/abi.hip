@@ -586,6 +586,7 @@ struct BatchKnobs {
     int rounds, variant, grid, emit_grid;
     bool exits;  // compile the exit certificate into the streamed walks
     unsigned long long* verify = nullptr;  // split walk: the verification build's counters
+    int max_grid = 0;  // longest Y grid of the launch's entries (0: not known, at graph capture)
 };
 
 // Whether any item of a batch table can use the exit certificate.
@@ -710,9 +711,16 @@ static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_p
 // The kernels of one batch on s: the persistent tile kernel, or the
 // streamed walk and the tile sums from its moments.
 static int launch_batch(const WorkDev* table, unsigned long long* item_next, BatchHdr* hdr, const PlanDev* plan,
-                        const Events& ev, const Gf2Col* jp, const BatchKnobs& k, int32_t grid,
+                        const Events& ev, const Gf2Col* jp, const BatchKnobs& k_in, int32_t grid,
                         unsigned long long* counters, hipStream_t s)
 {
+    // the split walk keeps the event table and occupancy counts of a walk in
+    // LDS; a launch with a longer Y grid runs the streamed walk (same results)
+    BatchKnobs k = k_in;
+    if (k.variant == kVariantSplit && k.max_grid > kMaxLdsGrid) {
+        k.variant = kVariantStream;
+        k.rounds = rounds_of(0, kVariantStream);
+    }
     if (!streamed(k.variant))
         return launch_batch_f32(table, hdr, plan, ev, jp, k.cap_width, k.rounds, grid, counters, s);
     int rc = k.variant == kVariantSplit
@@ -819,6 +827,10 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
     }
     const int variant = variant_of(p->classify_mode);
     if (d_verify) {
+        if (W.n_grid > kMaxLdsGrid) {
+            set_error("%s: the verification build takes walks of at most %d samples", who, kMaxLdsGrid);
+            return MADI_ERR_ARG;
+        }
         if (precision != 32 || variant != kVariantSplit) {
             set_error("%s: the verification build is the FP32 split walk (precision 32, classify_mode 0 or 4)", who);
             return MADI_ERR_ARG;
@@ -863,6 +875,7 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         if ((rc = batch_knobs(&tn, p, k))) return rc;
         k.exits = table_exits(table, 1);
         k.verify = (unsigned long long*)d_verify;
+        k.max_grid = W.n_grid;
         const int grid = (int)std::min<int64_t>(k.grid, (walkers + 63) / 64);
         rc = launch_batch((WorkDev*)(base + lay.table), (unsigned long long*)(base + lay.next),
                           (BatchHdr*)(base + lay.hdr), dplan, ev, jp, k, grid, (unsigned long long*)d_counters, s);
@@ -1151,6 +1164,7 @@ struct madi_batch_graph {
     hipGraphExec_t exec = nullptr;
     hipEvent_t done = nullptr;
     bool launched = false;
+    unsigned long long* counters = nullptr;
 };
 
 namespace {
@@ -1217,6 +1231,7 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     h->total_walkers = walkers;
     if (total == 0) return MADI_OK;
     k.exits = table_exits((const WorkDev*)(img.data() + batch_off_table()), n_work);
+    k.max_grid = max_grid;
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
     hipStream_t s = (hipStream_t)stream;
@@ -1249,6 +1264,7 @@ int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const
     }
     gr->plan = *plan;
     gr->capacity = capacity;
+    gr->counters = (unsigned long long*)d_counters;
     gr->tune_cap = tune ? tune->tier_cap_um : 0.f;
     if ((rc = batch_knobs(tune, nullptr, gr->knobs))) {
         delete gr;
@@ -1329,6 +1345,27 @@ int madi_batch_graph_launch(madi_batch_graph* gr, const madi_work* work, int32_t
     h->n_work = n_work;
     h->total_walkers = walkers;
     if (total == 0) return MADI_OK;
+    int max_grid = 0;
+    for (int i = 0; i < n_work; ++i) max_grid = std::max(max_grid, work[i].walk.n_grid);
+    if (gr->knobs.variant == kVariantSplit && max_grid > kMaxLdsGrid) {
+        // the captured split walk keeps its grid in LDS: this batch runs
+        // eagerly on the graph's buffers, and launch_batch picks the
+        // streamed walk for it
+        const Gf2Col* jp = nullptr;
+        if ((rc = get_jump_pow(&jp))) return rc;
+        hipStream_t s = (hipStream_t)stream;
+        HIP_TRY(hipMemcpyAsync(gr->d_buf, gr->h_buf, (size_t)gr->bytes, hipMemcpyHostToDevice, s));
+        BatchKnobs k = gr->knobs;
+        k.max_grid = max_grid;
+        rc = launch_batch((WorkDev*)(gr->d_buf + batch_off_table()),
+                          (unsigned long long*)(gr->d_buf + batch_off_next(gr->capacity)),
+                          (BatchHdr*)(gr->d_buf + batch_off_hdr()), (PlanDev*)gr->d_buf, gr->ev, jp, k, k.grid,
+                          gr->counters, s);
+        if (rc) return rc;
+        HIP_TRY(hipEventRecord(gr->done, s));
+        gr->launched = true;
+        return MADI_OK;
+    }
     HIP_TRY(hipGraphLaunch(gr->exec, (hipStream_t)stream));
     HIP_TRY(hipEventRecord(gr->done, (hipStream_t)stream));
     gr->launched = true;

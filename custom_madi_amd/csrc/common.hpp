@@ -27,14 +27,26 @@ __device__ __forceinline__ uint64_t rotr64_lt32(uint64_t x, uint32_t k)
     return ((uint64_t)nhi << 32) | nlo;
 }
 
+// x ^ y ^ z in one instruction: gfx950's three-input bitwise operation with
+// truth table 0x96 (the compiler does not form it from two XORs).
+__device__ __forceinline__ uint32_t xor3_u32(uint32_t x, uint32_t y, uint32_t z)
+{
+    uint32_t r;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x96" : "=v"(r) : "v"(x), "v"(y), "v"(z));
+    return r;
+}
+
 __device__ __forceinline__ uint64_t rng_next(Rng& r)
 {
     const uint64_t a = r.s0;
     uint64_t b = r.s1;
     const uint64_t out = a + b;
     b ^= a;
-    r.s0 = rotr64_lt32(a, 9) ^ b ^ (b << 14);  // rotl 55
-    r.s1 = rotr64_lt32(b, 28);                 // rotl 36
+    const uint64_t ra = rotr64_lt32(a, 9);  // rotl 55
+    const uint64_t bs = b << 14;
+    r.s0 = ((uint64_t)xor3_u32((uint32_t)(ra >> 32), (uint32_t)(b >> 32), (uint32_t)(bs >> 32)) << 32) |
+           xor3_u32((uint32_t)ra, (uint32_t)b, (uint32_t)bs);
+    r.s1 = rotr64_lt32(b, 28);  // rotl 36
     return out;
 }
 

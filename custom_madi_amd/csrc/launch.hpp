@@ -37,6 +37,11 @@ __host__ __device__ inline bool exit_certificate_pays(float alpha_max, double si
     return alpha_max < 2.5 * sigma;
 }
 
+// Longest Y grid (samples per walk) whose event table and occupancy counters
+// the walk kernels keep in LDS.  The split walk handles only such entries;
+// launches holding a longer one run the streamed walk (same results).
+constexpr int kMaxLdsGrid = 256;
+
 struct FusedOut {
     const PlanDev* plan;   // device copy
     Events ev;

@@ -666,7 +666,6 @@ struct TileOut {
     long long* flags;      // MADI_N_FLAGS
 };
 
-constexpr int kMaxLdsGrid = 256;  // occupancy counters kept in LDS up to this many Y-grid times
 
 // LDS a tile walk needs.
 template <int NP>
@@ -1507,11 +1506,12 @@ constexpr uint32_t kSplitStallLimit = 1u << 26;
 
 // 48 bytes: a ring of kReqSlots per classifier wave
 struct ReqSlot {
-    int seq, na;
+    int na;
     int off;       // the anchor's edge list (< 0: not known, the classifier loads it)
     int len_own;   // bits 0-23: that list's padded length; bits 24-31: the requesting lane
     float hmax, alpha;  // the anchor's largest half length and annulus
     float qx, qy, qz, ox, oy, oz;
+    int seq;       // last: published after the fields
 };
 
 // 144 bytes, the tier first (16-byte aligned in the array)
@@ -1521,6 +1521,41 @@ struct RespSlot {
     float qx, qy, qz, ox, oy, oz;
 };
 static_assert(sizeof(ReqSlot) == 48 && sizeof(RespSlot) == 144, "split ring slots");
+static_assert(sizeof(Tier) == 108, "tier layout of tier_lds_load / tier_lds_store");
+
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef float v3f_t __attribute__((ext_vector_type(3)));
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+
+// A tier to or from LDS as six 16-byte accesses and one 12-byte access from
+// one address (immediate offsets; the slots are 16-byte aligned).
+__device__ __forceinline__ Tier tier_lds_load(const Tier& src)
+{
+    const v4f_t* s4 = (const v4f_t*)&src;
+    const v4f_t a = s4[0], b = s4[1], c = s4[2], d = s4[3], e = s4[4], f = s4[5];
+    const v3f_t g = *(const v3f_t*)((const float*)&src + 24);
+    Tier t;
+    t.ux = a.x, t.uy = a.y, t.uz = a.z, t.h = a.w;
+    t.ux2 = b.x, t.uy2 = b.y, t.uz2 = b.z, t.h2 = b.w;
+    t.ux3 = c.x, t.uy3 = c.y, t.uz3 = c.z, t.h3 = c.w;
+    t.cx = d.x, t.cy = d.y, t.cz = d.z, t.ra_in2 = d.w;
+    t.ra_out2 = e.x, t.rk_in2 = e.y, t.rk_out2 = e.z, t.rout2 = e.w;
+    t.alpha_a = f.x, t.alpha_k = f.y, t.eps = f.z, t.k = __float_as_int(f.w);
+    t.a_off = __float_as_int(g.x), t.a_end = __float_as_int(g.y), t.a_hmax = g.z;
+    return t;
+}
+
+__device__ __forceinline__ void tier_lds_store(Tier& dst, const Tier& t)
+{
+    v4f_t* d4 = (v4f_t*)&dst;
+    d4[0] = v4f_t{t.ux, t.uy, t.uz, t.h};
+    d4[1] = v4f_t{t.ux2, t.uy2, t.uz2, t.h2};
+    d4[2] = v4f_t{t.ux3, t.uy3, t.uz3, t.h3};
+    d4[3] = v4f_t{t.cx, t.cy, t.cz, t.ra_in2};
+    d4[4] = v4f_t{t.ra_out2, t.rk_in2, t.rk_out2, t.rout2};
+    d4[5] = v4f_t{t.alpha_a, t.alpha_k, t.eps, __int_as_float(t.k)};
+    *(v3f_t*)((float*)&dst + 24) = v3f_t{__int_as_float(t.a_off), __int_as_float(t.a_end), t.a_hmax};
+}
 
 template <int NP, int WW, int CW>
 struct SplitLds {
@@ -1632,7 +1667,6 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
             if (lane == 0) L.geo[wv] = g;
             wave_lds_fence();
             const int64_t n_walk = W->n_walk, walker_lo = W->walker_lo;
-            const bool occ_in_lds = p.n_grid <= kMaxLdsGrid;
             const float nscale = (float)(-2.0 * p.sigma * p.sigma * 0.6931471805599453);
             const float leave = EXIT ? exit_bound(g, p) : 0.f;  // EXIT: the certificate is compiled in
             const float ts = (float)p.ts;
@@ -1743,10 +1777,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         escaped = 1;
                         phase = kDone;
                         if (in)
-                            for (int j = j_sample + 1; j < p.n_grid; ++j) {
-                                if (occ_in_lds) atomicAdd(&occ[j], 1);
-                                else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
-                            }
+                            for (int j = j_sample + 1; j < p.n_grid; ++j) atomicAdd(&occ[j], 1);
                     } else if (tier_query(t, a, qx, qy, qz, pcell, pin)) {
                         ++n_tier;
                         vkind = 0;
@@ -1777,18 +1808,12 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         if (enq) {
                             ticket = base + lanes_below(em);
                             ReqSlot& rq = L.req[wv % CW][ticket & (kReqSlots - 1)];
-                            rq.na = qna;
                             // a step's anchor is the tier's: its record rides along
-                            rq.off = init ? -1 : t.a_off;
-                            rq.len_own = ((t.a_end - t.a_off) & 0xffffff) | ((wv * 64 + lane) << 24);
-                            rq.hmax = t.a_hmax;
-                            rq.alpha = t.alpha_a;
-                            rq.qx = qx;
-                            rq.qy = qy;
-                            rq.qz = qz;
-                            rq.ox = qox;
-                            rq.oy = qoy;
-                            rq.oz = qoz;
+                            *(v4i_t*)&rq.na = v4i_t{qna, init ? -1 : t.a_off,
+                                                    ((t.a_end - t.a_off) & 0xffffff) | ((wv * 64 + lane) << 24),
+                                                    __float_as_int(t.a_hmax)};
+                            *(v4f_t*)&rq.alpha = v4f_t{t.alpha_a, qx, qy, qz};
+                            *(v3f_t*)&rq.ox = v3f_t{qox, qoy, qoz};
                             lds_store_release(&rq.seq, (int)(ticket + 1u));
                             enq = 0;
                         }
@@ -1821,7 +1846,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                                  v_bad, v_failed);
                 if (complete && init) {
                     // the walker's first label: no step, no acceptance test
-                    t = my_resp.t;
+                    t = tier_lds_load(my_resp.t);
                     a = rna;
                     in = rflags & 1;
                     cell = rna;
@@ -1831,10 +1856,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                     ox = -qx;
                     oy = -qy;
                     oz = -qz;
-                    if (in) {
-                        if (occ_in_lds) atomicAdd(&occ[0], 1);
-                        else atomicAdd((unsigned long long*)&W->occupancy[0], 1ull);
-                    }
+                    if (in) atomicAdd(&occ[0], 1);
                     init = 0;
                     phase = p.n_steps > 0 ? kStep : kDone;
                 } else if (complete) {
@@ -1851,7 +1873,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         }
                     }
                     if (accept) {
-                        if (fromq) t = my_resp.t;
+                        if (fromq) t = tier_lds_load(my_resp.t);
                         a = fromq ? rna : a;
                         in = pin;
                         cell = pcell;
@@ -1881,34 +1903,16 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         Cz = (tz - Sz) - yz;
                         Sz = tz;
                         hx = hy = hz = 0.f;
-                        if (in) {
-                            if (occ_in_lds) atomicAdd(&occ[j], 1);
-                            else atomicAdd((unsigned long long*)&W->occupancy[j], 1ull);
-                        }
+                        if (in) atomicAdd(&occ[j], 1);
                         const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
                         const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
                         const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
-                        if (occ_in_lds) {
 #pragma unroll
-                            for (int u = 0; u < NP; ++u) {
-                                const float sg = L.evc[j][u];
-                                dM[u][0] = fmaf(sg, Yx, dM[u][0]);
-                                dM[u][1] = fmaf(sg, Yy, dM[u][1]);
-                                dM[u][2] = fmaf(sg, Yz, dM[u][2]);
-                            }
-                        } else {
-                            while (next_ev < ev.n && ev.j[next_ev] == j) {
-                                const int q = ev.pair[next_ev];
-                                const float sg = (float)ev.sign[next_ev];
-#pragma unroll
-                                for (int u = 0; u < NP; ++u)
-                                    if (u == q) {
-                                        dM[u][0] += sg * Yx;
-                                        dM[u][1] += sg * Yy;
-                                        dM[u][2] += sg * Yz;
-                                    }
-                                ++next_ev;
-                            }
+                        for (int u = 0; u < NP; ++u) {
+                            const float sg = L.evc[j][u];
+                            dM[u][0] = fmaf(sg, Yx, dM[u][0]);
+                            dM[u][1] = fmaf(sg, Yy, dM[u][1]);
+                            dM[u][2] = fmaf(sg, Yz, dM[u][2]);
                         }
                     }
                     phase = step >= p.n_steps ? kDone : kStep;
@@ -1944,7 +1948,7 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                     stall = 0;
                 }
             }
-            if (occ_in_lds) {
+            {
                 wave_lds_fence();
                 for (int j = lane; j < p.n_grid; j += 64) {
                     const int v = occ[j];
@@ -2034,17 +2038,20 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
             if (!busy && rank < take) {
                 ticket = head + (unsigned int)rank;
                 const ReqSlot& rq = L.req[cw][ticket & (kReqSlots - 1)];
-                P.na = rq.na;
-                const int len_own = rq.len_own;
+                const v4i_t r0 = *(const v4i_t*)&rq.na;
+                const v4f_t r1 = *(const v4f_t*)&rq.alpha;
+                const v3f_t r2 = *(const v3f_t*)&rq.ox;
+                P.na = r0.x;
+                const int len_own = r0.z;
                 owner = (int)((unsigned)len_own >> 24);
-                P.qx = rq.qx;
-                P.qy = rq.qy;
-                P.qz = rq.qz;
-                P.ox = rq.ox;
-                P.oy = rq.oy;
-                P.oz = rq.oz;
-                const int off = rq.off, end = off + (len_own & 0xffffff);
-                const float hmax = rq.hmax, alpha = rq.alpha;
+                P.qx = r1.y;
+                P.qy = r1.z;
+                P.qz = r1.w;
+                P.ox = r2.x;
+                P.oy = r2.y;
+                P.oz = r2.z;
+                const int off = r0.y, end = off + (len_own & 0xffffff);
+                const float hmax = __int_as_float(r0.w), alpha = r1.x;
                 busy = 1;
                 pending_start(P, S, cphase);
                 if (off >= 0) scan_setup(S, off, end, hmax, alpha);
@@ -2082,7 +2089,7 @@ __device__ __forceinline__ void split_classifier(float cap_width, unsigned long 
             if (res > 0) {
                 Tier t;
                 tier_of(g, P, tmode, S, cap, t);
-                rs.t = t;
+                tier_lds_store(rs.t, t);
             }
             rs.na = P.na;
             rs.flags = (P.pin & 1) | (tmode << 1) | (res < 0 ? 16 : 0);

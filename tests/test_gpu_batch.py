@@ -272,3 +272,20 @@ def test_entries_retire_while_others_are_admitted(spec):
     for r in srv.results:
         assert r.n_escaped == 0 and r.n_error == 0
         assert _same(r, ref[r.key])
+
+
+def test_walks_longer_than_the_lds_grid_run_the_streamed_walk_with_the_same_results():
+    """A 300 ms walk on a 1 ms grid has 301 samples, more than the split walk
+    keeps in LDS (256): the default launch and a graph replay run the streamed
+    walk instead, with the per-tile walk's results bit for bit."""
+    req = batch.library_requests(1, seed=41, walkers=(3000, 3000), cells=(100, 100), T_max_ms=300.0, ts=0.1)[0]
+    assert req.config().n_grid == 301
+    spec = batch.acquisition_spec(n_dirs=3, delta=6.0, Deltas=(15.0, 250.0), b_values=(0.0, 1000.0))
+    prep = batch.prepare_entry(req)
+    lookup = lambda r: prep  # noqa: E731
+    say("long grid: default (split requested) and per-tile plain runs, then a graph-replayed batch")
+    default = batch.run_plain([req], spec, prepare=lookup)[0]
+    tile = batch.run_plain([req], spec, prepare=lookup, classify_mode=1)[0]
+    batched = batch.run_batched([prep], spec)[0]
+    assert default.occupancy.shape == (301,)
+    assert _same(default, tile) and _same(default, batched)
