@@ -20,9 +20,12 @@ resident on the device.  The same line carries:
   restatement) on the host cores, single process and across the worker
   processes, and the FP64 C oracle with OpenMP.
 
+* ``universal_reduce``: the universal library's FP64 column reduce at
+  production size (50,000 walkers x 129 Y samples x 31,125 columns).
+
 Other workloads (--workload): c2, c3 on their own, c4 (a seeded sample of the
 full-range sweep), map (MAP matching of 1e6 voxels against a 14,105-entry
-library, FP64).  --gpus N > 1 exits: this benchmark uses one GPU.
+library, FP64), reduce (the column reduce alone, with its CPU baseline).  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
 from __future__ import annotations
@@ -329,6 +332,114 @@ def run_map(args):
     return 0
 
 
+# FP64 operations per (sample, column) of the column reduce: dM's two adds
+# shared by the pair's b values are left out; phase 1 multiply; Cody-Waite
+# reduction 1 multiply + 3 FMA; r^2 and r^3 factor 2 multiplies; the sin and
+# cos polynomials 8 FMA each; the two sums 2 adds (FMA = 2).
+F_REDUCE = 1 + (1 + 3 * 2) + 2 + 2 * 8 * 2 + 2
+
+
+def measure_reduce(args, log, cpu_baseline=True):
+    """The universal library's column reduce (madi_reduce_f64) at production
+    size: 50,000 walkers x 129 Y samples x 31,125 columns, Y resident."""
+    import torch
+    from custom_madi_amd import _hip, workloads
+    _hip.require_gpu()
+    lib = _hip.load_library()
+    if args.copy_probe:
+        # a 1 GiB device copy: its FETCH_SIZE / WRITE_SIZE scale the reduce's
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    Y, cols = workloads.universal_reduce()
+    n, G, _ = Y.shape
+    n_cols = len(cols.phase_coef)
+    dev = lambda a, dt: torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()  # noqa: E731
+    dY = dev(Y, np.float64)
+    jd, jD, js = dev(cols.j_delta, np.int32), dev(cols.j_Delta, np.int32), dev(cols.j_sum, np.int32)
+    pc = dev(cols.phase_coef, np.float64)
+    cs = torch.empty(n_cols, dtype=torch.float64, device="cuda")
+    sn = torch.empty_like(cs)
+    stream = torch.cuda.current_stream()
+
+    def launch():
+        _hip.check(lib.madi_reduce_f64(_hip.dptr(dY), n, G, _hip.dptr(jd), _hip.dptr(jD), _hip.dptr(js),
+                                       _hip.dptr(pc), n_cols, _hip.dptr(cs), _hip.dptr(sn), _hip.stream_ptr()),
+                   "madi_reduce_f64")
+
+    steps = max(5, args.steps)
+    launch()
+    torch.cuda.synchronize()
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for k in range(steps):
+        starts[k].record(stream)
+        launch()
+        ends[k].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kern_ms = [a.elapsed_time(b) for a, b in zip(starts, ends)]
+    kavg = float(np.mean(kern_ms)) * 1e-3
+    sample_cols = 3.0 * n * n_cols
+    achieved = sample_cols * F_REDUCE / kavg / 1e12
+    traffic = None
+    pmc = ROOT / "profiles" / "reduce_f64_universal_traffic.json"
+    if pmc.is_file():
+        traffic = json.loads(pmc.read_text()).get("bytes_per_launch")
+    out = {
+        "workload": f"universal reduce: {n} walkers x {G} Y samples x {n_cols} columns ({cols.n_pairs} "
+                    f"(delta, Delta) pairs x {cols.n_b} b), FP64, Y resident ({Y.nbytes / 1e6:.0f} MB)",
+        "kernel_ms": kavg * 1e3, "kernel_ms_all": kern_ms, "ms_per_call": 1e3 * wall / steps,
+        "sample_columns_per_s": sample_cols / kavg,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": traffic, "kernel_ms": kavg * 1e3,
+                     "flops_per_sample_column": F_REDUCE},
+    }
+    log(f"universal reduce: {kavg * 1e3:.2f} ms per launch, {achieved:.1f} TFLOP/s")
+    if cpu_baseline:
+        # the reference's CPU reduce (_reduce_cpu: numpy cos/sin over all
+        # walkers and axes, one column at a time) on an evenly spread subset
+        from oracle import oracle as O
+        got_c = cs.cpu().numpy()
+        sub = np.arange(0, n_cols, 7)
+        t0 = time.perf_counter()
+        done = 0
+        for c in sub:
+            O.reduce_reference(Y, cols.j_delta[c:c + 1], cols.j_Delta[c:c + 1], cols.j_sum[c:c + 1],
+                               cols.phase_coef[c:c + 1])
+            done += 1
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        ocs, _ = O.reduce_reference(Y, cols.j_delta[sub[:8]], cols.j_Delta[sub[:8]], cols.j_sum[sub[:8]],
+                                    cols.phase_coef[sub[:8]])
+        out["cpu_baseline"] = {
+            "value": 3.0 * n * done / dt, "unit": "sample-columns/s", "cores": 1, "kind": "port",
+            "sample": f"{done} of the {n_cols} columns over all {n} walkers x 3 axes with the reference's CPU "
+                      f"reduce arithmetic (numpy, one process) in {dt:.1f} s on {cpu_model()}",
+            "max_rel_diff_vs_gpu": float(np.max(np.abs(got_c[sub[:8]] - ocs) / np.maximum(np.abs(ocs), 1.0))),
+        }
+    return out
+
+
+def run_reduce(args):
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    r = measure_reduce(args, log, cpu_baseline=not args.no_cpu_baseline)
+    out = {
+        "metric": "sample-columns/s", "value": r["sample_columns_per_s"], "unit": "sample-columns/s", "n_gpus": 1,
+        "steps": max(5, args.steps), "warmup": 1, "ms_per_step": r["ms_per_call"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": r["workload"]}, "roofline": r["roofline"],
+    }
+    if "cpu_baseline" in r:
+        out["cpu_baseline"] = r["cpu_baseline"]
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _prepared_lookup(preps):
     table = {p.req.key: p for p in preps}
     return lambda req: table[req.key]
@@ -445,7 +556,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target seconds per CPU baseline sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce"], default="c1")
     ap.add_argument("--voxels", type=int, default=1_000_000, help="map: voxels matched per step")
     ap.add_argument("--library-entries", type=int, default=14_105, help="map: library entries")
     ap.add_argument("--entries", type=int, default=256)
@@ -466,6 +577,7 @@ def main(argv=None):
     ap.add_argument("--c3-duration", type=float, default=60.0, help="default line: seconds of c3 arrivals")
     ap.add_argument("--no-c2", action="store_true", help="default line without the c2 object")
     ap.add_argument("--no-c3", action="store_true", help="default line without the c3 object")
+    ap.add_argument("--no-reduce", action="store_true", help="default line without the universal_reduce object")
     ap.add_argument("--library-out", default=None)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -481,6 +593,8 @@ def main(argv=None):
         return run_c4(args)
     if args.workload == "map":
         return run_map(args)
+    if args.workload == "reduce":
+        return run_reduce(args)
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     pool = None
@@ -495,6 +609,8 @@ def main(argv=None):
     if not args.no_c3:
         log(f"c3: Poisson arrivals at {args.rate}/s for {args.c3_duration:.0f} s")
         out["c3"] = measure_c3(args, pool, lambda m: log(f"c3: {m}"), args.c3_duration)
+    if not args.no_reduce:
+        out["universal_reduce"] = measure_reduce(args, log, cpu_baseline=False)
     if not args.no_cpu_baseline:
         procs, vis = max(1, int(args.workers)), visible_cpus()
         single, multi = cpu_baseline_reference(ens, cfg, pp, pool, procs, args.cpu_seconds, log)

@@ -28,6 +28,7 @@ MADI_ERR_BUSY = 6
 MADI_MAX_PAIRS = 8
 MADI_MAX_B = 16
 MADI_MAX_DIRS = 64
+MADI_MAX_REDUCE_COLS = 2_000_000
 N_COUNTERS = 13
 N_FLAGS = 2
 N_VERIFY = 7

@@ -1066,13 +1066,25 @@ int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t*
         return MADI_ERR_ARG;
     }
     if (n_cols == 0) return MADI_OK;
-    int32_t chunks = (int32_t)((n * 3 + 8191) / 8192);
-    if (chunks < 1) chunks = 1;
-    if (chunks > 512) chunks = 512;
+    if (n_cols > MADI_MAX_REDUCE_COLS) {
+        set_error("madi_reduce_f64: more than MADI_MAX_REDUCE_COLS columns");
+        return MADI_ERR_ARG;
+    }
+    const int32_t chunks = reduce_chunks(n, n_cols);
     double* part = nullptr;
+    double* yt = nullptr;
     HIP_TRY(hipMallocAsync((void**)&part, (size_t)n_cols * chunks * 2 * sizeof(double), (hipStream_t)stream));
-    int rc = launch_reduce_f64(d_Y, n, n_grid, d_jd, d_jD, d_js, d_pc, n_cols, part, chunks, d_cos, d_sin,
+    if (n > 0) {
+        const hipError_t e = hipMallocAsync((void**)&yt, (size_t)n * n_grid * 3 * sizeof(double), (hipStream_t)stream);
+        if (e != hipSuccess) {
+            (void)hipFreeAsync(part, (hipStream_t)stream);
+            set_error("madi_reduce_f64: Y staging allocation failed: %s", hipGetErrorString(e));
+            return e == hipErrorOutOfMemory ? MADI_ERR_OOM : MADI_ERR_HIP;
+        }
+    }
+    int rc = launch_reduce_f64(d_Y, n, n_grid, d_jd, d_jD, d_js, d_pc, n_cols, yt, part, chunks, d_cos, d_sin,
                                (hipStream_t)stream);
+    if (yt) (void)hipFreeAsync(yt, (hipStream_t)stream);
     (void)hipFreeAsync(part, (hipStream_t)stream);
     return rc;
 }

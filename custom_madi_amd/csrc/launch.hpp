@@ -98,8 +98,12 @@ int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, 
                     hipStream_t s);
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
                           unsigned long long* counters, hipStream_t s);
+// Sample chunks of the column reduce for n walkers and n_cols columns (the
+// partial buffer holds n_cols x chunks x 2 doubles).
+int32_t reduce_chunks(int64_t n, int32_t n_cols);
+// Yt: n * G * 3 doubles of scratch (Y in grid-major order).
 int launch_reduce_f64(const double* Y, int64_t n, int32_t G, const int32_t* jd, const int32_t* jD, const int32_t* js,
-                      const double* pc, int32_t n_cols, double* part, int32_t n_chunks, double* cos_out,
+                      const double* pc, int32_t n_cols, double* Yt, double* part, int32_t n_chunks, double* cos_out,
                       double* sin_out, hipStream_t s);
 
 int launch_classify_graph(const GeomDev& g, const double* pts, int64_t n, int32_t* cell, int8_t* inside, int8_t* ok,

@@ -222,46 +222,224 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
 
 // -------------------------------------------------------------- reduce
 
-// Columns x chunks grid; each block sums a contiguous range of (walker, axis)
-// samples of one column; chunk partials are then added in chunk order.
-__global__ __launch_bounds__(kBlock) void k_reduce_f64(const double* __restrict__ Y, int64_t n, int32_t G,
+// sin and cos of x in FP64 for |x| <= kSinCosFast.  Cody and Waite's
+// additive reduction: n = rint(x * 2/pi) and r = x - n*pi/2 with pi/2 split
+// into three doubles (pi/2 to 160 bits, each part applied with one FMA), so
+// |r| <= pi/4 to within a few units of its last place; then the Taylor series
+// of sin and cos on r through r^17 and r^16 (the first omitted terms are below
+// 1e-17 there), Horner in r^2, and the quadrant n mod 4.  No branch: callers
+// route larger or non-finite arguments to the device library's sincos.
+constexpr double kSinCosFast = 0x1p20;
+
+// The constants of sincos_cw.  Kernels take them by value, so they sit in
+// scalar registers and each Horner step is one three-input FMA (an FP64
+// literal cannot be an operand of one: a constant written in the code costs
+// a copy into the destination register at every step).
+struct SinCosCoef {
+    double two_over_pi;
+    double p1, p2, p3;  // pi/2 rounded to a double, then the rounded remainders
+    double s[8];        // sin r = r + r^3 * (s[0] + z s[1] + ... + z^7 s[7]), s[k] = (-1)^(k+1) / (2k+3)!
+    double c[8];        // cos r = 1 + z * (c[0] + z c[1] + ... + z^7 c[7]), c[k] = (-1)^(k+1) / (2k+2)!
+};
+constexpr SinCosCoef kSinCos = {
+    0x1.45f306dc9c883p-1,
+    0x1.921fb54442d18p+0, 0x1.1a62633145c07p-54, -0x1.f1976b7ed8fbcp-110,
+    {-1.0 / 6.0, 1.0 / 120.0, -1.0 / 5040.0, 1.0 / 362880.0, -1.0 / 39916800.0, 1.0 / 6227020800.0,
+     -1.0 / 1307674368000.0, 1.0 / 355687428096000.0},
+    {-1.0 / 2.0, 1.0 / 24.0, -1.0 / 720.0, 1.0 / 40320.0, -1.0 / 3628800.0, 1.0 / 479001600.0,
+     -1.0 / 87178291200.0, 1.0 / 20922789888000.0},
+};
+
+__device__ __forceinline__ void sincos_cw(double x, const SinCosCoef& k, double& s, double& c)
+{
+    const double n = __builtin_rint(x * k.two_over_pi);
+    double r = __builtin_fma(-n, k.p1, x);
+    r = __builtin_fma(-n, k.p2, r);
+    r = __builtin_fma(-n, k.p3, r);
+    const double z = r * r;
+    double ps = k.s[7], pq = k.c[7];
+#pragma unroll
+    for (int j = 6; j >= 0; --j) {
+        ps = __builtin_fma(ps, z, k.s[j]);
+        pq = __builtin_fma(pq, z, k.c[j]);
+    }
+    const double sr = __builtin_fma(r * z, ps, r);
+    const double cr = __builtin_fma(z, pq, 1.0);
+    // x = n*pi/2 + r: quadrant q gives (sin, cos) = (sr, cr), (cr, -sr), (-sr, -cr), (-cr, sr)
+    const int q = (int)n & 3;
+    const double a = (q & 1) ? cr : sr, b = (q & 1) ? sr : cr;
+    s = (q & 2) ? -a : a;
+    c = ((q + 1) & 2) ? -b : b;
+}
+
+// Y (walker, grid, axis) to grid-major Yt (grid, sample), sample i = walker *
+// 3 + axis: block (walker tile, grid tile) stages 16 walkers x 128 grid
+// points through LDS, reading each walker's segment and writing each grid
+// row's segment contiguously.
+constexpr int kTrW = 16;
+constexpr int kTrG = 128;
+
+__global__ __launch_bounds__(kBlock) void k_reduce_transpose(const double* __restrict__ Y, int64_t n, int32_t G,
+                                                            double* __restrict__ Yt)
+{
+    __shared__ double tile[kTrW][kTrG * 3 + 1];
+    const int64_t w0 = (int64_t)blockIdx.x * kTrW;
+    const int g0 = blockIdx.y * kTrG;
+    const int wn = n - w0 < kTrW ? (int)(n - w0) : kTrW;
+    const int gn = G - g0 < kTrG ? G - g0 : kTrG;
+    for (int idx = threadIdx.x; idx < wn * gn * 3; idx += kBlock) {
+        const int w = idx / (gn * 3), r = idx - w * (gn * 3);
+        tile[w][r] = Y[(size_t)(w0 + w) * G * 3 + (size_t)g0 * 3 + r];
+    }
+    __syncthreads();
+    const int64_t total = n * 3;
+    for (int idx = threadIdx.x; idx < gn * wn * 3; idx += kBlock) {
+        const int g = idx / (wn * 3), r = idx - g * (wn * 3);
+        Yt[(size_t)(g0 + g) * total + w0 * 3 + r] = tile[r / 3][g * 3 + r % 3];
+    }
+}
+
+// Column reduce over Yt: block b sums kRedCols consecutive columns (column
+// block b / n_chunks) over the tiles of kRedTile samples of chunk b % n_chunks
+// (sample i = walker i / 3, axis i % 3 as in the reference).  Workgroups go
+// to the eight XCDs in turn and n_chunks is a multiple of 8, so a chunk stays
+// on one XCD for every column block and neighbouring pairs' shared grid rows
+// are reused from that XCD's L2.  A thread holds kRedPer samples of the tile
+// (i = tile start + thread + k * kBlock) and their dM = Yt[jd] + Yt[jD] -
+// Yt[js], reloaded only when a column's grid indices differ from the previous
+// column's (the b values of one (delta, Delta) pair share them).  A column
+// whose phases all lie within kSinCosFast on the wave (bounded by |pc| times
+// the wave's largest |dM|) takes sincos_cw without a branch; any other column
+// takes the device library's sincos for every sample.  Per column: the thread's sum in k order, a wave butterfly,
+// added into the wave's LDS row; at the end the waves' rows in wave order
+// give the (column, chunk) partial.  k_reduce_finish adds partials in chunk
+// order, so the result does not depend on scheduling.
+constexpr int kRedCols = 32;
+constexpr int kRedPer = 16;
+constexpr int kRedTile = kBlock * kRedPer;
+
+__global__ __launch_bounds__(kBlock) void k_reduce_f64(const double* __restrict__ Yt, int64_t n,
                                                       const int32_t* __restrict__ jd, const int32_t* __restrict__ jD,
                                                       const int32_t* __restrict__ js, const double* __restrict__ pc,
-                                                      int32_t n_cols, int32_t n_chunks, double* __restrict__ part)
+                                                      int32_t n_cols, int32_t n_chunks, double* __restrict__ part,
+                                                      SinCosCoef coef)
 {
-    __shared__ double sc[kBlock], ss[kBlock];
-    const int col = blockIdx.y;
-    const int chunk = blockIdx.x;
+    __shared__ double red[kWaves][kRedCols][2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int chunk = (int)(blockIdx.x % (unsigned)n_chunks);
+    const int c0 = (int)(blockIdx.x / (unsigned)n_chunks) * kRedCols;
+    const int nc = n_cols - c0 < kRedCols ? n_cols - c0 : kRedCols;
     const int64_t total = n * 3;
-    const int64_t per = (total + n_chunks - 1) / n_chunks;
-    const int64_t lo = chunk * per;
-    const int64_t hi = lo + per < total ? lo + per : total;
-    const int a = jd[col], b = jD[col], c = js[col];
-    const double k = pc[col];
-    double cs = 0.0, sn = 0.0;
-    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
-        const int64_t w = i / 3;
-        const int ax = (int)(i - 3 * w);
-        const double* Yw = Y + (size_t)w * G * 3;
-        const double dMv = Yw[3 * a + ax] + Yw[3 * b + ax] - Yw[3 * c + ax];
-        const double ph = k * dMv;
-        cs += cos(ph);
-        sn += sin(ph);
+    const int64_t n_tiles = (total + kRedTile - 1) / kRedTile;
+    const int64_t per_chunk = (n_tiles + n_chunks - 1) / n_chunks;
+    const int64_t t_lo = (int64_t)chunk * per_chunk;
+    const int64_t t_hi = t_lo + per_chunk < n_tiles ? t_lo + per_chunk : n_tiles;
+    for (int j = threadIdx.x; j < kWaves * kRedCols; j += kBlock) {
+        red[j / kRedCols][j % kRedCols][0] = 0.0;
+        red[j / kRedCols][j % kRedCols][1] = 0.0;
     }
-    sc[threadIdx.x] = cs;
-    ss[threadIdx.x] = sn;
     __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) {
-            sc[threadIdx.x] += sc[threadIdx.x + s];
-            ss[threadIdx.x] += ss[threadIdx.x + s];
+    double dm[kRedPer];
+    for (int64_t tile = t_lo; tile < t_hi; ++tile) {
+        const int64_t i0 = tile * kRedTile + threadIdx.x;
+        const int64_t left = total - i0;
+        const int nv = left <= 0 ? 0 : (left >= (int64_t)kRedTile ? kRedPer : (int)((left + kBlock - 1) / kBlock));
+        const bool full = (tile + 1) * kRedTile <= total;  // every thread holds kRedPer samples
+        int pa = -1, pb = -1, ps = -1;
+        double dmax = 0.0;  // the wave's largest |dM| (NaN if any is NaN)
+        for (int j = 0; j < nc; ++j) {
+            const int col = c0 + j;
+            const int a = jd[col], b = jD[col], c = js[col];
+            if (a != pa || b != pb || c != ps) {
+                pa = a;
+                pb = b;
+                ps = c;
+                const double* Ya = Yt + (size_t)a * total + i0;
+                const double* Yb = Yt + (size_t)b * total + i0;
+                const double* Yc = Yt + (size_t)c * total + i0;
+#pragma unroll
+                for (int k = 0; k < kRedPer; ++k) {
+                    dm[k] = 0.0;
+                    if (k < nv) dm[k] = Ya[k * kBlock] + Yb[k * kBlock] - Yc[k * kBlock];
+                }
+                double m = 0.0;
+#pragma unroll
+                for (int k = 0; k < kRedPer; ++k) {
+                    const double v = __builtin_fabs(dm[k]);
+                    m = (v > m || v != v) ? v : m;
+                }
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const double o = __shfl_xor(m, off, 64);
+                    m = (o > m || o != o) ? o : m;
+                }
+                dmax = m;
+            }
+            const double kc = pc[col];
+            double cs = 0.0, sn = 0.0;
+            const bool fast = __builtin_fabs(kc) * dmax <= kSinCosFast;
+            if (fast && full) {
+#pragma unroll
+                for (int k = 0; k < kRedPer; ++k) {
+                    double sv, cv;
+                    sincos_cw(kc * dm[k], coef, sv, cv);
+                    cs += cv;
+                    sn += sv;
+                }
+            } else if (fast) {
+#pragma unroll
+                for (int k = 0; k < kRedPer; ++k) {
+                    double sv, cv;
+                    sincos_cw(kc * dm[k], coef, sv, cv);
+                    if (k < nv) {
+                        cs += cv;
+                        sn += sv;
+                    }
+                }
+            } else {
+                for (int k = 0; k < nv; ++k) {
+                    double sv, cv;
+                    sincos(kc * dm[k], &sv, &cv);
+                    cs += cv;
+                    sn += sv;
+                }
+            }
+            cs = wave_sum_f64(cs);
+            sn = wave_sum_f64(sn);
+            if (lane == 0) {
+                red[wave][j][0] += cs;
+                red[wave][j][1] += sn;
+            }
         }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        part[((size_t)col * n_chunks + chunk) * 2 + 0] = sc[0];
-        part[((size_t)col * n_chunks + chunk) * 2 + 1] = ss[0];
+    __syncthreads();
+    const int t = threadIdx.x;
+    if (t < nc) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) {
+            a += red[w][t][0];
+            b += red[w][t][1];
+        }
+        part[((size_t)(c0 + t) * n_chunks + chunk) * 2 + 0] = a;
+        part[((size_t)(c0 + t) * n_chunks + chunk) * 2 + 1] = b;
     }
+}
+
+int32_t reduce_chunks(int64_t n, int32_t n_cols)
+{
+    const int64_t n_tiles = (n * 3 + kRedTile - 1) / kRedTile;
+    const int64_t col_blocks = ((int64_t)n_cols + kRedCols - 1) / kRedCols;
+    // enough blocks to fill the chip several times over, and partials of at
+    // most a few tens of MB
+    int64_t want = (16384 + col_blocks - 1) / col_blocks;
+    if (want < 64) want = 64;
+    int64_t chunks = n_tiles < want ? n_tiles : want;
+    // a multiple of the XCD count (see k_reduce_f64); chunks past the last
+    // tile add zeros
+    chunks = (chunks + 7) / 8 * 8;
+    if (chunks < 8) chunks = 8;
+    return (int32_t)chunks;
 }
 
 __global__ void k_reduce_finish(const double* __restrict__ part, int32_t n_cols, int32_t n_chunks,
@@ -322,12 +500,19 @@ int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
 }
 
 int launch_reduce_f64(const double* Y, int64_t n, int32_t G, const int32_t* jd, const int32_t* jD, const int32_t* js,
-                      const double* pc, int32_t n_cols, double* part, int32_t n_chunks, double* cos_out,
+                      const double* pc, int32_t n_cols, double* Yt, double* part, int32_t n_chunks, double* cos_out,
                       double* sin_out, hipStream_t s)
 {
     if (n_cols == 0) return MADI_OK;
-    hipLaunchKernelGGL(k_reduce_f64, dim3(n_chunks, n_cols), dim3(kBlock), 0, s, Y, n, G, jd, jD, js, pc, n_cols,
-                       n_chunks, part);
+    if (n > 0) {
+        hipLaunchKernelGGL(k_reduce_transpose, dim3((unsigned)((n + kTrW - 1) / kTrW), (G + kTrG - 1) / kTrG),
+                           dim3(kBlock), 0, s, Y, n, G, Yt);
+        const int rc = check_launch("k_reduce_transpose");
+        if (rc) return rc;
+    }
+    const int64_t blocks = (int64_t)n_chunks * ((n_cols + kRedCols - 1) / kRedCols);
+    hipLaunchKernelGGL(k_reduce_f64, dim3((unsigned)blocks), dim3(kBlock), 0, s, Yt, n, jd, jD, js, pc, n_cols,
+                       n_chunks, part, kSinCos);
     int rc = check_launch("k_reduce_f64");
     if (rc) return rc;
     hipLaunchKernelGGL(k_reduce_finish, dim3((n_cols + 127) / 128), dim3(128), 0, s, part, n_cols, n_chunks, cos_out,

@@ -173,6 +173,12 @@ def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
     torch = _hip.require_gpu()
     n, G, _ = Y_dev.shape
     n_cols = len(pcs)
+    if n_cols > _hip.MADI_MAX_REDUCE_COLS:
+        raise ValueError(f"{n_cols} columns exceed one reduce call's {_hip.MADI_MAX_REDUCE_COLS}")
+    for name, j in (("j_delta", j_delta), ("j_Delta", j_Delta), ("j_sum", j_sum)):
+        j = np.asarray(j)
+        if j.size and (j.min() < 0 or j.max() >= G):
+            raise ValueError(f"{name} holds grid indices outside [0, {G})")
     jd = _hip.to_device(np.asarray(j_delta, dtype=np.int32))
     jD = _hip.to_device(np.asarray(j_Delta, dtype=np.int32))
     js = _hip.to_device(np.asarray(j_sum, dtype=np.int32))

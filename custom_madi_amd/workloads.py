@@ -90,3 +90,20 @@ def c3_requests(rate_hz: float = 20.0, duration_s: float = 600.0):
 
 def c4_requests() -> List[batch.EntryRequest]:
     return batch.sweep_requests()
+
+
+def universal_reduce(n_walkers: int = 50_000, seed: int = 7):
+    """The universal library's column reduce at production size: the default
+    storage grid (1245 (delta, Delta) pairs x 25 b = 31,125 columns, h = 1 ms
+    over T_max = 128 ms, so 129 Y samples) over n_walkers synthetic walkers.
+    Y is the integrated displacement of free diffusion at the default D0,
+    sampled every h, so phases have their production magnitudes.  Returns (Y, columns)."""
+    from . import signal
+    cfg = SimConfig()
+    cols = signal.build_columns(cfg)
+    G = cfg.n_grid
+    rng = np.random.default_rng(seed)
+    inc = rng.normal(0.0, np.sqrt(2.0 * cfg.D0 * cfg.h_ms), size=(n_walkers, G, 3))
+    inc[:, 0, :] = 0.0
+    Y = np.cumsum(np.cumsum(inc, axis=1), axis=1) * cfg.h_ms
+    return Y, cols

@@ -37,6 +37,8 @@ extern "C" {
 #define MADI_MAX_PAIRS 8
 #define MADI_MAX_B 16
 #define MADI_MAX_DIRS 64
+/* Columns of one madi_reduce_f64 call (its grid holds 32 columns per block row). */
+#define MADI_MAX_REDUCE_COLS 2000000
 
 /* Counters returned by the walk kernels (int64 each). */
 #define MADI_CNT_CLASSIFY 0      /* endpoint classifications run (graph or KD) */
@@ -202,7 +204,10 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
                   int64_t* d_counters, void* stream);
 
 /* Sigma cos / sigma sin over walkers and axes of pc*(Y[jd]+Y[jD]-Y[js]) per
- * column, FP64.  Replaces _reduce_kernel (madi/walker_gpu.py:658-695). */
+ * column, FP64.  Replaces _reduce_kernel (madi/walker_gpu.py:658-695).  Every
+ * jd/jD/js must lie in [0, n_grid); at most MADI_MAX_REDUCE_COLS columns.
+ * Sums are formed in a fixed order (bitwise reproducible); consecutive
+ * columns with equal grid indices (the b values of one pair) share dM. */
 int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
                     const int32_t* d_js, const double* d_pc, int32_t n_cols, double* d_cos, double* d_sin,
                     void* stream);

@@ -162,6 +162,52 @@ def test_reduce_matches_oracle():
     assert np.allclose(sn, osn, rtol=1e-11, atol=1e-9)
 
 
+def test_reduce_universal_grid_production_size():
+    """The universal library's reduce at production size (50,000 walkers x 129
+    Y samples x 31,125 columns; reference kernel madi/walker_gpu.py:658-695):
+    every 61st column against the oracle's sequential sums, b = 0 columns
+    exactly 3n / 0 (phase 0 everywhere), and two launches bitwise equal."""
+    torch = _hip.require_gpu()
+    Y, cols = workloads.universal_reduce()
+    n = Y.shape[0]
+    dY = torch.from_numpy(Y).cuda()
+    cs, sn = engine.reduce_columns(dY, cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    cs2, sn2 = engine.reduce_columns(dY, cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    assert np.array_equal(cs, cs2) and np.array_equal(sn, sn2)
+    sub = np.arange(0, len(cols.phase_coef), 61)
+    ocs, osn = O.reduce(Y, cols.j_delta[sub], cols.j_Delta[sub], cols.j_sum[sub], cols.phase_coef[sub])
+    # sums of 150,000 unit-bounded terms in different orders
+    assert np.all(np.abs(cs[sub] - ocs) <= 1e-11 * np.maximum(np.abs(ocs), 1.0) + 1e-9)
+    assert np.all(np.abs(sn[sub] - osn) <= 1e-11 * np.maximum(np.abs(osn), 1.0) + 1e-9)
+    zero = cols.phase_coef == 0.0
+    assert zero.sum() == cols.n_pairs
+    assert np.all(cs[zero] == 3.0 * n) and np.all(sn[zero] == 0.0)
+
+
+def test_reduce_sincos_elementwise():
+    """One walker with Y[0] = 0 and Y[1] = (1, 0, 0), columns (jd, jD, js) =
+    (1, 0, 0): column c's phase is exactly pc[c] on axis 0 and 0 on the others,
+    so its sums are sin(pc) and 2 + cos(pc).  Against numpy's sin and cos over
+    the fast range (to 2^20), across quadrant boundaries (multiples of pi/2),
+    tiny arguments and the library path beyond 2^20."""
+    torch = _hip.require_gpu()
+    rng = np.random.default_rng(1)
+    xs = np.concatenate([rng.uniform(-60, 60, 200_000), rng.uniform(-2e5, 2e5, 20_000),
+                         rng.uniform(-1e-6, 1e-6, 5_000), np.arange(-200, 201) * (np.pi / 2),
+                         np.array([0.0, -0.0, 1e-300, 7.0e5, -3.3e6, 1e12, 1048576.0, 1048577.0, -1048576.5])])
+    Y1 = np.zeros((1, 2, 3))
+    Y1[0, 1, 0] = 1.0
+    one, zero = np.ones(len(xs), dtype=np.int32), np.zeros(len(xs), dtype=np.int32)
+    cs, sn = engine.reduce_columns(torch.from_numpy(Y1).cuda(), one, zero, zero, xs)
+    ref_s, ref_c = np.sin(xs), np.cos(xs)
+    fast = np.abs(xs) <= 2.0 ** 20
+    ulps = np.abs(sn - ref_s) / np.spacing(np.maximum(np.abs(ref_s), 1e-300))
+    assert ulps[fast].max() <= 2.0
+    assert np.abs(sn - ref_s).max() <= 2e-16
+    # cos carries the rounding of 2 + cos (half an ulp of 2..3)
+    assert np.abs((cs - 2.0) - ref_c).max() <= 5e-16
+
+
 def _spec(G, sph_ms=1.0, dirs=engine.AXIS_DIRECTIONS, bs=(0.0, 1000.0, 2500.0, 6000.0)):
     delta, Deltas = 2.0, (5.0, 8.0, 10.0)
     pc = np.array([[engine.phase_coef(b, delta, D) for b in bs] for D in Deltas])
