@@ -31,6 +31,8 @@ MADI_MAX_DIRS = 64
 MADI_MAX_REDUCE_COLS = 2_000_000
 N_COUNTERS = 13
 N_FLAGS = 2
+MADI_FLAG_ESCAPED = 0
+MADI_FLAG_CLASSIFIER_ERR = 1
 N_VERIFY = 7
 VERIFY_NAMES = ("tier_checked", "tier_mismatch", "exit_checked", "exit_mismatch", "classifier_checked",
                 "classifier_mismatch", "exact_failed")
@@ -72,7 +74,7 @@ class Work(C.Structure):
     _fields_ = [
         ("geom", C.c_void_p), ("walk", WalkParams), ("tile_lo", C.c_int64), ("n_tiles", C.c_int64),
         ("d_partial", C.c_void_p), ("d_occupancy", C.c_void_p), ("d_flags", C.c_void_p),
-        ("d_moments", C.c_void_p),
+        ("d_moments", C.c_void_p), ("d_Y", C.c_void_p),
     ]
 
 
@@ -124,6 +126,8 @@ SIGNATURES = {
     "madi_walk_fused_static": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, _p, _p,
                                          _p, _p, _p]),
     "madi_moments_bytes": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
+    "madi_walk_y_f32_workspace": (C.c_int, [C.c_int64, _i64p]),
+    "madi_walk_y_f32": (C.c_int, [_p, C.POINTER(WalkParams), _p, C.c_int64, _p, _p, _p, _p, _p]),
     "madi_batch_workspace": (C.c_int, [C.c_int32, _i64p]),
     "madi_batch_launch": (C.c_int, [C.POINTER(Work), C.c_int32, C.POINTER(PhasePlan), C.POINTER(Tune), _p, C.c_int64,
                                     _p, _p]),

@@ -702,6 +702,12 @@ static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_p
         D.walker_begin = n_w;
         D.moments = (float*)w.d_moments;
         D.active = w.d_moments ? (uint8_t*)w.d_moments + w.walk.n_walkers * (int64_t)plan->n_pairs * 3 * 4 : nullptr;
+        if (w.d_Y && !stream) {
+            set_error("%s: work item %d asks for Y, which only the streamed walks write (classify_mode 3 and 4)", who,
+                      i);
+            return MADI_ERR_ARG;
+        }
+        D.y_out = w.d_Y;
         n_w += D.n_walk;
     }
     if (walkers) *walkers = n_w;
@@ -773,7 +779,7 @@ constexpr int64_t kFusedImgBytes = 4096;
 static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan, int32_t precision,
                       char* base, int64_t workspace_bytes, double* d_cos, double* d_sin, int64_t* d_occupancy,
                       int64_t* d_flags, int64_t* d_counters, hipStream_t s, const char* who,
-                      int64_t* d_verify = nullptr)
+                      int64_t* d_verify = nullptr, double* d_y = nullptr)
 {
     WalkDev W;
     int rc = make_walk(p, W);
@@ -841,6 +847,10 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         }
         if ((rc = zero_i64((long long*)d_verify, MADI_N_VERIFY, s))) return rc;
     }
+    if (d_y && (precision != 32 || !streamed(variant))) {
+        set_error("%s: Y output is written by the FP32 streamed and split walks (classify_mode 0, 3 or 4)", who);
+        return MADI_ERR_ARG;
+    }
     if (precision == 32 && streamed(variant)) {
         // one work item covering the entry
         madi_work item;
@@ -853,6 +863,7 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         item.d_occupancy = d_occupancy;
         item.d_flags = d_flags;
         item.d_moments = base + lay.moments;
+        item.d_Y = d_y;
         alignas(256) char img[kFusedImgBytes];
         const int64_t img_bytes = lay.moments - lay.hdr;
         memset(img, 0, (size_t)img_bytes);
@@ -1119,6 +1130,51 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
     }
     return fused_walk(g, p, plan, precision, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
                       d_counters, (hipStream_t)stream, "madi_walk_fused");
+}
+
+// The phase plan of madi_walk_y_f32: one column (pair (0, 0, 0), pc 0), so
+// the fused machinery runs unchanged and its single sum is ignored.
+static madi_phase_plan y_walk_plan()
+{
+    madi_phase_plan plan;
+    memset(&plan, 0, sizeof plan);
+    plan.n_pairs = 1;
+    plan.n_b = 1;
+    plan.n_dirs = 1;
+    plan.dirs[0] = 1.0;
+    return plan;
+}
+
+int madi_walk_y_f32_workspace(int64_t n_walkers, int64_t* bytes)
+{
+    if (!bytes || n_walkers < 0) {
+        set_error("madi_walk_y_f32_workspace: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const madi_phase_plan plan = y_walk_plan();
+    *bytes = fused_layout(n_walkers, &plan).total + kAlign;  // + the dummy column's two sums
+    return MADI_OK;
+}
+
+int madi_walk_y_f32(const madi_geom* g, const madi_walk_params* p, void* d_workspace, int64_t workspace_bytes,
+                    double* d_Y, int64_t* d_occupancy, int64_t* d_flags, int64_t* d_counters, void* stream)
+{
+    if (!g || !p || !d_workspace || !d_Y || p->n_walkers < 0 || p->n_grid < 1) {
+        set_error("madi_walk_y_f32: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    const madi_phase_plan plan = y_walk_plan();
+    const FusedLayout lay = fused_layout(p->n_walkers, &plan);
+    if (workspace_bytes < lay.total + kAlign) {
+        set_error("madi_walk_y_f32: workspace of %lld bytes is smaller than the %lld required",
+                  (long long)workspace_bytes, (long long)(lay.total + kAlign));
+        return MADI_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    HIP_TRY(hipMemsetAsync(d_Y, 0, (size_t)p->n_walkers * p->n_grid * 3 * sizeof(double), s));
+    double* sums = (double*)((char*)d_workspace + lay.total);
+    return fused_walk(g, p, &plan, 32, (char*)d_workspace, lay.total, sums, sums + 1, d_occupancy, d_flags,
+                      d_counters, s, "madi_walk_y_f32", nullptr, d_Y);
 }
 
 int madi_walk_fused_verify(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,

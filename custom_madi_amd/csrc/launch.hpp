@@ -85,6 +85,7 @@ struct WorkDev {
     int64_t walker_begin;
     float* moments;
     uint8_t* active;
+    double* y_out;  // NULL, or Y at every sample: walker x n_grid x 3 (streamed walks only)
 };
 
 void set_error(const char* fmt, ...);

@@ -1387,6 +1387,13 @@ __device__ __forceinline__ void stream_wave(const WorkDev* __restrict__ work, un
                         const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
                         const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
                         const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+                        if (W->y_out) {
+                            // the walker's Y sample, widened to FP64
+                            double* yo = W->y_out + ((walker_lo + w) * (int64_t)p.n_grid + j) * 3;
+                            yo[0] = (double)Yx;
+                            yo[1] = (double)Yy;
+                            yo[2] = (double)Yz;
+                        }
                         if (occ_in_lds) {
 #pragma unroll
                             for (int u = 0; u < NP; ++u) {
@@ -1907,6 +1914,13 @@ __device__ __forceinline__ void split_walker(const WorkDev* __restrict__ work, u
                         const float Yx = ts * ((Sx - Cx) - 0.5f * Dx);
                         const float Yy = ts * ((Sy - Cy) - 0.5f * Dy);
                         const float Yz = ts * ((Sz - Cz) - 0.5f * Dz);
+                        if (W->y_out) {
+                            // the walker's Y sample, widened to FP64
+                            double* yo = W->y_out + ((walker_lo + w) * (int64_t)p.n_grid + j) * 3;
+                            yo[0] = (double)Yx;
+                            yo[1] = (double)Yy;
+                            yo[2] = (double)Yz;
+                        }
 #pragma unroll
                         for (int u = 0; u < NP; ++u) {
                             const float sg = L.evc[j][u];

@@ -168,6 +168,37 @@ def walk_y(geom: DeviceGeometry, params: _hip.WalkParams, stream=None) -> WalkY:
                  dict(zip(_hip.COUNTER_NAMES, cnt.cpu().numpy().tolist())))
 
 
+@dataclass
+class WalkYF32:
+    Y: object              # device tensor (n, n_grid, 3) float64: the FP32 walk's Y widened
+    occupancy: np.ndarray  # (n_grid,) int64 inside counts
+    n_escaped: int
+    n_error: int
+    counters: dict
+
+
+def walk_y_f32(geom: DeviceGeometry, params: _hip.WalkParams) -> WalkYF32:
+    """The FP32 production walk writing Y(t) on the device (madi_walk_y_f32):
+    the same walk as walk_fused at precision 32, for any column set."""
+    torch = _hip.require_gpu()
+    n, G = int(params.n_walkers), int(params.n_grid)
+    lib = _hip.load_library()
+    nbytes = C.c_int64()
+    _hip.check(lib.madi_walk_y_f32_workspace(n, C.byref(nbytes)), "madi_walk_y_f32_workspace")
+    ws = torch.empty(int(nbytes.value), dtype=torch.uint8, device="cuda")
+    Y = torch.empty((n, G, 3), dtype=torch.float64, device="cuda")
+    occ = torch.empty(G, dtype=torch.int64, device="cuda")
+    flags = torch.empty(_hip.N_FLAGS, dtype=torch.int64, device="cuda")
+    cnt = torch.empty(_hip.N_COUNTERS, dtype=torch.int64, device="cuda")
+    rc = lib.madi_walk_y_f32(geom.handle, C.byref(params), _hip.dptr(ws), int(nbytes.value), _hip.dptr(Y),
+                             _hip.dptr(occ), _hip.dptr(flags), _hip.dptr(cnt), _hip.stream_ptr())
+    _hip.check(rc, "madi_walk_y_f32")
+    torch.cuda.synchronize()
+    fl = flags.cpu().numpy()
+    return WalkYF32(Y, occ.cpu().numpy(), int(fl[_hip.MADI_FLAG_ESCAPED]), int(fl[_hip.MADI_FLAG_CLASSIFIER_ERR]),
+                    dict(zip(_hip.COUNTER_NAMES, cnt.cpu().numpy().tolist())))
+
+
 def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
     """Sigma cos / sigma sin per column from a device Y (FP64)."""
     torch = _hip.require_gpu()

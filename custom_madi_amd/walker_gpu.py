@@ -302,8 +302,6 @@ def _walk_and_reduce_one_ensemble(ens: Ensemble, pp: float, cfg: SimConfig, j_de
     chunk = int(cfg.walker_chunk or cfg.n_walkers)
     fused = engine.spec_from_columns(j_delta, j_Delta, j_sum, phase_coef)
     precision = 32 if cfg.precision == "fp32" else 64
-    if precision == 32 and fused is None:
-        raise ValueError("the FP32 production walk needs a fusable column set (<= 8 pairs x 16 b values)")
     geom = ens.device_geometry()
     runner = None
     for offset in range(0, int(cfg.n_walkers), chunk):
@@ -323,6 +321,17 @@ def _walk_and_reduce_one_ensemble(ens: Ensemble, pp: float, cfg: SimConfig, j_de
             s = res.sin.sum(axis=2)[col_pair, col_b]
             occ = res.occupancy
             counters = res.counters
+        elif precision == 32:
+            # any other column set: the FP32 walk writes Y, then the FP64 column reduce
+            out = engine.walk_y_f32(geom, params)
+            if out.n_error:
+                raise RuntimeError("GPU full-facet classification failed")
+            if out.n_escaped:
+                raise RuntimeError(f"SI §S.III fatal boundary condition: {out.n_escaped}/{n_this} walkers exited Ω_sim.")
+            c, s = engine.reduce_columns(out.Y, j_delta, j_Delta, j_sum, phase_coef)
+            occ = out.occupancy
+            counters = out.counters
+            del out
         else:
             out = engine.walk_y(geom, params)
             if np.any(out.err):

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 5
+#define MADI_ABI_VERSION 6
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -203,6 +203,21 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
                   const double* d_gold_u, double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err,
                   int64_t* d_counters, void* stream);
 
+/* Workspace bytes of madi_walk_y_f32 for n_walkers. */
+int madi_walk_y_f32_workspace(int64_t n_walkers, int64_t* bytes);
+
+/* The FP32 production walk (the walk of madi_walk_fused at precision 32:
+ * same random stream, labels and acceptance) writing Y instead of fused
+ * moments: d_Y n x n_grid x 3 FP64, Y(t_j) being the walker's compensated
+ * FP32 trapezoid sum widened to FP64 (sample 0 is zero).  With
+ * madi_reduce_f64 it serves any column set, the universal grid included.
+ * Replaces the Y output of _walk_kernel (madi/walker_gpu.py:520-656) on the
+ * production path; d_occupancy (n_grid), d_flags (MADI_N_FLAGS) and
+ * d_counters (may be NULL) as for madi_walk_fused.  classify_mode 0 or 4
+ * (split walk) or 3 (streamed walk). */
+int madi_walk_y_f32(const madi_geom* g, const madi_walk_params* p, void* d_workspace, int64_t workspace_bytes,
+                    double* d_Y, int64_t* d_occupancy, int64_t* d_flags, int64_t* d_counters, void* stream);
+
 /* Sigma cos / sigma sin over walkers and axes of pc*(Y[jd]+Y[jD]-Y[js]) per
  * column, FP64.  Replaces _reduce_kernel (madi/walker_gpu.py:658-695).  Every
  * jd/jD/js must lie in [0, n_grid); at most MADI_MAX_REDUCE_COLS columns.
@@ -306,6 +321,9 @@ typedef struct {
     int64_t* d_occupancy;
     int64_t* d_flags;
     void* d_moments;  /* the entry's madi_moments_bytes buffer; required by classify_mode 3 and 4 */
+    double* d_Y;      /* NULL, or the entry's Y (n_walkers x n_grid x 3 FP64, zeroed by the caller): each
+                         walker stores Y at samples 1.. (classify_mode 3 and 4; samples after an escape
+                         are not written) */
 } madi_work;
 
 typedef struct {

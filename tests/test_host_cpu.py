@@ -295,6 +295,30 @@ def test_hip_library_exports_every_declared_symbol():
     assert set(_hip.SIGNATURES) >= declared
 
 
+def test_y_walk_and_reduce_abi_argument_checks():
+    """madi_walk_y_f32's workspace size and argument errors, and the column
+    reduce's limits, which return before any device call."""
+    import ctypes as C
+    from custom_madi_amd import _hip
+    lib = _hip.load_library()
+    b0, b1 = C.c_int64(), C.c_int64()
+    assert lib.madi_walk_y_f32_workspace(0, C.byref(b0)) == _hip.MADI_OK and b0.value > 0
+    assert lib.madi_walk_y_f32_workspace(1000, C.byref(b1)) == _hip.MADI_OK and b1.value > b0.value
+    assert lib.madi_walk_y_f32_workspace(-1, C.byref(b0)) == _hip.MADI_ERR_ARG
+    assert lib.madi_walk_y_f32_workspace(10, None) == _hip.MADI_ERR_ARG
+    p = _hip.WalkParams()
+    p.n_walkers, p.n_steps, p.steps_per_h, p.n_grid = 10, 20, 10, 3
+    fake = 4096  # never dereferenced: each call below returns before touching memory
+    assert lib.madi_walk_y_f32(None, C.byref(p), fake, b1.value, fake, fake, fake, None, None) == _hip.MADI_ERR_ARG
+    assert lib.madi_walk_y_f32(fake, C.byref(p), fake, b1.value, None, fake, fake, None, None) == _hip.MADI_ERR_ARG
+    assert lib.madi_walk_y_f32(fake, C.byref(p), fake, 16, fake, fake, fake, None, None) == _hip.MADI_ERR_ARG
+    assert "smaller than" in _hip.last_error()
+    cs = C.c_double()
+    assert lib.madi_reduce_f64(fake, 10, 3, fake, fake, fake, fake, _hip.MADI_MAX_REDUCE_COLS + 1, fake, fake,
+                               None) == _hip.MADI_ERR_ARG
+    assert lib.madi_reduce_f64(fake, 10, 3, fake, fake, fake, fake, 0, C.byref(cs), C.byref(cs), None) == _hip.MADI_OK
+
+
 def test_matcher_and_finalize_abi_argument_checks():
     """Workspace sizes and argument errors of the matcher and tile-sum entry
     points, which return before any device call."""
