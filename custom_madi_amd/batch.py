@@ -66,6 +66,10 @@ class PreparedEntry:
     alpha_star: float
     L: float
     prep_s: float
+    cfg: Optional[SimConfig] = None  # the walk's configuration (default: req.config())
+
+    def walk_config(self) -> SimConfig:
+        return self.cfg if self.cfg is not None else self.req.config()
 
 
 def prepare_entry(req: EntryRequest) -> PreparedEntry:
@@ -123,18 +127,37 @@ class _Active:
     next_tile: int = 0
     admitted: float = 0.0
     device_bytes: int = 0
+    Y: object = None  # Y mode: the entry's (n_walkers, n_grid, 3) FP64 samples
 
 
 class DeviceScheduler:
     """Admission, generations and finalisation of entries on the current GPU."""
 
-    def __init__(self, spec: engine.PhaseSpec, grid_per_cu: int = 0, tier_cap_um: float = 0.0, scan_rounds: int = 0,
-                 classify_mode: int = 0,
+    def __init__(self, spec: Optional[engine.PhaseSpec], grid_per_cu: int = 0, tier_cap_um: float = 0.0,
+                 scan_rounds: int = 0, classify_mode: int = 0,
                  max_tiles_per_generation: int = 65536, hbm_reserve_bytes: int = 8 << 30,
-                 max_inflight: int = 2, use_graphs: bool = True, graph_capacity: int = 1024):
+                 max_inflight: int = 2, use_graphs: bool = True, graph_capacity: int = 1024,
+                 y_columns: Optional[tuple] = None):
+        """spec: the fused phase plan.  Y mode (y_columns = (j_delta, j_Delta,
+        j_sum, phase_coef), any number of columns; spec is then unused): each
+        entry's walkers store Y, and behind the entry's last generation the
+        FP64 column reduce (madi_reduce_f64) sums those columns; results hold
+        the column sums (n_columns,) summed over the three axes."""
         torch = _hip.require_gpu()
         self.torch = torch
         self.lib = _hip.load_library()
+        self.y_n = None
+        if y_columns is not None:
+            if int(classify_mode) not in (0, 3, 4):
+                raise ValueError("Y mode needs a streamed walk (classify_mode 0, 3 or 4)")
+            jd, jD, js, pc = y_columns
+            self.y_n = len(pc)
+            if self.y_n > _hip.MADI_MAX_REDUCE_COLS:
+                raise ValueError(f"{self.y_n} columns exceed one reduce call's {_hip.MADI_MAX_REDUCE_COLS}")
+            self.y_dev = tuple(torch.from_numpy(np.ascontiguousarray(a, dtype=dt)).cuda()
+                               for a, dt in ((jd, np.int32), (jD, np.int32), (js, np.int32), (pc, np.float64)))
+            self.y_max_grid = int(max(np.max(jd), np.max(jD), np.max(js))) + 1 if self.y_n else 0
+            spec = engine.single_column_spec()
         self.spec = spec
         self.plan = spec.to_ctypes()
         self.n_cols = int(np.prod(spec.shape))
@@ -183,7 +206,11 @@ class DeviceScheduler:
         a = prep.arrays
         geom = (a.nb_vec.nbytes + a.nb_ids.nbytes * 2 + a.nb_offsets.nbytes + a.cell_xyz.nbytes * 2
                 + a.hint.nbytes + a.cell_annulus.nbytes * 3)
-        return tiles * self.n_cols * 16 + self.moments_bytes(prep.req.n_walkers) + geom + 4096
+        y = 0
+        if self.y_n is not None:
+            # Y, and the reduce's grid-major copy of it
+            y = 2 * prep.req.n_walkers * prep.walk_config().n_grid * 24 + self.y_n * 16
+        return tiles * self.n_cols * 16 + self.moments_bytes(prep.req.n_walkers) + geom + y + 4096
 
     def moments_bytes(self, n_walkers: int) -> int:
         b = C.c_int64()
@@ -200,7 +227,7 @@ class DeviceScheduler:
             return False
         torch = self.torch
         req = prep.req
-        cfg = req.config()
+        cfg = prep.walk_config()
         geom = DeviceGeometry(prep.arrays)
         params = engine.walk_params(req.n_walkers, cfg, prep.pp, req.walk_seed)
         tiles = -(-req.n_walkers // _hip.TILE_WALKERS)
@@ -212,8 +239,13 @@ class DeviceScheduler:
         s = _hip.stream_ptr()
         _hip.check(self.lib.madi_fill_i64(_hip.dptr(occ), cfg.n_grid, 0, s), "madi_fill_i64")
         _hip.check(self.lib.madi_fill_i64(_hip.dptr(flags), _hip.N_FLAGS, 0, s), "madi_fill_i64")
+        Y = None
+        if self.y_n is not None:
+            if cfg.n_grid < self.y_max_grid:
+                raise ValueError(f"entry {req.key}: {cfg.n_grid} Y samples, the columns need {self.y_max_grid}")
+            Y = torch.zeros((req.n_walkers, cfg.n_grid, 3), dtype=torch.float64, device="cuda")
         self.active.append(_Active(prep, geom, params, tiles, partial, occ, flags, moments, admitted=time.perf_counter(),
-                                   device_bytes=self.entry_bytes(prep)))
+                                   device_bytes=self.entry_bytes(prep), Y=Y))
         return True
 
     # ---------------------------------------------------------- generations
@@ -242,6 +274,7 @@ class DeviceScheduler:
             w.d_occupancy = _hip.dptr(a.occ)
             w.d_flags = _hip.dptr(a.flags)
             w.d_moments = _hip.dptr(a.moments)
+            w.d_Y = _hip.dptr(a.Y) if a.Y is not None else None
             items.append(w)
             owners.append(a)
             a.next_tile += take
@@ -288,6 +321,8 @@ class DeviceScheduler:
         stream-ordered allocator may hand a freed block to a geometry uploaded
         on another stream before the launch stream reaches the free)."""
         torch = self.torch
+        if a.Y is not None:
+            return self._stage_y(a, s)
         n, g = self.n_cols, a.occ.numel()
         need = C.c_int64()
         _hip.check(self.lib.madi_tiles_finalize_workspace(C.byref(self.plan), a.n_tiles, C.byref(need)),
@@ -303,11 +338,28 @@ class DeviceScheduler:
         host[2 * n + g:].copy_(a.flags, non_blocking=True)
         return a, host
 
+    def _stage_y(self, a: _Active, s: int) -> tuple:
+        """Y mode's _stage: the column reduce of the entry's Y, then the same
+        host copies, all on the launch stream."""
+        torch = self.torch
+        n, g = self.y_n, a.occ.numel()
+        nw, G = a.Y.shape[0], a.Y.shape[1]
+        sums = torch.empty(2 * n, dtype=torch.float64, device="cuda")
+        base = _hip.dptr(sums)
+        jd, jD, js, pc = (_hip.dptr(t) for t in self.y_dev)
+        rc = self.lib.madi_reduce_f64(_hip.dptr(a.Y), nw, G, jd, jD, js, pc, n, base, base + 8 * n, s)
+        _hip.check(rc, "madi_reduce_f64")
+        host = torch.empty(2 * n + g + _hip.N_FLAGS, dtype=torch.int64, pin_memory=True)
+        host[:2 * n].view(torch.float64).copy_(sums, non_blocking=True)
+        host[2 * n:2 * n + g].copy_(a.occ, non_blocking=True)
+        host[2 * n + g:].copy_(a.flags, non_blocking=True)
+        return a, host
+
     def _result(self, a: _Active, host) -> EntryResult:
         """The entry's results from its staged host copy (its generation has completed)."""
-        n, g = self.n_cols, a.occ.numel()
+        n, g = (self.y_n if a.Y is not None else self.n_cols), a.occ.numel()
         h = host.numpy()
-        shape = self.spec.shape
+        shape = (n,) if a.Y is not None else self.spec.shape
         sums = h[:2 * n].view(np.float64)
         cos, sin = sums[:n].reshape(shape).copy(), sums[n:].reshape(shape).copy()
         occ, flags = h[2 * n:2 * n + g].copy(), h[2 * n + g:]

@@ -199,6 +199,13 @@ def walk_y_f32(geom: DeviceGeometry, params: _hip.WalkParams) -> WalkYF32:
                     dict(zip(_hip.COUNTER_NAMES, cnt.cpu().numpy().tolist())))
 
 
+def single_column_spec() -> "PhaseSpec":
+    """A one-column phase plan (pair (0, 0, 0), pc 0, one direction): the plan
+    of walks whose output is Y, whose single sum nobody reads."""
+    return PhaseSpec(j_delta=np.array([0]), j_Delta=np.array([0]), j_sum=np.array([0]), pc=np.zeros((1, 1)),
+                     dirs=np.array([[1.0, 0.0, 0.0]]))
+
+
 def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
     """Sigma cos / sigma sin per column from a device Y (FP64)."""
     torch = _hip.require_gpu()

@@ -13,6 +13,7 @@ library live in ``match.py``.
 from __future__ import annotations
 
 import json
+import os
 import time
 from collections import defaultdict
 from dataclasses import dataclass, field
@@ -283,12 +284,28 @@ def _entry_from_result(kio, rho, V, result, columns, cfg, entry_weights):
     )
 
 
+def _gpu_present() -> bool:
+    try:
+        import torch
+        return bool(torch.cuda.is_available())
+    except ImportError:
+        return False
+
+
 def build_library_from_triplets(triplets: List[Tuple[float, float, float]], cfg: SimConfig | None = None,
                                 save_path: Optional[str] = None, existing_library: Optional[list] = None,
                                 seed: int = 0, vi_min: float = 0.0, vi_max: float = VI_HARD_MAX,
                                 entry_weights: Optional[Mapping[tuple, float]] = None,
-                                grid_metadata: Optional[dict] = None, verbose: bool = True) -> list:
-    """Build or extend a library from (k_io, rho, V) triplets (madi/library.py:445-642)."""
+                                grid_metadata: Optional[dict] = None, verbose: bool = True,
+                                batched: Optional[bool] = None, pool=None) -> list:
+    """Build or extend a library from (k_io, rho, V) triplets (madi/library.py:445-642).
+
+    ``batched`` (default: whenever the configuration allows it) runs every
+    walk of every group on the batched device scheduler (library_batch),
+    with the same entries, order and checkpoints as the group-by-group loop;
+    ``pool`` (a worker pool from serve.make_pool) builds group geometry in
+    worker processes (each measures v_i with the HIP classifier, so each
+    holds a HIP context of its own)."""
     if cfg is None:
         cfg = SimConfig()
     if int(cfg.n_ensembles) < 2:
@@ -313,7 +330,22 @@ def build_library_from_triplets(triplets: List[Tuple[float, float, float]], cfg:
     for k, r, v in todo:
         groups[(r, v)].append(k)
     t0 = time.time()
-    for (rho, V), kios in sorted(groups.items(), key=lambda kv: kv[0][0] * kv[0][1]):
+    ordered = sorted(groups.items(), key=lambda kv: kv[0][0] * kv[0][1])
+    from . import library_batch
+    if batched is None:
+        batched = library_batch.batchable(cfg, columns) and _gpu_present()
+    if batched:
+        cp = library_batch.LibraryCheckpoint(save_path, cfg, columns, library, grid_metadata=grid_metadata,
+                                             build_seed=seed)
+        try:
+            library_batch.build_groups_batched([(rv, sorted(ks)) for rv, ks in ordered], cfg, columns, seed, cp,
+                                               entry_weights=entry_weights, pool=pool, verbose=verbose)
+        finally:
+            library[:] = cp.library
+        if verbose:
+            print(f"Library: {len(library)} entries total ({len(todo)} new in {time.time() - t0:.0f}s, batched)")
+        return library
+    for (rho, V), kios in ordered:
         kios = sorted(kios)
         results = sig.compute_signals_multi_kio(rho, V, kios, cfg, columns=columns, seed=seed, verbose=False)
         for kio in kios:
@@ -572,6 +604,63 @@ def load_library_meta(path: str) -> dict:
     meta.update(delta_pairs=[(small, D) for D in deltas], n_b=n_b, b_values=bv, small_delta=small, h_ms=None,
                 has_v5_diagnostics=False)
     return meta
+
+
+def merge_library_shards(paths: List[str], out_path: str, require_shards: Optional[int] = None,
+                         verbose: bool = True) -> list:
+    """Merge per-shard libraries into one file (scripts/merge_shards.py:54-132).
+
+    Every shard must carry the first shard's (delta, Delta) pairs, b values,
+    schema, build seed, grid metadata and v5 uncertainty contract.  Entries
+    keep shard order; a (k_io, rho, V) key already taken (rounded to 4, 1 and
+    6 decimals; the free-water atom is one key) is skipped, first occurrence
+    winning.  With require_shards, the file names must carry shard ids
+    0..require_shards-1 (``.shard<k>``) exactly once.  The merged file keeps
+    the first shard's build metadata and grid."""
+    import re
+    if not paths:
+        raise ValueError("no shard files given")
+    if require_shards is not None:
+        found = []
+        for p in paths:
+            m = re.search(r"\.shard(\d+)(?:\.|$)", os.path.basename(p))
+            if m is None:
+                raise ValueError(f"cannot infer shard id from {p}")
+            found.append(int(m.group(1)))
+        if set(found) != set(range(require_shards)) or len(found) != len(set(found)):
+            raise ValueError(f"shard coverage mismatch; found={sorted(found)}, expected={list(range(require_shards))}")
+    first = load_library_meta(paths[0])
+    fb = first.get("build_metadata") or {}
+
+    def contract(meta):
+        b = meta.get("build_metadata") or {}
+        return (meta["delta_pairs"], meta["b_values"], meta.get("library_schema"), b.get("build_seed"),
+                json.dumps(b.get("grid", {}), sort_keys=True), json.dumps(b.get("uncertainty", {}), sort_keys=True))
+
+    want = contract(first)
+    merged, seen = [], set()
+    for p in paths:
+        if contract(load_library_meta(p)) != want:
+            raise ValueError(f"shard metadata/schema/uncertainty mismatch: {p}")
+        lib = load_library(p)
+        before = len(merged)
+        for e in lib:
+            key = ("free_water",) if e.is_free_water else (round(e.kio, 4), round(e.rho, 1), round(e.V, 6))
+            if key in seen:
+                continue
+            seen.add(key)
+            merged.append(e)
+        if verbose:
+            print(f"  {p}: +{len(merged) - before} new  ({len(lib)} in file)")
+    pairs = [tuple(x) for x in first["delta_pairs"]]
+    empty = np.array([], dtype=np.int32)
+    columns = sig.ColumnGrid(delta_pairs=pairs, b_values=np.asarray(first["b_values"], dtype=float), j_delta=empty,
+                             j_Delta=empty, j_sum=empty, phase_coef=np.array([], dtype=np.float64),
+                             n_pairs=len(pairs), n_b=int(first["n_b"]))
+    cfg = SimConfig(h_ms=first.get("h_ms") or 1.0)
+    os.makedirs(os.path.dirname(os.path.abspath(out_path)) or ".", exist_ok=True)
+    _save_library(merged, out_path, cfg=cfg, columns=columns, build_metadata=fb or None)
+    return merged
 
 
 def library_summary(lib: list, meta: dict | None = None):
