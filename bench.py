@@ -25,7 +25,9 @@ resident on the device.  The same line carries:
 
 Other workloads (--workload): c2, c3 on their own, c4 (a seeded sample of the
 full-range sweep), map (MAP matching of 1e6 voxels against a 14,105-entry
-library, FP64), reduce (the column reduce alone, with its CPU baseline).  --gpus N > 1 exits: this benchmark uses one GPU.
+library, FP64), reduce (the column reduce alone, with its CPU baseline),
+library (the drop-in library build at the default configuration, group by
+group against the batched scheduler).  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
 from __future__ import annotations
@@ -440,6 +442,63 @@ def run_reduce(args):
     return 0
 
 
+def run_library(args):
+    """The drop-in library build (build_library_from_triplets) at the default
+    production configuration, group by group and on the batched scheduler."""
+    import tempfile
+    from custom_madi_amd import _hip, library, serve
+    from custom_madi_amd.config import SimConfig
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    # ensemble construction for the batched build runs in worker processes,
+    # spawned before this process touches the GPU
+    pool = serve.make_pool(args.workers) if args.workers > 0 else None
+    if pool is not None:
+        pool.map(_noop, range(args.workers))
+    _hip.require_gpu()
+    cfg = SimConfig(precision="fp32")
+    groups = [(5.0e5, 1.2), (2.0e5, 3.0), (1.0e5, 6.0)][:max(1, args.library_groups)]
+    kios = (5.0, 22.0, 60.0)[:max(1, args.library_kios)]
+    triplets = [(k, r, v) for r, v in groups for k in kios]
+    n_walks = len(triplets) * cfg.n_ensembles
+    steps = n_walks * cfg.n_walkers * cfg.n_steps
+    with tempfile.TemporaryDirectory() as d:
+        times = {}
+        for mode in ("sequential", "batched"):
+            log(f"library: {len(triplets)} entries, {mode}")
+            t0 = time.perf_counter()
+            library.build_library_from_triplets(triplets, cfg=cfg, seed=11, save_path=f"{d}/{mode}.npz",
+                                                verbose=False, batched=(mode == "batched"),
+                                                pool=pool if mode == "batched" else None)
+            times[mode] = time.perf_counter() - t0
+            log(f"library: {mode} {times[mode]:.1f} s")
+        a, b = np.load(f"{d}/sequential.npz"), np.load(f"{d}/batched.npz")
+        same = sorted(a.files) == sorted(b.files) and all(
+            np.array_equal(a[k], b[k], equal_nan=a[k].dtype.kind in "fc") for k in a.files)
+    if pool is not None:
+        pool.close()
+        pool.join()
+    eph = {m: len(triplets) * 3600.0 / t for m, t in times.items()}
+    out = {
+        "metric": "library entries/h", "value": eph["batched"], "unit": "entries/h", "n_gpus": 1, "steps": 1,
+        "warmup": 0, "ms_per_step": 1e3 * times["batched"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 (fp64 sums)", "data": "synthetic",
+        "config": {"workload": f"library: build_library_from_triplets, {len(triplets)} entries ({len(groups)} "
+                               f"(rho, V) groups x {len(kios)} k_io), default configuration: {cfg.n_ensembles} "
+                               f"ensembles x {cfg.n_walkers} walkers x {cfg.n_steps} steps, "
+                               f"{len(cfg.delta_pairs()) * len(cfg.b_values)} columns (v5)",
+                   "walks": n_walks, "walker_steps": steps},
+        "sequential": {"wall_s": times["sequential"], "entries_per_hour": eph["sequential"],
+                       "walker_steps_per_s": steps / times["sequential"]},
+        "batched": {"wall_s": times["batched"], "entries_per_hour": eph["batched"],
+                    "walker_steps_per_s": steps / times["batched"]},
+        "batched_over_sequential": times["sequential"] / times["batched"],
+        "files_equal": bool(same),
+        "host_workers": {"sequential": 0, "batched": int(args.workers)},
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def _prepared_lookup(preps):
     table = {p.req.key: p for p in preps}
     return lambda req: table[req.key]
@@ -556,7 +615,7 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target seconds per CPU baseline sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce", "library"], default="c1")
     ap.add_argument("--voxels", type=int, default=1_000_000, help="map: voxels matched per step")
     ap.add_argument("--library-entries", type=int, default=14_105, help="map: library entries")
     ap.add_argument("--entries", type=int, default=256)
@@ -578,6 +637,8 @@ def main(argv=None):
     ap.add_argument("--no-c2", action="store_true", help="default line without the c2 object")
     ap.add_argument("--no-c3", action="store_true", help="default line without the c3 object")
     ap.add_argument("--no-reduce", action="store_true", help="default line without the universal_reduce object")
+    ap.add_argument("--library-groups", type=int, default=2, help="library: (rho, V) groups (at most 3)")
+    ap.add_argument("--library-kios", type=int, default=3, help="library: k_io values per group (at most 3)")
     ap.add_argument("--library-out", default=None)
     args = ap.parse_args(argv)
     if args.gpus != 1:
@@ -595,6 +656,8 @@ def main(argv=None):
         return run_map(args)
     if args.workload == "reduce":
         return run_reduce(args)
+    if args.workload == "library":
+        return run_library(args)
 
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     pool = None

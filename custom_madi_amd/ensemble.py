@@ -365,6 +365,11 @@ class Ensemble:
             self._device = DeviceGeometry(self)
         return self._device
 
+    def kernel_arrays(self) -> "GeometryArrays":
+        """The arrays the kernels take: those of the device geometry when it
+        has been built (the v_i measurement builds it), else derived now."""
+        return self._device.arrays if self._device is not None else geometry_arrays(self)
+
     def classify(self, positions: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
         """Exact SI Eq. S2 labels (cell ids, inside flags) from the HIP classifier."""
         pts = np.asarray(positions, dtype=np.float64)
@@ -410,11 +415,30 @@ def _measure_vi(ens: Ensemble, n: int, rng: np.random.Generator) -> Tuple[float,
     return vi, se
 
 
-def create_ensemble(rho: float, V: float, cfg: SimConfig | None = None, seed: int | None = None,
-                    verbose: bool = False, verify_vi: bool = True) -> Ensemble:
-    """SI-conformant ensemble for requested (rho, V) (madi/ensemble.py:649-757)."""
-    if cfg is None:
-        cfg = SimConfig()
+@dataclass
+class EnsembleDraft:
+    """An ensemble before its v_i measurement: everything create_ensemble
+    computes on the CPU, and the generator in the state the measurement
+    draws its points from."""
+
+    ens: "Ensemble"
+    rng: np.random.Generator
+    target_vi: float
+    rho_real: float
+    alpha_star: float
+    mean_av: float
+    annulus_sim: np.ndarray
+    n_seeds: int
+    n_sim: int
+    side: float
+    source_side: float
+    population: object
+    reference: object
+
+
+def build_ensemble_host(rho: float, V: float, cfg: SimConfig, seed: int | None = None) -> EnsembleDraft:
+    """The CPU part of create_ensemble (population, KD tree, annulus); no GPU
+    call, so it may run in a worker process."""
     cfg.assert_grid_alignment()
     rho_um3 = _rho_um3(rho)
     target_vi = rho_um3 * _V_um3(V)
@@ -444,23 +468,47 @@ def create_ensemble(rho: float, V: float, cfg: SimConfig | None = None, seed: in
         rho_requested=float(rho), V_requested=float(V), kd_node_seed=nodes[0], kd_node_axis=nodes[1],
         kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], tree=tree,
     )
-    realised_vi, vi_se = _measure_vi(ens, cfg.geometry_validation_points, rng)
+    return EnsembleDraft(ens, rng, target_vi, rho_real, alpha_star, mean_av, annulus[in_sim], len(seeds), n_sim,
+                         side, source_side, population, reference)
+
+
+def finish_ensemble(draft: EnsembleDraft, cfg: SimConfig, verbose: bool = False, verify_vi: bool = True,
+                    arrays: Optional["GeometryArrays"] = None) -> "Ensemble":
+    """The GPU part of create_ensemble: v_i measured with the HIP classifier,
+    the SI acceptance test, the final geometry statistics.  arrays: the
+    ensemble's geometry_arrays computed elsewhere (uploaded instead of
+    derived again)."""
+    ens = draft.ens
+    if arrays is not None:
+        ens._device = DeviceGeometry(arrays)
+    realised_vi, vi_se = _measure_vi(ens, cfg.geometry_validation_points, draft.rng)
     allowed = max(float(cfg.geometry_vi_tolerance), 4.0 * vi_se)
-    if verify_vi and abs(realised_vi - target_vi) > allowed:
+    if verify_vi and abs(realised_vi - draft.target_vi) > allowed:
         raise RuntimeError(
             "Finite Ω_sim packing failed SI geometry acceptance: "
-            f"target={target_vi:.6f}, measured={realised_vi:.6f}, allowed={allowed:.6f}."
+            f"target={draft.target_vi:.6f}, measured={realised_vi:.6f}, allowed={allowed:.6f}."
         )
+    side, n_sim = draft.side, draft.n_sim
     V_real = realised_vi * side ** 3 / n_sim / 1.0e3
-    ens.geometry = _geometry_stats(target_vi, realised_vi, vi_se, rho_real, V_real, alpha_star, mean_av,
-                                   annulus[in_sim], len(seeds), n_sim, cfg.geometry_validation_points, side,
-                                   source_side, population, reference)
+    ens.geometry = _geometry_stats(draft.target_vi, realised_vi, vi_se, draft.rho_real, V_real, draft.alpha_star,
+                                   draft.mean_av, draft.annulus_sim, draft.n_seeds, n_sim,
+                                   cfg.geometry_validation_points, side, draft.source_side, draft.population,
+                                   draft.reference)
     ens.V = V_real
     ens.vi = realised_vi
     if verbose:
-        print(f"    [SI geometry] W={side:.3f} um, seeds(sim/pop)={n_sim}/{len(seeds)}, target v_i={target_vi:.5f}, "
-              f"realised={realised_vi:.5f}±{vi_se:.5f}, alpha*={alpha_star:.5f} um, <A/V>={mean_av:.5f} um^-1")
+        print(f"    [SI geometry] W={side:.3f} um, seeds(sim/pop)={n_sim}/{draft.n_seeds}, "
+              f"target v_i={draft.target_vi:.5f}, realised={realised_vi:.5f}±{vi_se:.5f}, "
+              f"alpha*={draft.alpha_star:.5f} um, <A/V>={draft.mean_av:.5f} um^-1")
     return ens
+
+
+def create_ensemble(rho: float, V: float, cfg: SimConfig | None = None, seed: int | None = None,
+                    verbose: bool = False, verify_vi: bool = True) -> Ensemble:
+    """SI-conformant ensemble for requested (rho, V) (madi/ensemble.py:649-757)."""
+    if cfg is None:
+        cfg = SimConfig()
+    return finish_ensemble(build_ensemble_host(rho, V, cfg, seed), cfg, verbose=verbose, verify_vi=verify_vi)
 
 
 def create_dummy_ensemble(cfg: SimConfig) -> Ensemble:

@@ -303,9 +303,9 @@ def build_library_from_triplets(triplets: List[Tuple[float, float, float]], cfg:
     ``batched`` (default: whenever the configuration allows it) runs every
     walk of every group on the batched device scheduler (library_batch),
     with the same entries, order and checkpoints as the group-by-group loop;
-    ``pool`` (a worker pool from serve.make_pool) builds group geometry in
-    worker processes (each measures v_i with the HIP classifier, so each
-    holds a HIP context of its own)."""
+    ``pool`` (a worker pool from serve.make_pool) builds the ensembles'
+    CPU geometry in worker processes; their v_i measurement (the HIP
+    classifier) runs in this process."""
     if cfg is None:
         cfg = SimConfig()
     if int(cfg.n_ensembles) < 2:

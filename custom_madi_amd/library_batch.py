@@ -48,7 +48,7 @@ import numpy as np
 from . import _hip, batch, engine
 from . import signal as sig
 from .config import SimConfig
-from .ensemble import GeometryArrays, geometry_arrays
+from .ensemble import GeometryArrays
 
 
 @dataclass
@@ -61,18 +61,48 @@ class GroupEnsemble:
     geometry: dict
 
 
+def prepare_ensemble(rho: float, V: float, cfg: SimConfig, seed: int, index: int) -> GroupEnsemble:
+    """Ensemble `index` of one (rho, V) group, as the sequential build
+    constructs it (walker_gpu._build_ensemble_for_entry, the same seed),
+    reduced to the arrays its walks upload (those DeviceGeometry(ens) holds on
+    the sequential path) and what the entry metadata needs.  Building a
+    cellular ensemble measures v_i with the HIP classifier, so a worker
+    process running this holds a HIP context of its own."""
+    from .walker_gpu import _build_ensemble_for_entry
+    ens = _build_ensemble_for_entry(rho, V, cfg, seed, index)
+    return GroupEnsemble(ens.kernel_arrays(), float(ens.mean_AV), bool(ens.is_free_water), ens.geometry.to_dict())
+
+
+def prepare_ensemble_host(rho: float, V: float, cfg: SimConfig, seed: int, index: int) -> tuple:
+    """The CPU part of prepare_ensemble, for a worker process with no GPU:
+    ("done", GroupEnsemble) for the free-water and periodic ensembles, whose
+    construction is CPU only; ("draft", EnsembleDraft, arrays) for an Eq. S8
+    ensemble, whose v_i measurement finish_prepared runs on the GPU."""
+    from .ensemble import build_ensemble_host, geometry_arrays
+    from .walker_gpu import _build_ensemble_for_entry, _ensemble_seed
+    if rho > 0.0 and V > 0.0 and not cfg.periodic:
+        draft = build_ensemble_host(rho, V, cfg, _ensemble_seed(seed, index))
+        return ("draft", draft, geometry_arrays(draft.ens))
+    ens = _build_ensemble_for_entry(rho, V, cfg, seed, index)
+    return ("done", GroupEnsemble(ens.kernel_arrays(), float(ens.mean_AV), bool(ens.is_free_water),
+                                  ens.geometry.to_dict()))
+
+
+def finish_prepared(item: tuple, cfg: SimConfig) -> GroupEnsemble:
+    """The rest of prepare_ensemble for a prepare_ensemble_host result (in the
+    process that holds the GPU): the v_i measurement and acceptance test of
+    create_ensemble, on the arrays the worker derived."""
+    if item[0] == "done":
+        return item[1]
+    from .ensemble import finish_ensemble
+    _, draft, arrays = item
+    ens = finish_ensemble(draft, cfg, arrays=arrays)
+    return GroupEnsemble(arrays, float(ens.mean_AV), bool(ens.is_free_water), ens.geometry.to_dict())
+
+
 def prepare_group(rho: float, V: float, cfg: SimConfig, seed: int) -> List[GroupEnsemble]:
-    """Host geometry of one (rho, V) group: the ensembles the sequential build
-    constructs (walker_gpu._build_ensembles_for_entry, the same seeds), reduced
-    to their device arrays and what the entry metadata needs.  CPU only, so it may run
-    in a worker process."""
-    from .walker_gpu import _build_ensembles_for_entry
-    out = []
-    for ens in _build_ensembles_for_entry(rho, V, cfg, seed, False):
-        # the arrays DeviceGeometry(ens) uploads on the sequential path
-        out.append(GroupEnsemble(geometry_arrays(ens), float(ens.mean_AV), bool(ens.is_free_water),
-                                 ens.geometry.to_dict()))
-    return out
+    """Every ensemble of one (rho, V) group (prepare_ensemble in index order)."""
+    return [prepare_ensemble(rho, V, cfg, seed, i) for i in range(int(cfg.n_ensembles))]
 
 
 def batchable(cfg: SimConfig, columns) -> bool:
@@ -156,8 +186,12 @@ class LibraryBuilder:
         g = _Group(self.next_index, float(rho), float(V), kios, time.perf_counter())
         self.groups[g.index] = g
         self.next_index += 1
-        job = self.pool.apply_async(self.prepare, (g.rho, g.V, self.cfg, self.seed)) if self.pool else None
-        self.preparing.append((g, job))
+        jobs = None
+        if self.pool is not None:
+            # one job per ensemble: a group's ensembles are built in parallel
+            jobs = [self.pool.apply_async(prepare_ensemble_host, (g.rho, g.V, self.cfg, self.seed, i))
+                    for i in range(int(self.cfg.n_ensembles))]
+        self.preparing.append((g, jobs))
         return g.index
 
     def _walks_of(self, g: _Group) -> List[batch.PreparedEntry]:
@@ -178,17 +212,17 @@ class LibraryBuilder:
 
     def _collect_prepared(self, block: bool) -> None:
         while self.preparing:
-            g, job = self.preparing[0]
-            if job is None:
+            g, jobs = self.preparing[0]
+            if jobs is None:
                 # built here: one group per step, so the device walks the
                 # groups already admitted meanwhile
                 g.ensembles = self.prepare(g.rho, g.V, self.cfg, self.seed)
                 self.preparing.popleft()
                 self.waiting.extend(self._walks_of(g))
                 return
-            if not block and not job.ready():
+            if not block and not all(j.ready() for j in jobs):
                 return
-            g.ensembles = job.get()
+            g.ensembles = [finish_prepared(j.get(), self.cfg) for j in jobs]
             self.preparing.popleft()
             self.waiting.extend(self._walks_of(g))
             block = False  # at most one blocking wait per step

@@ -391,18 +391,23 @@ def _build_ensembles_for_entry(rho: float, V: float, cfg: SimConfig, seed: int, 
     With ``cfg.periodic`` the ensembles are periodic boxes of side cfg.L
     holding round(rho * L^3) cells.
     """
+    return [_build_ensemble_for_entry(rho, V, cfg, seed, i, verbose) for i in range(int(cfg.n_ensembles))]
+
+
+def _build_ensemble_for_entry(rho: float, V: float, cfg: SimConfig, seed: int, index: int,
+                              verbose: bool = False) -> Ensemble:
+    """Ensemble `index` of one (rho, V) entry: each ensemble depends only on
+    its own seed, so they may be built apart (in parallel)."""
     if rho <= 0.0 or V <= 0.0:
-        return [create_dummy_ensemble(cfg) for _ in range(int(cfg.n_ensembles))]
+        return create_dummy_ensemble(cfg)
     if cfg.periodic:
         if cfg.L is None:
             raise ValueError("a periodic geometry needs cfg.L")
         n_cells = max(2, int(round(rho * 1e-9 * cfg.L ** 3)))
         rho_box = n_cells / cfg.L ** 3 * 1e9
         vi = rho * V * 1e-6
-        return [create_periodic_ensemble(n_cells, rho_box, vi, cfg, seed=_ensemble_seed(seed, i))
-                for i in range(int(cfg.n_ensembles))]
-    return [create_ensemble(rho, V, cfg, seed=_ensemble_seed(seed, i), verbose=verbose)
-            for i in range(int(cfg.n_ensembles))]
+        return create_periodic_ensemble(n_cells, rho_box, vi, cfg, seed=_ensemble_seed(seed, index))
+    return create_ensemble(rho, V, cfg, seed=_ensemble_seed(seed, index), verbose=verbose)
 
 
 def run_simulation_reduced(rho: float, V: float, kio: float, j_delta, j_Delta, j_sum, phase_coef,

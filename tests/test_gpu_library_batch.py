@@ -4,7 +4,8 @@ against the sequential group-by-group build (madi/library.py:546-629).
 * build_library_from_triplets with batched=True gives the sequential
   build's entries bit for bit, in the same order, and the same v5 file.
   FP32, v5 grid (so the scheduler's Y mode), two ensembles, two walker
-  chunks per walk.
+  chunks per walk; also with the ensembles' CPU geometry prepared apart
+  from their GPU v_i measurement (the worker-pool split).
 * A group with a failed walk raises the sequential path's RuntimeError
   ("fatal boundary condition") after the groups before it are
   checkpointed, as the sequential loop does.
@@ -78,6 +79,35 @@ def test_batched_build_equals_sequential(sequential):
     back = library.load_library(str(d / "bat.npz"))
     assert library.load_library_meta(str(d / "bat.npz"))["has_v5_diagnostics"]
     for a, b in zip(back, library.load_library(str(d / "seq.npz"))):
+        _same_entry(a, b)
+
+
+class _InlinePool:
+    """apply_async runs the job at once in this process: the worker/GPU split
+    of ensemble construction (prepare_ensemble_host in a worker,
+    finish_prepared here) without spawning processes."""
+
+    class _Done:
+        def __init__(self, value):
+            self.value = value
+
+        def ready(self):
+            return True
+
+        def get(self):
+            return self.value
+
+    def apply_async(self, fn, args):
+        return self._Done(fn(*args))
+
+
+def test_batched_build_with_worker_prepared_geometry_equals_sequential(sequential):
+    d, seq = sequential
+    say("[libbatch] batched build, ensembles prepared by the worker split")
+    bat = library.build_library_from_triplets(TRIPLETS, cfg=_cfg(), seed=7, save_path=str(d / "pool.npz"),
+                                              verbose=False, batched=True, pool=_InlinePool())
+    assert len(bat) == len(seq)
+    for a, b in zip(seq, bat):
         _same_entry(a, b)
 
 
