@@ -999,6 +999,61 @@ int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes)
     return MADI_OK;
 }
 
+int madi_kd_build(const double* pts, int64_t n, int32_t* node_seed, int8_t* node_axis, int32_t* left, int32_t* right,
+                  int32_t* parent)
+{
+    if (n < 0 || n > INT32_MAX || (n > 0 && (!pts || !node_seed || !node_axis || !left || !right || !parent))) {
+        set_error("madi_kd_build: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    std::vector<int32_t> idx((size_t)n);
+    for (int64_t i = 0; i < n; ++i) {
+        idx[(size_t)i] = (int32_t)i;
+        left[i] = right[i] = parent[i] = -1;
+    }
+    // subsets [lo, hi) of idx with their parent node and side (0 left, 1
+    // right, -1 root); right pushed before left, so ids come out in pre-order
+    struct Item {
+        int64_t lo, hi;
+        int32_t par;
+        int32_t side;
+    };
+    std::vector<Item> stack;
+    stack.push_back({0, n, -1, -1});
+    int32_t next = 0;
+    while (!stack.empty()) {
+        const Item it = stack.back();
+        stack.pop_back();
+        if (it.hi == it.lo) continue;
+        double mn[3] = {INFINITY, INFINITY, INFINITY}, mx[3] = {-INFINITY, -INFINITY, -INFINITY};
+        for (int64_t i = it.lo; i < it.hi; ++i) {
+            const double* p = pts + 3 * (int64_t)idx[(size_t)i];
+            for (int d = 0; d < 3; ++d) {
+                mn[d] = std::min(mn[d], p[d]);
+                mx[d] = std::max(mx[d], p[d]);
+            }
+        }
+        int ax = 0;
+        for (int d = 1; d < 3; ++d)
+            if (mx[d] - mn[d] > mx[ax] - mn[ax]) ax = d;
+        const int64_t mid = (it.hi - it.lo) / 2;
+        std::nth_element(idx.begin() + it.lo, idx.begin() + it.lo + mid, idx.begin() + it.hi,
+                         [&](int32_t u, int32_t v) {
+                             const double a = pts[3 * (int64_t)u + ax], b = pts[3 * (int64_t)v + ax];
+                             return a < b || (a == b && u < v);
+                         });
+        const int32_t node = next++;
+        node_seed[node] = idx[(size_t)(it.lo + mid)];
+        node_axis[node] = (int8_t)ax;
+        parent[node] = it.par;
+        if (it.side == 0) left[it.par] = node;
+        else if (it.side == 1) right[it.par] = node;
+        stack.push_back({it.lo + mid + 1, it.hi, node, 1});
+        stack.push_back({it.lo, it.lo + mid, node, 0});
+    }
+    return MADI_OK;
+}
+
 int madi_classify(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside, int8_t* d_ok,
                   void* stream)
 {

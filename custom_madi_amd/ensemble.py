@@ -223,43 +223,26 @@ def governing_mean_A_over_V(vi: float, rho_per_uL: float, cfg: SimConfig) -> Tup
 # ------------------------------------------------------------------ KD tree
 
 def build_kdtree(points: np.ndarray) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
-    """Balanced KD tree in pre-order (madi/ensemble.py:525-563).
+    """Balanced KD tree in pre-order (madi/ensemble.py:525-563), built by the
+    native madi_kd_build (host code of the HIP library, no device call).
 
     Each node splits its subset on the axis of largest extent at the element
     of rank len//2 along that axis; the node ids are assigned in pre-order
     (root 0, left subtree before right).  The arrays are a function of the
-    point set only, so they equal the reference's for the same seeds.
+    point set only, so they equal the reference's for the same seeds (whose
+    split coordinates are distinct; equal coordinates are ordered by index).
     """
-    pts = np.asarray(points, dtype=np.float64)
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
     n = len(pts)
     seed = np.empty(n, dtype=np.int32)
     axis = np.empty(n, dtype=np.int8)
-    left = np.full(n, -1, dtype=np.int32)
-    right = np.full(n, -1, dtype=np.int32)
-    parent = np.full(n, -1, dtype=np.int32)
-    next_id = 0
-    # stack of (indices, parent node, side: 0 left / 1 right / -1 root)
-    stack = [(np.arange(n, dtype=np.int64), -1, -1)]
-    while stack:
-        idx, par, side = stack.pop()
-        if len(idx) == 0:
-            continue
-        sub = pts[idx]
-        ax = int(np.argmax(sub.max(axis=0) - sub.min(axis=0)))
-        mid = len(idx) // 2
-        order = np.argpartition(sub[:, ax], mid)
-        arranged = idx[order]
-        node = next_id
-        next_id += 1
-        seed[node] = arranged[mid]
-        axis[node] = ax
-        parent[node] = par
-        if side == 0:
-            left[par] = node
-        elif side == 1:
-            right[par] = node
-        stack.append((arranged[mid + 1:], node, 1))
-        stack.append((arranged[:mid], node, 0))
+    left = np.empty(n, dtype=np.int32)
+    right = np.empty(n, dtype=np.int32)
+    parent = np.empty(n, dtype=np.int32)
+    lib = _hip.load_library()
+    rc = lib.madi_kd_build(pts.ctypes.data, n, seed.ctypes.data, axis.ctypes.data, left.ctypes.data,
+                           right.ctypes.data, parent.ctypes.data)
+    _hip.check(rc, "madi_kd_build")
     if n and parent[0] != -1:
         raise RuntimeError("KD tree root must occupy node zero")
     return seed, axis, left, right, parent

@@ -146,6 +146,17 @@ int madi_geom_destroy_async(madi_geom* g, void* stream);
 /* Device bytes held by the handle (admission control). */
 int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes);
 
+/* ---------------------------------------------------------- host geometry */
+
+/* Balanced KD tree of n points (pts: n x 3 FP64, host memory) in pre-order:
+ * each node splits its subset on the axis of largest extent (the first on
+ * ties) at the element of rank (size / 2) along that axis (equal coordinates
+ * ordered by point index), its left subset before its right.  Outputs (host,
+ * n each): node_seed, node_axis, left, right, parent (-1: none).  Replaces
+ * build_kdtree (madi/ensemble.py:525-563); runs on the CPU, no device call. */
+int madi_kd_build(const double* pts, int64_t n, int32_t* node_seed, int8_t* node_axis, int32_t* left, int32_t* right,
+                  int32_t* parent);
+
 /* --------------------------------------------------------- classifiers */
 
 /* Exact SI Eq. S2 label of n points (d_pts: n x 3 FP64).  FP64 stackless KD
