@@ -107,6 +107,26 @@ def test_uniform_and_normal_follow_the_numba_definitions():
     assert np.allclose(z, ref_z, rtol=1e-15, atol=1e-15)
 
 
+def test_normal_draws_are_standard_normal():
+    """The oracle's normals (numba's Box-Muller on float32 uniforms) pinned by
+    what the mathematics fixes, not by their formula: 2e6 draws have mean 0
+    and variance 1 within 5 standard errors, fourth moment 3 within 5 SE,
+    pass a Kolmogorov-Smirnov test against N(0, 1), and successive draws are
+    uncorrelated.  A dropped factor (sqrt(-log u) instead of sqrt(-2 log u)),
+    log10 for log, or a missing square root each fails (checked on the CPU).
+    The angle's period is not visible here (cos of a uniform angle on [0, pi)
+    is distributed as on [0, 2 pi)); the formula test above pins it."""
+    from scipy import stats
+    st = O.lib().og_seed_state(2024)
+    z = np.asarray(O.draws(st.s0, st.s1, 2, 2_000_000), dtype=np.float64)
+    n = z.size
+    assert abs(z.mean()) <= 5.0 / np.sqrt(n)
+    assert abs(z.var() - 1.0) <= 5.0 * np.sqrt(2.0 / n)
+    assert abs(np.mean(z ** 4) - 3.0) <= 5.0 * np.sqrt(96.0 / n)   # Var(z^4) = 105 - 9
+    assert stats.kstest(z, "norm").pvalue > 1e-4
+    assert abs(np.corrcoef(z[:-1], z[1:])[0, 1]) <= 5.0 / np.sqrt(n)
+
+
 # ----------------------------------------------------------------- classify
 
 def test_kd_classifier_matches_all_seed_brute_force():
