@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstring>
 #include <algorithm>
+#include <memory>
 #include <mutex>
 #include <utility>
 #include <vector>
@@ -148,11 +149,24 @@ static int get_jump_pow(const Gf2Col** out)
 
 }  // namespace madi
 
+// An event recorded after a launch: complete once the launch (and all
+// earlier work on its stream) has finished.  Shared by every geometry the
+// launch reads; destroyed with its last reference.
+struct UseEvent {
+    hipEvent_t ev = nullptr;
+    ~UseEvent()
+    {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+
 struct madi_geom {
     madi::GeomDev dev;
     std::vector<void*> allocs;
     int64_t bytes = 0;
     int device = 0;
+    // the last launch that reads this geometry (launches outside graph capture)
+    mutable std::shared_ptr<UseEvent> used;
 };
 
 namespace madi {
@@ -195,6 +209,31 @@ static int upload(madi_geom* g, const T* host, size_t count, const T** out)
 
 // Release after all work on the device (a plain destroy: whatever stream
 // used the geometry has finished with it).
+// Records on s an event behind everything queued there so far and makes it
+// the last use of each geometry in gs.  Not while s is being captured into
+// a graph: a captured record only runs at replay, so it says nothing about
+// the kernels queued now (graph-captured uses are not tracked).
+static int mark_used(const madi_geom* const* gs, int n, hipStream_t s)
+{
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return MADI_OK;
+    auto u = std::make_shared<UseEvent>();
+    HIP_TRY(hipEventCreateWithFlags(&u->ev, hipEventDisableTiming));
+    HIP_TRY(hipEventRecord(u->ev, s));
+    for (int i = 0; i < n; ++i)
+        if (gs[i]) gs[i]->used = u;
+    return MADI_OK;
+}
+
+static int mark_used(const madi_geom* g, hipStream_t s) { return mark_used(&g, 1, s); }
+
+static int mark_work_used(const madi_work* work, int32_t n_work, hipStream_t s)
+{
+    std::vector<const madi_geom*> gs((size_t)n_work);
+    for (int32_t i = 0; i < n_work; ++i) gs[(size_t)i] = work[i].geom;
+    return mark_used(gs.data(), n_work, s);
+}
+
 static void free_geom(madi_geom* g)
 {
     if (g->allocs.empty()) return;
@@ -975,6 +1014,18 @@ int madi_geom_destroy(madi_geom* g)
 int madi_geom_destroy_async(madi_geom* g, void* stream)
 {
     if (!g) return MADI_OK;
+    if (g->used) {
+        const hipError_t q = hipEventQuery(g->used->ev);
+        if (q == hipErrorNotReady) {
+            set_error("madi_geom_destroy_async: a kernel that reads the geometry has not finished (wait for its "
+                      "stream, or use madi_geom_destroy); nothing was freed");
+            return MADI_ERR_BUSY;
+        }
+        if (q != hipSuccess) {
+            set_error("madi_geom_destroy_async: %s", hipGetErrorString(q));
+            return MADI_ERR_HIP;
+        }
+    }
     hipError_t first = hipSuccess;
     for (void* p : g->allocs) {
         const hipError_t e = hipFreeAsync(p, (hipStream_t)stream);
@@ -1065,7 +1116,8 @@ int madi_classify(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d
         set_error("madi_classify: geometry has no KD tree");
         return MADI_ERR_ARG;
     }
-    return launch_classify_f64(g->dev, d_pts, n, d_cell, d_inside, d_ok, nullptr, (hipStream_t)stream);
+    const int rc = launch_classify_f64(g->dev, d_pts, n, d_cell, d_inside, d_ok, nullptr, (hipStream_t)stream);
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 int madi_classify_graph(const madi_geom* g, const double* d_pts, int64_t n, int32_t* d_cell, int8_t* d_inside,
@@ -1079,8 +1131,9 @@ int madi_classify_graph(const madi_geom* g, const double* d_pts, int64_t n, int3
         set_error("madi_classify_graph: geometry has no Delaunay graph");
         return MADI_ERR_ARG;
     }
-    return launch_classify_graph(g->dev, d_pts, n, d_cell, d_inside, d_ok, (unsigned long long*)d_counters,
-                                 (hipStream_t)stream);
+    const int rc = launch_classify_graph(g->dev, d_pts, n, d_cell, d_inside, d_ok, (unsigned long long*)d_counters,
+                                         (hipStream_t)stream);
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 int madi_rng_init(uint64_t seed, uint64_t first, int64_t n, uint64_t* d_state, void* stream)
@@ -1118,8 +1171,9 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
     }
     const Gf2Col* jp = nullptr;
     if ((rc = get_jump_pow(&jp))) return rc;
-    return launch_walk_f64(g->dev, W, jp, d_gold_init, d_gold_incr, d_gold_u, d_Y, d_inside_trace, d_escaped, d_err,
-                           (unsigned long long*)d_counters, (hipStream_t)stream);
+    rc = launch_walk_f64(g->dev, W, jp, d_gold_init, d_gold_incr, d_gold_u, d_Y, d_inside_trace, d_escaped, d_err,
+                         (unsigned long long*)d_counters, (hipStream_t)stream);
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
@@ -1183,8 +1237,9 @@ int madi_walk_fused(const madi_geom* g, const madi_walk_params* p, const madi_ph
         set_error("madi_walk_fused: invalid arguments");
         return MADI_ERR_ARG;
     }
-    return fused_walk(g, p, plan, precision, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
-                      d_counters, (hipStream_t)stream, "madi_walk_fused");
+    const int rc = fused_walk(g, p, plan, precision, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy,
+                              d_flags, d_counters, (hipStream_t)stream, "madi_walk_fused");
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 // The phase plan of madi_walk_y_f32: one column (pair (0, 0, 0), pc 0), so
@@ -1228,8 +1283,9 @@ int madi_walk_y_f32(const madi_geom* g, const madi_walk_params* p, void* d_works
     hipStream_t s = (hipStream_t)stream;
     HIP_TRY(hipMemsetAsync(d_Y, 0, (size_t)p->n_walkers * p->n_grid * 3 * sizeof(double), s));
     double* sums = (double*)((char*)d_workspace + lay.total);
-    return fused_walk(g, p, &plan, 32, (char*)d_workspace, lay.total, sums, sums + 1, d_occupancy, d_flags,
-                      d_counters, s, "madi_walk_y_f32", nullptr, d_Y);
+    const int rc = fused_walk(g, p, &plan, 32, (char*)d_workspace, lay.total, sums, sums + 1, d_occupancy, d_flags,
+                              d_counters, s, "madi_walk_y_f32", nullptr, d_Y);
+    return rc ? rc : mark_used(g, s);
 }
 
 int madi_walk_fused_verify(const madi_geom* g, const madi_walk_params* p, const madi_phase_plan* plan,
@@ -1241,8 +1297,9 @@ int madi_walk_fused_verify(const madi_geom* g, const madi_walk_params* p, const 
         set_error("madi_walk_fused_verify: invalid arguments");
         return MADI_ERR_ARG;
     }
-    return fused_walk(g, p, plan, 32, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
-                      d_counters, (hipStream_t)stream, "madi_walk_fused_verify", d_verify);
+    const int rc = fused_walk(g, p, plan, 32, (char*)d_workspace, workspace_bytes, d_cos, d_sin, d_occupancy, d_flags,
+                              d_counters, (hipStream_t)stream, "madi_walk_fused_verify", d_verify);
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 int madi_walk_fused_static_max_bytes(int64_t* bytes)
@@ -1262,8 +1319,9 @@ int madi_walk_fused_static(const madi_geom* g, const madi_walk_params* p, const 
     char* base = nullptr;
     int rc = static_scratch(&base);
     if (rc) return rc;
-    return fused_walk(g, p, plan, precision, base, kStaticScratchBytes, d_cos, d_sin, d_occupancy, d_flags,
-                      d_counters, (hipStream_t)stream, "madi_walk_fused_static");
+    rc = fused_walk(g, p, plan, precision, base, kStaticScratchBytes, d_cos, d_sin, d_occupancy, d_flags, d_counters,
+                    (hipStream_t)stream, "madi_walk_fused_static");
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
 }  // extern "C"
@@ -1361,9 +1419,10 @@ int madi_batch_launch(const madi_work* work, int32_t n_work, const madi_phase_pl
     char* base = (char*)d_workspace;
     HIP_TRY(hipMemcpyAsync(base, img.data(), img.size(), hipMemcpyHostToDevice, s));
     const int grid = (int)std::min<int64_t>(k.grid, streamed(k.variant) ? (walkers + 63) / 64 : total);
-    return launch_batch((WorkDev*)(base + batch_off_table()), (unsigned long long*)(base + batch_off_next(n_work)),
-                        (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base, ev, jp, k, grid,
-                        (unsigned long long*)d_counters, s);
+    rc = launch_batch((WorkDev*)(base + batch_off_table()), (unsigned long long*)(base + batch_off_next(n_work)),
+                      (BatchHdr*)(base + batch_off_hdr()), (PlanDev*)base, ev, jp, k, grid,
+                      (unsigned long long*)d_counters, s);
+    return rc ? rc : mark_work_used(work, n_work, s);
 }
 
 int madi_batch_graph_create(const madi_phase_plan* plan, int32_t capacity, const madi_tune* tune, int64_t* d_counters,
@@ -1487,12 +1546,12 @@ int madi_batch_graph_launch(madi_batch_graph* gr, const madi_work* work, int32_t
         if (rc) return rc;
         HIP_TRY(hipEventRecord(gr->done, s));
         gr->launched = true;
-        return MADI_OK;
+        return mark_work_used(work, n_work, s);
     }
     HIP_TRY(hipGraphLaunch(gr->exec, (hipStream_t)stream));
     HIP_TRY(hipEventRecord(gr->done, (hipStream_t)stream));
     gr->launched = true;
-    return MADI_OK;
+    return mark_work_used(work, n_work, (hipStream_t)stream);
 }
 
 int madi_batch_graph_destroy(madi_batch_graph* gr)

@@ -806,12 +806,14 @@ class DeviceGeometry:
             self.handle = None
 
     def release(self, stream: int) -> None:
-        """Free the device copy in the order of `stream` without waiting (every
-        use of the geometry was queued on that stream or has finished)."""
+        """Free the device copy in the order of `stream` without waiting.  Every
+        kernel that reads the geometry must have finished: the library refuses
+        (RuntimeError, MADI_ERR_BUSY) while the last one it launched is still
+        queued or running, and the handle is then kept."""
         h = getattr(self, "handle", None)
         if h is not None and h.value:
-            self.handle = None
             _hip.check(self._lib.madi_geom_destroy_async(h, stream), "madi_geom_destroy_async")
+            self.handle = None
 
     @property
     def device_bytes(self) -> int:

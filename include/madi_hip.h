@@ -139,7 +139,11 @@ int madi_geom_destroy(madi_geom* g);
  * to the stream-ordered allocator in the order of `stream` (a hipStream_t).
  * Every kernel that reads the geometry must have finished (an event recorded
  * after its last launch has completed): the freed memory may back a geometry
- * created afterwards before `stream` reaches this point.  No counterpart in
+ * created afterwards before `stream` reaches this point.  The library checks
+ * it: each entry point that launches kernels reading a geometry records an
+ * event behind them (not while the stream is captured into a graph), and
+ * while that event has not completed this call frees nothing and returns
+ * MADI_ERR_BUSY.  No counterpart in
  * the reference (it re-uploads per walk); for schedulers that retire entries
  * while other work runs. */
 int madi_geom_destroy_async(madi_geom* g, void* stream);

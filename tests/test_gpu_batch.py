@@ -289,3 +289,39 @@ def test_walks_longer_than_the_lds_grid_run_the_streamed_walk_with_the_same_resu
     batched = batch.run_batched([prep], spec)[0]
     assert default.occupancy.shape == (301,)
     assert _same(default, tile) and _same(default, batched)
+
+
+class _WalkCfg:
+    def __init__(self, steps, sph, G, sigma, ts):
+        self.n_steps, self.steps_per_h, self.n_grid, self.sigma, self.ts = steps, sph, G, sigma, ts
+
+
+def test_geometry_release_is_refused_while_a_kernel_reads_it():
+    """madi_geom_destroy_async frees nothing (MADI_ERR_BUSY) while the last
+    kernel launched on the geometry is still queued or running, and frees it
+    once that kernel has finished.  The walk and the free are on one stream,
+    so the free is ordered behind the walk either way."""
+    import ctypes as C
+    import torch
+    from custom_madi_amd import _hip
+    from tests import _geom
+    dg = DeviceGeometry(geometry_arrays(_geom.cellular(L=150.0, seed=9)))
+    n, cfg = 65_536, _WalkCfg(2000, 50, 41, np.sqrt(2 * 2.0 * 0.02), 0.02)
+    p = engine.walk_params(n, cfg, 0.03, 5)
+    Y = torch.empty((n, cfg.n_grid, 3), dtype=torch.float64, device="cuda")
+    itr = torch.empty((n, cfg.n_grid), dtype=torch.int8, device="cuda")
+    esc = torch.empty(n, dtype=torch.int8, device="cuda")
+    err = torch.empty(n, dtype=torch.int8, device="cuda")
+    lib = _hip.load_library()
+    s = _hip.stream_ptr()
+    torch.cuda.synchronize()
+    say("[batch] FP64 walk launched, release requested before it finishes")
+    _hip.check(lib.madi_walk_f64(dg.handle, C.byref(p), None, None, None, _hip.dptr(Y), _hip.dptr(itr),
+                                 _hip.dptr(esc), _hip.dptr(err), None, s), "madi_walk_f64")
+    with pytest.raises(RuntimeError, match="has not finished"):
+        dg.release(s)
+    assert dg.handle is not None and dg.handle.value
+    torch.cuda.synchronize()
+    dg.release(s)
+    assert dg.handle is None
+    torch.cuda.synchronize()
