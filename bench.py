@@ -566,7 +566,8 @@ def run_c4(args):
     t0 = time.perf_counter()
     for r in reqs:
         srv.submit(r)
-    srv.drain()
+    srv.drain(progress=lambda: log(f"{len(srv.results)} of {len(reqs)} entries done, "
+                                   f"{time.perf_counter() - t0:.0f} s"))
     wall = time.perf_counter() - t0
     sch.close()
     pool.close()

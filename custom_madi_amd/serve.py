@@ -145,8 +145,14 @@ class Server:
     def idle(self) -> bool:
         return not (self.queued or self.preparing or self.ready or self.sch.active or self.sch.inflight)
 
-    def drain(self) -> None:
+    def drain(self, progress: Optional[Callable[[], None]] = None, every_s: float = 30.0) -> None:
+        """Run until every submitted request has finished; `progress` is called
+        every `every_s` seconds meanwhile."""
+        last = time.perf_counter()
         while not self.idle():
+            if progress is not None and time.perf_counter() - last >= every_s:
+                progress()
+                last = time.perf_counter()
             self._collect_prepared()
             if not self.preparing and self.ready and not self.sch.active and not self.sch.inflight:
                 if not self.sch.admit(self.ready[0]):
