@@ -211,6 +211,8 @@ def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
     torch = _hip.require_gpu()
     n, G, _ = Y_dev.shape
     n_cols = len(pcs)
+    if not (len(j_delta) == len(j_Delta) == len(j_sum) == n_cols):
+        raise ValueError("j_delta, j_Delta, j_sum and the phase coefficients need one entry per column")
     if n_cols > _hip.MADI_MAX_REDUCE_COLS:
         raise ValueError(f"{n_cols} columns exceed one reduce call's {_hip.MADI_MAX_REDUCE_COLS}")
     for name, j in (("j_delta", j_delta), ("j_Delta", j_Delta), ("j_sum", j_sum)):
