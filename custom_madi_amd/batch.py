@@ -151,6 +151,8 @@ class DeviceScheduler:
             if int(classify_mode) not in (0, 3, 4):
                 raise ValueError("Y mode needs a streamed walk (classify_mode 0, 3 or 4)")
             jd, jD, js, pc = y_columns
+            if not (len(jd) == len(jD) == len(js) == len(pc)):
+                raise ValueError("y_columns: j_delta, j_Delta, j_sum and phase_coef need one entry per column")
             self.y_n = len(pc)
             if self.y_n > _hip.MADI_MAX_REDUCE_COLS:
                 raise ValueError(f"{self.y_n} columns exceed one reduce call's {_hip.MADI_MAX_REDUCE_COLS}")

@@ -208,7 +208,6 @@ def single_column_spec() -> "PhaseSpec":
 
 def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
     """Sigma cos / sigma sin per column from a device Y (FP64)."""
-    torch = _hip.require_gpu()
     n, G, _ = Y_dev.shape
     n_cols = len(pcs)
     if not (len(j_delta) == len(j_Delta) == len(j_sum) == n_cols):
@@ -219,6 +218,7 @@ def reduce_columns(Y_dev, j_delta, j_Delta, j_sum, pcs):
         j = np.asarray(j)
         if j.size and (j.min() < 0 or j.max() >= G):
             raise ValueError(f"{name} holds grid indices outside [0, {G})")
+    torch = _hip.require_gpu()
     jd = _hip.to_device(np.asarray(j_delta, dtype=np.int32))
     jD = _hip.to_device(np.asarray(j_Delta, dtype=np.int32))
     js = _hip.to_device(np.asarray(j_sum, dtype=np.int32))

@@ -422,3 +422,16 @@ def test_coincident_seeds_are_a_classifier_failure_before_the_walk():
     srv = serve.Server(_Sch(), prepare=bad_prepare)
     srv.submit(req)
     assert len(srv.failed) == 1 and not srv.results and not srv.preparing
+
+
+def test_reduce_columns_checks_its_column_lists_before_any_device_call():
+    """Column lists of unequal length or grid indices outside Y are refused on
+    the host (the kernel would read past them)."""
+    from custom_madi_amd import engine
+    Y = np.zeros((4, 5, 3))
+    with pytest.raises(ValueError, match="one entry per column"):
+        engine.reduce_columns(Y, [1, 1, 2], [2, 3, 3], [3, 4, 4], [0.0, 0.05, 0.3, 0.1])
+    with pytest.raises(ValueError, match="outside"):
+        engine.reduce_columns(Y, [1], [2], [5], [0.1])
+    with pytest.raises(ValueError, match="outside"):
+        engine.reduce_columns(Y, [-1], [2], [3], [0.1])
