@@ -211,7 +211,9 @@ def measure_c2(args, pool, log):
                     "latency_s_p50": float(np.percentile(lat, 50)), "latency_s_p99": float(np.percentile(lat, 99)),
                     "host_workers": args.workers, "escapes_or_errors": esc},
         "plain": {"entries": n_plain, "wall_s": t_plain, "entries_per_hour": eph_p,
-                  "walker_steps_per_s": ws_plain / t_plain},
+                  "walker_steps_per_s": ws_plain / t_plain,
+                  "host_prep_ms_mean": 1e3 * float(np.mean([r.prep_s for r in res_p])),
+                  "host_prep_ms_max": 1e3 * float(np.max([r.prep_s for r in res_p]))},
         "plain_prebuilt_geometry": {"entries": n_plain, "wall_s": t_plain_dev, "entries_per_hour": eph_pd,
                                     "walker_steps_per_s": ws_plain / t_plain_dev},
         "batched_over_plain": eph_b / eph_p, "batched_over_plain_prebuilt": eph_b / eph_pd,

@@ -127,6 +127,9 @@ SIGNATURES = {
                                          _p, _p, _p]),
     "madi_moments_bytes": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_kd_build": (C.c_int, [_p, C.c_int64, _p, _p, _p, _p, _p]),
+    "madi_nearest_points": (C.c_int, [_p, C.c_int64, _p, C.c_int64, C.c_double, C.c_int32, C.c_int32, _p, _p]),
+    "madi_voronoi_graph": (C.c_int, [_p, C.c_int64, C.c_int64, C.c_double, C.c_int32, _p, _p, C.c_int64, _p, _p,
+                                     _p]),
     "madi_walk_y_f32_workspace": (C.c_int, [C.c_int64, _i64p]),
     "madi_walk_y_f32": (C.c_int, [_p, C.POINTER(WalkParams), _p, C.c_int64, _p, _p, _p, _p, _p]),
     "madi_batch_workspace": (C.c_int, [C.c_int32, _i64p]),

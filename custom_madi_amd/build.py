@@ -36,6 +36,7 @@ UNITS = {
     "prod.hip": [],
     "abi.hip": [],
     "match.hip": ["-ffp-contract=off"],
+    "geom_host.cpp": ["-ffp-contract=off"],
 }
 
 

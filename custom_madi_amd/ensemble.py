@@ -15,6 +15,13 @@ starting seed.  A periodic ensemble (a torus of side L) is represented by
 periodic images: the KD tree indexes the seeds plus their images within a
 certified margin, and every Delaunay edge carries the image shift, so the
 walk kernels never wrap coordinates.
+
+Every geometric query of the construction is native host code of the HIP
+library: the KD build (``madi_kd_build``), nearest-point queries on a bin
+grid (``madi_nearest_points``: nearest-neighbour distances, the Eq. S10
+certification grid, the hint grid) and the Voronoi neighbour graph
+(``madi_voronoi_graph``: each cell cut out by its neighbours' bisector
+planes).
 """
 
 from __future__ import annotations
@@ -27,7 +34,6 @@ from pathlib import Path
 from typing import Any, Optional, Tuple
 
 import numpy as np
-from scipy.spatial import Delaunay, cKDTree
 
 from . import _hip
 from .config import SimConfig
@@ -248,22 +254,74 @@ def build_kdtree(points: np.ndarray) -> Tuple[np.ndarray, np.ndarray, np.ndarray
     return seed, axis, left, right, parent
 
 
+# ------------------------------------------------------- native point queries
+
+def nearest_points(points: np.ndarray, queries: Optional[np.ndarray] = None, period: float = 0.0,
+                   n_threads: int = 0) -> Tuple[np.ndarray, np.ndarray]:
+    """(index, distance) of the nearest point to each query (madi_nearest_points).
+
+    queries=None queries the points themselves and returns each point's
+    nearest OTHER point (the k=2 column of a KD-tree query).  period > 0:
+    periodic cube [0, period)^3, minimum-image distances."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    self_query = queries is None
+    q = pts if self_query else np.ascontiguousarray(queries, dtype=np.float64).reshape(-1, 3)
+    idx = np.empty(len(q), dtype=np.int32)
+    dist = np.empty(len(q), dtype=np.float64)
+    lib = _hip.load_library()
+    rc = lib.madi_nearest_points(pts.ctypes.data, len(pts), q.ctypes.data, len(q), float(period), int(self_query),
+                                 int(n_threads), idx.ctypes.data, dist.ctypes.data)
+    _hip.check(rc, "madi_nearest_points")
+    return idx, dist
+
+
+def voronoi_graph(points: np.ndarray, owned: int, pad: float,
+                  n_threads: int = 0) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
+    """CSR Voronoi neighbours of the first ``owned`` points (madi_voronoi_graph):
+    (offsets (owned + 1,) int64, ids int32 into ``points``, each cell's largest
+    vertex distance, 1 where a cell reaches the points' bounding box widened
+    by ``pad``).
+
+    Coincident seeds have no facets between them, so no label is defined
+    near them: that is the reference's classifier failure (the exact
+    classifier divides by the seeds' separation, madi/walker_gpu.py:272-335,
+    938-940), raised here as RuntimeError before any walk starts."""
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+    owned = int(owned)
+    offsets = np.zeros(owned + 1, dtype=np.int64)
+    radius = np.zeros(owned, dtype=np.float64)
+    boxed = np.zeros(owned, dtype=np.int8)
+    n_edges = C.c_int64(0)
+    lib = _hip.load_library()
+    capacity = 24 * owned + 64
+    for _ in range(2):
+        ids = np.empty(capacity, dtype=np.int32)
+        rc = lib.madi_voronoi_graph(pts.ctypes.data, len(pts), owned, float(pad), int(n_threads),
+                                    offsets.ctypes.data, ids.ctypes.data, capacity, C.byref(n_edges),
+                                    radius.ctypes.data, boxed.ctypes.data)
+        if rc == _hip.MADI_ERR_ARG and n_edges.value > capacity:
+            capacity = int(n_edges.value)
+            continue
+        if rc == _hip.MADI_ERR_CLASSIFIER:
+            raise RuntimeError(_hip.last_error())
+        _hip.check(rc, "madi_voronoi_graph")
+        return offsets, ids[: n_edges.value].copy(), radius, boxed
+    raise RuntimeError("madi_voronoi_graph: edge count changed between calls")
+
+
 # ---------------------------------------------------------------- population
 
-def _nearest_distance_upper_bound(tree: cKDTree, side_um: float, spacing_um: float) -> Tuple[float, float]:
+def _nearest_distance_upper_bound(seeds: np.ndarray, side_um: float, spacing_um: float) -> Tuple[float, float]:
     """Grid + Lipschitz upper bound of the nearest-seed distance over [0, side]^3
-    (madi/ensemble.py:566-582)."""
+    (madi/ensemble.py:566-582): the largest nearest-seed distance over the
+    centres of an m^3 grid of spacing h <= spacing_um, plus half the grid
+    cell's diagonal."""
     m = max(1, int(np.ceil(side_um / spacing_um)))
     h = side_um / m
     axis = (np.arange(m, dtype=np.float64) + 0.5) * h
-    yy, zz = np.meshgrid(axis, axis, indexing="ij")
-    plane = np.column_stack((np.zeros(yy.size), yy.ravel(), zz.ravel()))
-    worst = 0.0
-    for x in axis:
-        plane[:, 0] = x
-        d, _ = tree.query(plane, k=1)
-        worst = max(worst, float(np.max(d)))
-    return worst + 0.5 * math.sqrt(3.0) * h, h
+    X, Y, Z = np.meshgrid(axis, axis, axis, indexing="ij")
+    _, d = nearest_points(seeds, np.column_stack((X.ravel(), Y.ravel(), Z.ravel())))
+    return float(np.max(d)) + 0.5 * math.sqrt(3.0) * h, h
 
 
 def _make_population(rho_um3: float, side_um: float, cfg: SimConfig, rng: np.random.Generator):
@@ -281,10 +339,8 @@ def _make_population(rho_um3: float, side_um: float, cfg: SimConfig, rng: np.ran
             margin *= 1.75
             continue
         seeds = rng.uniform(lo, hi, size=(count, 3)).astype(np.float64)
-        tree = cKDTree(seeds)
-        d2, _ = tree.query(seeds, k=2)
-        nn = np.asarray(d2[:, 1], dtype=np.float64)
-        s10_upper, h = _nearest_distance_upper_bound(tree, side_um, cfg.population_certification_spacing_um)
+        _, nn = nearest_points(seeds)
+        s10_upper, h = _nearest_distance_upper_bound(seeds, side_um, cfg.population_certification_spacing_um)
         s10_pass = bool(s10_upper < margin)
         relevant = _distance_to_cube(seeds, 0.0, side_um) <= s10_upper
         clearance = np.min(np.minimum(seeds - lo, hi - seeds), axis=1)
@@ -297,7 +353,7 @@ def _make_population(rho_um3: float, side_um: float, cfg: SimConfig, rng: np.ran
             s11_pass=s11_pass, certification_spacing_um=float(h), expansion_index=expansion,
         )
         if s10_pass and s11_pass:
-            return seeds, tree, nn, cert
+            return seeds, nn, cert
         margin *= 1.75
     raise RuntimeError(
         f"Could not certify Ω_pop against SI Eqs. S10--S12 after {cfg.population_max_expansions} expansions."
@@ -337,8 +393,11 @@ class Ensemble:
     periodic: bool = False
     _device: Optional["DeviceGeometry"] = None
 
-    def _tree(self) -> cKDTree:
+    def _tree(self):
+        """A scipy cKDTree of the seeds (the reference's private accessor, for
+        callers that query it; nothing in this package does)."""
         if self.tree is None:
+            from scipy.spatial import cKDTree
             self.tree = cKDTree(self.seeds, boxsize=self.L if self.periodic else None)
         return self.tree
 
@@ -431,7 +490,7 @@ def build_ensemble_host(rho: float, V: float, cfg: SimConfig, seed: int | None =
     mean_av, reference = governing_mean_A_over_V(target_vi, rho, cfg)
     side = si_domain_side_um(rho, cfg)
     rng = np.random.default_rng(seed)
-    seeds, tree, nn, population = _make_population(rho_um3, side, cfg, rng)
+    seeds, nn, population = _make_population(rho_um3, side, cfg, rng)
     annulus = np.minimum(alpha_star, cfg.kappa * nn / 2.0).astype(np.float64)
     source_side = cfg.source_fraction * side
     source_lo = 0.5 * (side - source_side)
@@ -449,7 +508,7 @@ def build_ensemble_host(rho: float, V: float, cfg: SimConfig, seed: int | None =
         seeds=seeds, annulus=annulus, rho=rho_real, V=float("nan"), vi=float("nan"), alpha_star=alpha_star,
         L=side, source_lo=source_lo, source_hi=source_hi, mean_AV=mean_av, geometry=stats,
         rho_requested=float(rho), V_requested=float(V), kd_node_seed=nodes[0], kd_node_axis=nodes[1],
-        kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], tree=tree,
+        kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4],
     )
     return EnsembleDraft(ens, rng, target_vi, rho_real, alpha_star, mean_av, annulus[in_sim], len(seeds), n_sim,
                          side, source_side, population, reference)
@@ -531,9 +590,8 @@ def governing_reference_summary(ens: Ensemble) -> dict:
 # ----------------------------------------------------------- periodic boxes
 
 def periodic_nearest_distances(seeds: np.ndarray, L: float) -> np.ndarray:
-    tree = cKDTree(seeds, boxsize=L)
-    d, _ = tree.query(seeds, k=2)
-    return np.asarray(d[:, 1], dtype=np.float64)
+    """Each seed's distance to its nearest other seed on the torus of side L."""
+    return nearest_points(seeds, period=float(L))[1]
 
 
 def create_periodic_ensemble(n_cells: int, rho: float, vi: float, cfg: SimConfig | None = None,
@@ -603,39 +661,13 @@ def periodic_images(seeds: np.ndarray, L: float, margin: float) -> Tuple[np.ndar
 
 # ------------------------------------------------------------- device data
 
-def _circumspheres(points: np.ndarray, simplices: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
-    p0 = points[simplices[:, 0]]
-    A = points[simplices[:, 1:]] - p0[:, None, :]
-    rhs = 0.5 * np.sum(A * A, axis=2)
-    centre = np.linalg.solve(A, rhs[..., None])[..., 0]
-    return centre + p0, np.sqrt(np.sum(centre * centre, axis=1))
-
-
-def delaunay_graph(points: np.ndarray, owned: int) -> Tuple[np.ndarray, np.ndarray, Delaunay]:
-    """CSR Delaunay neighbours (offsets over all points, ids) of a point set.
-
-    A seed that coincides with another (or that the triangulation leaves
-    out) has no neighbours, so no label is defined near it: that is the
-    reference's classifier failure (the exact classifier divides by the
-    seeds' separation, madi/walker_gpu.py:272-335, 938-940), raised here as
-    RuntimeError before any walk starts."""
-    tri = Delaunay(points)
-    indptr, indices = tri.vertex_neighbor_vertices
-    indptr = np.asarray(indptr, dtype=np.int64)
-    isolated = np.flatnonzero(np.diff(indptr[: owned + 1]) == 0)
-    if len(isolated):
-        raise RuntimeError(
-            f"GPU full-facet classification failed: {len(isolated)} seed(s) (first: {int(isolated[0])}) coincide "
-            "with another seed or fall out of the Delaunay triangulation, so their cells have no facets")
-    return indptr, np.asarray(indices, dtype=np.int64), tri
-
-
-def _hint_grid(tree: cKDTree, L: float, n_hint: int) -> np.ndarray:
+def _hint_grid(seeds: np.ndarray, L: float, n_hint: int, periodic: bool) -> np.ndarray:
+    """The nearest seed to the centre of each cell of an n_hint^3 grid over [0, L]^3."""
     h = L / n_hint
     axis = (np.arange(n_hint) + 0.5) * h
     X, Y, Z = np.meshgrid(axis, axis, axis, indexing="ij")
-    _, idx = tree.query(np.column_stack((X.ravel(), Y.ravel(), Z.ravel())), k=1)
-    return np.asarray(idx, dtype=np.int32)
+    idx, _ = nearest_points(seeds, np.column_stack((X.ravel(), Y.ravel(), Z.ravel())), period=L if periodic else 0.0)
+    return idx
 
 
 @dataclass
@@ -678,62 +710,59 @@ def geometry_arrays(ens: Ensemble, with_kd: bool = True) -> GeometryArrays:
     ann = np.ascontiguousarray(ens.annulus, dtype=np.float64)
     n = len(seeds)
     spacing = (L ** 3 / n) ** (1.0 / 3.0)
+    n_hint = int(min(64, max(4, round(1.5 * L / spacing))))
     if not ens.periodic:
-        indptr, indices, _ = delaunay_graph(seeds, n)
-        nb_vec = seeds[indices] - np.repeat(seeds, np.diff(indptr), axis=0)
-        tree = ens._tree()
-        n_hint = int(min(64, max(4, round(1.5 * L / spacing))))
+        # cells at the population's hull are cut by the box one spacing
+        # beyond the seeds; walks stay in Omega_sim, deep inside it
+        offs, ids, _, _ = voronoi_graph(seeds, n, pad=spacing)
+        nb_vec = seeds[ids] - np.repeat(seeds, np.diff(offs), axis=0)
         nodes = (ens.kd_node_seed, ens.kd_node_axis, ens.kd_node_left, ens.kd_node_right, ens.kd_node_parent)
         return GeometryArrays(
             kd_xyz=seeds if with_kd else seeds[:0], kd_annulus=ann, kd_label=np.arange(n, dtype=np.int32),
             nodes=nodes if with_kd else (),
-            cell_xyz=seeds, cell_annulus=ann, nb_offsets=indptr, nb_ids=indices.astype(np.int32),
-            nb_vec=np.ascontiguousarray(nb_vec), hint_n=n_hint, hint=_hint_grid(tree, L, n_hint), L=L,
+            cell_xyz=seeds, cell_annulus=ann, nb_offsets=offs, nb_ids=ids,
+            nb_vec=np.ascontiguousarray(nb_vec), hint_n=n_hint, hint=_hint_grid(seeds, L, n_hint, False), L=L,
             source_lo=float(ens.source_lo), source_hi=float(ens.source_hi), periodic=False, free_water=False,
         )
     # Periodic: pad with images.  The margin must cover (a) the exact
     # classifier's ball d1 + 2 alpha1 around any point of the box and (b)
-    # the circumspheres of every Delaunay simplex that touches a cell.
-    ptree = cKDTree(seeds, boxsize=L)
+    # twice each cell's largest vertex distance: every image within that
+    # distance of a seed of the box (all of them within `margin`) is then
+    # among the points its cell was cut by, so the cell is exact.
+    # (twice the largest vertex distance is 2.5-3.1 spacings in boxes of
+    # 50-4,000 cells, so 3.5 spacings rarely needs a second pass)
+    margin = min(L, 3.5 * spacing)
     if with_kd:
-        d1_upper, _ = _nearest_distance_upper_bound_periodic(ptree, L, min(spacing / 4.0, 2.0))
-        margin = min(L, d1_upper + 2.0 * float(np.max(ann)) + 1e-6)
-        margin = max(margin, 2.0 * spacing)
-    else:
-        margin = min(L, 3.0 * spacing)
+        d1_upper, _ = _nearest_distance_upper_bound_periodic(seeds, L, min(spacing / 4.0, 2.0))
+        margin = max(margin, min(L, d1_upper + 2.0 * float(np.max(ann)) + 1e-6), 2.0 * spacing)
     for _ in range(8):
         pts, label, shift = periodic_images(seeds, L, margin)
-        indptr, indices, tri = delaunay_graph(pts, n)
-        touching = np.any(tri.simplices < n, axis=1)
-        centre, radius = _circumspheres(pts, tri.simplices[touching])
-        lo_ok = np.all(centre - radius[:, None] >= -margin)
-        hi_ok = np.all(centre + radius[:, None] <= L + margin)
-        if lo_ok and hi_ok:
+        offs, ids, radius, boxed = voronoi_graph(pts, n, pad=spacing)
+        need = 2.0 * float(np.max(radius))
+        if need <= margin and not boxed.any():
             break
-        margin *= 1.25
+        margin = max(1.25 * margin, 1.05 * need)
     else:
         raise RuntimeError("periodic Delaunay padding could not be certified")
-    offs = indptr[: n + 1]
-    ids = indices[: offs[-1]]
     owner = np.repeat(np.arange(n), np.diff(offs))
     nb_vec = pts[ids] - pts[owner]
     nodes = build_kdtree(pts) if with_kd else ()
-    n_hint = int(min(64, max(4, round(1.5 * L / spacing))))
     return GeometryArrays(
         kd_xyz=np.ascontiguousarray(pts) if with_kd else pts[:0], kd_annulus=np.ascontiguousarray(ann[label]),
-        kd_label=label, nodes=nodes, cell_xyz=seeds, cell_annulus=ann, nb_offsets=np.ascontiguousarray(offs),
+        kd_label=label, nodes=nodes, cell_xyz=seeds, cell_annulus=ann, nb_offsets=offs,
         nb_ids=np.ascontiguousarray(label[ids]).astype(np.int32), nb_vec=np.ascontiguousarray(nb_vec),
-        hint_n=n_hint, hint=_hint_grid(ptree, L, n_hint), L=L, source_lo=float(ens.source_lo),
+        hint_n=n_hint, hint=_hint_grid(seeds, L, n_hint, True), L=L, source_lo=float(ens.source_lo),
         source_hi=float(ens.source_hi), periodic=True, free_water=False, pad_margin=float(margin),
     )
 
 
-def _nearest_distance_upper_bound_periodic(tree: cKDTree, L: float, spacing: float) -> Tuple[float, float]:
+def _nearest_distance_upper_bound_periodic(seeds: np.ndarray, L: float, spacing: float) -> Tuple[float, float]:
+    """The periodic twin of _nearest_distance_upper_bound over one period."""
     m = max(1, int(np.ceil(L / spacing)))
     h = L / m
     axis = (np.arange(m) + 0.5) * h
     X, Y, Z = np.meshgrid(axis, axis, axis, indexing="ij")
-    d, _ = tree.query(np.column_stack((X.ravel(), Y.ravel(), Z.ravel())), k=1)
+    _, d = nearest_points(seeds, np.column_stack((X.ravel(), Y.ravel(), Z.ravel())), period=L)
     return float(np.max(d)) + 0.5 * math.sqrt(3.0) * h, h
 
 

@@ -25,9 +25,11 @@ from .batch import DeviceScheduler, EntryRequest, EntryResult, PreparedEntry, fa
 
 
 def _worker_init() -> None:
-    # import the geometry code (numpy, scipy.spatial) when the worker starts,
-    # not inside the first request it prepares
-    from . import batch, ensemble  # noqa: F401
+    # import the geometry code (numpy and the library's native host
+    # geometry, which makes no device call) when the worker starts, not
+    # inside the first request it prepares
+    from . import _hip, batch, ensemble  # noqa: F401
+    _hip.load_library()
 
 
 def make_pool(workers: int):

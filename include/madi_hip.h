@@ -161,6 +161,38 @@ int madi_geom_device_bytes(const madi_geom* g, int64_t* bytes);
 int madi_kd_build(const double* pts, int64_t n, int32_t* node_seed, int8_t* node_axis, int32_t* left, int32_t* right,
                   int32_t* parent);
 
+/* Nearest of n points (pts: n x 3 FP64) to each of nq queries (q: nq x 3).
+ * period > 0: the points and queries live in the periodic cube [0, period)^3
+ * and distances take the minimum image; 0: plain distances.  exclude_self:
+ * query i ignores point i (q == pts gives each point's nearest other point).
+ * idx[nq] (nearest point, lowest index on ties) and dist[nq] (its distance,
+ * sqrt((dx^2 + dy^2) + dz^2)) may each be NULL.  n_threads <= 0: up to 8
+ * threads.  Replaces the cKDTree queries of the reference's geometry
+ * construction: nearest-neighbour distances and the Eq. S10 certification
+ * grid of _make_population / _nearest_distance_upper_bound
+ * (madi/ensemble.py:566-606); runs on the CPU, no device call. */
+int madi_nearest_points(const double* pts, int64_t n, const double* q, int64_t nq, double period,
+                        int32_t exclude_self, int32_t n_threads, int32_t* idx, double* dist);
+
+/* Voronoi neighbours (the Delaunay graph's edges) of the first n_owned of n
+ * points (pts: n x 3 FP64).  Each cell is cut out of the points' bounding box
+ * widened by pad (> 0) by the bisector planes of the other points in order of
+ * distance, until every remaining point is farther than twice the cell's
+ * largest vertex distance.  A plane within 1e-10 bin widths of a vertex is
+ * kept as a neighbour (a superset of the exact edges is safe for the
+ * classifier).  Output: offsets[n_owned + 1] and ids[offsets[i] ..
+ * offsets[i+1]) ascending; n_edges = the total, also when it exceeds
+ * capacity (then MADI_ERR_ARG and ids untouched); radius[n_owned] (or NULL)
+ * each cell's largest vertex distance; boxed[n_owned] (or NULL) 1 where a cell
+ * reaches the box.  Coincident points: MADI_ERR_CLASSIFIER (the reference's
+ * classifier failure on such seeds, madi/walker_gpu.py:938-940).  No
+ * counterpart in the reference, whose exact classifier scans a KD tree
+ * (madi/walker_gpu.py:222-335); this graph feeds the FP32 walk's classifier.
+ * Runs on the CPU, no device call. */
+int madi_voronoi_graph(const double* pts, int64_t n, int64_t n_owned, double pad, int32_t n_threads,
+                       int64_t* offsets, int32_t* ids, int64_t capacity, int64_t* n_edges, double* radius,
+                       int8_t* boxed);
+
 /* --------------------------------------------------------- classifiers */
 
 /* Exact SI Eq. S2 label of n points (d_pts: n x 3 FP64).  FP64 stackless KD

@@ -179,10 +179,22 @@ def test_population_certificate_and_annulus(uncertified_reference):
     cfg = SimConfig(L=60.0, geometry_reference_path=str(uncertified_reference),
                     allow_uncertified_geometry_reference=True)
     rng = np.random.default_rng(5)
-    seeds, tree, nn, cert = _make_population(5.0e5 * 1e-9, 60.0, cfg, rng)
+    seeds, nn, cert = _make_population(5.0e5 * 1e-9, 60.0, cfg, rng)
     assert cert.s10_pass and cert.s11_pass
+    # the native nearest-neighbour distances are bitwise those of a KD tree
     d2 = cKDTree(seeds).query(seeds, k=2)[0][:, 1]
-    assert np.allclose(nn, d2)
+    assert np.array_equal(nn, d2)
+    # the S10 bound is the reference's plane-by-plane grid query
+    # (madi/ensemble.py:566-582), restated with a KD tree
+    tree = cKDTree(seeds)
+    m = int(np.ceil(60.0 / cfg.population_certification_spacing_um))
+    h = 60.0 / m
+    coords = (np.arange(m) + 0.5) * h
+    yy, zz = np.meshgrid(coords, coords, indexing="ij")
+    worst = max(float(np.max(tree.query(np.column_stack((np.full(yy.size, x), yy.ravel(), zz.ravel())), k=1)[0]))
+                for x in coords)
+    assert cert.s10_distance_upper_um == worst + 0.5 * np.sqrt(3.0) * h
+    assert cert.certification_spacing_um == h
 
 
 def _graph_classify_np(arr, pts):
