@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 6
+#define MADI_ABI_VERSION 7
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1

@@ -27,10 +27,16 @@
  * Parity pins (tests/test_oracle_cpu.py, tests/test_match_cpu.py):
  * brute-force all-seed S2 labels, the trapezoid Y closed form (bitwise),
  * free-water Stejskal-Tanner, an independent GF(2) derivation of the jump,
- * the numpy restatement of the reference CPU walk on injected streams; for
- * the matchers the reference's known-answer test, exact-copy and scaled-copy
- * libraries with zero residual, and the numpy form of the reference's CPU
- * matcher path.
+ * the numpy restatement of the reference CPU walk on injected streams, and
+ * the reference's own CPU walk and reduce on a seeded Eq. S2 population
+ * (tests/golden/walk_vectors.npz: og_walk's Y equals it bit for bit,
+ * tests/test_walk_golden_cpu.py); for the matchers the reference's
+ * known-answer test, exact-copy and scaled-copy libraries with zero
+ * residual, and the numpy form of the reference's CPU matcher path.
+ *
+ * Parity unpinned: og_normal_f64's float32 uniforms follow numba 0.54.1's
+ * text; whether numba >= 0.57 (the reference's pin) still draws them that way
+ * cannot be checked offline, and no reference test pins the numba stream.
  */
 #include <math.h>
 #include <omp.h>

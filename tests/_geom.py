@@ -64,3 +64,34 @@ def lattice(n=4, h=2.0, jitter=1e-3, annulus=0.9, seed=0):
                     source_lo=0.0, source_hi=L, mean_AV=0.6, geometry=_stats(L), rho_requested=rho,
                     V_requested=h ** 3 * 1e-3, kd_node_seed=nodes[0], kd_node_axis=nodes[1],
                     kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4], periodic=True)
+
+
+def reference_walk_golden():
+    """(ensemble, cfg, p_p, fixture) of tests/golden/walk_vectors.npz: the
+    reference's Eq. S2 population for generator seed 21 and its own CPU walk
+    of 32 walkers on it (tests/golden/make_walk_vectors.py)."""
+    from pathlib import Path
+
+    from custom_madi_amd.config import SimConfig
+
+    g = np.load(Path(__file__).resolve().parent / "golden" / "walk_vectors.npz")
+    (side, lo, hi, pp, D0, ts, T_max, h, _sigma, _n_steps, _sph, _n_grid, n_walkers, _stream_seed, rho,
+     vi) = g["scalars"]
+    cfg = SimConfig(D0=float(D0), ts=float(ts), T_max_ms=float(T_max), h_ms=float(h), n_walkers=int(n_walkers),
+                    n_ensembles=1, small_deltas=[0.4], big_deltas=[1.2], b_values=[0.0, 1000.0])
+    seeds, ann = g["seeds"], g["annulus"]
+    nodes = (g["kd_seed"], g["kd_axis"], g["kd_left"], g["kd_right"], g["kd_parent"])
+    a_star = alpha_star_from_vi(float(vi), float(rho))
+    ens = Ensemble(seeds=seeds, annulus=ann, rho=float(rho), V=float("nan"), vi=float("nan"), alpha_star=a_star,
+                   L=float(side), source_lo=float(lo), source_hi=float(hi), mean_AV=1.0, geometry=_stats(float(side)),
+                   rho_requested=float(rho), V_requested=float("nan"), kd_node_seed=nodes[0],
+                   kd_node_axis=nodes[1], kd_node_left=nodes[2], kd_node_right=nodes[3], kd_node_parent=nodes[4])
+    return ens, cfg, float(pp), g
+
+
+def stream_digest(stream) -> str:
+    import hashlib
+    h = hashlib.sha256()
+    for a in (stream.initial_positions, stream.increments, stream.acceptance_uniforms):
+        h.update(np.ascontiguousarray(a, dtype=np.float64).tobytes())
+    return h.hexdigest()

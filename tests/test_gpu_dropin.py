@@ -70,6 +70,27 @@ def test_run_walk_Y_with_the_reference_stream_matches_the_oracle():
     assert tel["classifier_cache"]["requests"] >= cfg.n_walkers * cfg.n_steps
 
 
+def test_run_walk_Y_equals_the_reference_cpu_walk_golden():
+    """The reference's own _walk_cpu output (tests/golden/walk_vectors.npz,
+    made by tests/golden/make_walk_vectors.py) on its seeded Eq. S2 population
+    and injected stream: the HIP FP64 walk through run_walk_Y within the
+    reference's golden tolerance, its occupancy equal to the reference's inside
+    trace; madi_reduce_f64 on the reference's Y against its _reduce_cpu sums."""
+    ens, cfg, pp, g = _geom.reference_walk_golden()
+    stream = walker_gpu.make_walk_random_stream(cfg, int(g["scalars"][13]), ensemble=ens)
+    assert _geom.stream_digest(stream) == str(g["stream_sha256"])
+    say("[dropin] run_walk_Y against the reference CPU walk golden")
+    Y, n_esc, tel = walker_gpu.run_walk_Y(ens, 0.0, cfg, verbose=False, pp=pp, return_telemetry=True,
+                                          random_stream=stream)
+    assert n_esc == 0
+    assert np.allclose(Y, g["Y"], rtol=5e-11, atol=5e-12)
+    assert np.array_equal(tel["occupancy_counts"], g["inside_trace"].astype(np.int64).sum(axis=0))
+    from custom_madi_amd import _hip
+    Yd = _hip.to_device(np.ascontiguousarray(g["Y"], dtype=np.float64))
+    cs, sn = engine.reduce_columns(Yd, g["cols_jd"], g["cols_jD"], g["cols_js"], g["cols_pc"])
+    assert np.allclose(cs, g["cos"], rtol=1e-11, atol=1e-11) and np.allclose(sn, g["sin"], rtol=1e-11, atol=1e-11)
+
+
 def _free_water_cfg():
     # the reference's free-water check: S8 box of the acellular control,
     # 512 walkers, 70 ms, microsecond steps
