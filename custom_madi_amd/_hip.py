@@ -58,6 +58,7 @@ class GeomDesc(C.Structure):
         ("hint_n", C.c_int32), ("hint", _i32p),
         ("L", C.c_double), ("source_lo", C.c_double), ("source_hi", C.c_double),
         ("periodic", C.c_int32), ("free_water", C.c_int32),
+        ("cell_incomplete", _i8p),
     ]
 
 

@@ -429,6 +429,31 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
         if ((rc = upload(g, crec.data(), crec.size(), &D.cell_rec))) return rc;
         if ((rc = upload(g, krec.data(), krec.size(), &D.nb_krec))) return rc;
         if ((rc = upload(g, kalpha.data(), kalpha.size(), &D.nb_kalpha))) return rc;
+        if (n > 0) {
+            // the FP64 walk's graph labels (classify_mode 5): FP64 unit edge
+            // vectors and half lengths, and the cells whose lists may miss a
+            // facet (cut by the construction's box) or hold a seed nearer than
+            // 2e-3 um (where the KD classifier's nearest-seed rounding could
+            // differ): those are left to the KD classifier
+            std::vector<double4> u64(padded);
+            for (int64_t j = 0; j < padded; ++j) {
+                const double ex = vec[3 * j], ey = vec[3 * j + 1], ez = vec[3 * j + 2];
+                if (ex >= 1e29) {
+                    u64[j] = make_double4(0.0, 0.0, 0.0, INFINITY);
+                    continue;
+                }
+                const double len = std::sqrt(ex * ex + ey * ey + ez * ez);
+                u64[j] = make_double4(ex / len, ey / len, ez / len, 0.5 * len);
+            }
+            std::vector<int8_t> flags(nc, 0);
+            for (int c = 0; c < nc; ++c) {
+                if (d->cell_incomplete && d->cell_incomplete[c]) flags[c] = 1;
+                for (int64_t j = off[c]; j < off[c + 1]; ++j)
+                    if (u64[j].w < 1e-3) flags[c] = 1;
+            }
+            if ((rc = upload(g, u64.data(), u64.size(), &D.nb_u64))) return rc;
+            if ((rc = upload(g, flags.data(), flags.size(), &D.cell_flags))) return rc;
+        }
         if ((rc = upload(g, d->cell_xyz, (size_t)nc * 3, &D.cell_xyz))) return rc;
         if (d->hint_n < 1 || !d->hint) {
             set_error("geometry: a hint grid is required with the Delaunay graph");
@@ -452,6 +477,8 @@ static int build_geom(const madi_geom_desc* d, madi_geom* g)
 
 static bool graph_ready(const GeomDev& g) { return g.free_water || g.nb_off != nullptr; }
 static bool kd_ready(const GeomDev& g) { return g.free_water || g.kd_f != nullptr; }
+// the FP64 walk's graph labels: FP64 edge records (uploaded with a KD tree)
+static bool graph64_ready(const GeomDev& g) { return g.free_water || (g.nb_u64 != nullptr && g.cell_flags != nullptr); }
 
 static float cap_width_of(float v) { return v > 0.f ? v : 5.0f; }
 // FP32 walk classification mode -> kernel variant (0 per-lane state machine
@@ -544,12 +571,13 @@ static int make_walk(const madi_walk_params* p, WalkDev& W)
         set_error("p_p must be finite and in [0, 1], got %g", p->pp);
         return MADI_ERR_ARG;
     }
-    if (p->classify_mode < 0 || p->classify_mode > 4 || p->classify_mode == 2 || p->scan_rounds < 0 ||
-        p->scan_rounds > 2) {
-        set_error("walk params: classify_mode must be 0, 1, 3 or 4 and scan_rounds 0-2, got %d and %d", p->classify_mode,
-                  p->scan_rounds);
+    if (p->classify_mode < 0 || p->classify_mode > MADI_CLASSIFY_FP64_CACHED || p->classify_mode == 2 ||
+        p->scan_rounds < 0 || p->scan_rounds > 2) {
+        set_error("walk params: classify_mode must be 0, 1, 3, 4 or 5 and scan_rounds 0-2, got %d and %d",
+                  p->classify_mode, p->scan_rounds);
         return MADI_ERR_ARG;
     }
+
     memset(&W, 0, sizeof W);
     W.n_walkers = p->n_walkers;
     W.n_steps = p->n_steps;
@@ -560,6 +588,7 @@ static int make_walk(const madi_walk_params* p, WalkDev& W)
     W.pp = p->pp;
     W.seed = p->seed;
     W.first_walker = p->first_walker;
+    W.graph_labels = p->classify_mode == MADI_CLASSIFY_FP64_CACHED ? 1 : 0;
     if (p->pp >= 1.0) {
         W.rej1 = W.rej2 = 1ull << 53;
     } else {
@@ -706,6 +735,11 @@ static int batch_table(const madi_work* work, int32_t n_work, const madi_phase_p
             return MADI_ERR_ARG;
         }
         if ((rc = make_walk(&w.walk, D.w))) return rc;
+        if (w.walk.classify_mode == MADI_CLASSIFY_FP64_CACHED) {
+            set_error("%s: work item %d: classify_mode 5 is the FP64 walk's label cache; batches run the FP32 walk",
+                      who, i);
+            return MADI_ERR_ARG;
+        }
         for (int p = 0; p < plan->n_pairs; ++p)
             if (plan->j_sum[p] >= w.walk.n_grid || plan->j_Delta[p] >= w.walk.n_grid ||
                 plan->j_delta[p] >= w.walk.n_grid) {
@@ -835,8 +869,17 @@ static int fused_walk(const madi_geom* g, const madi_walk_params* p, const madi_
         set_error("%s: the FP32 walk needs the geometry's Delaunay graph", who);
         return MADI_ERR_ARG;
     }
+    if (precision == 32 && p->classify_mode == MADI_CLASSIFY_FP64_CACHED) {
+        set_error("%s: classify_mode 5 is the FP64 walk's label cache (precision 64)", who);
+        return MADI_ERR_ARG;
+    }
     if (precision == 64 && !kd_ready(g->dev)) {
         set_error("%s: the FP64 walk needs the geometry's KD tree", who);
+        return MADI_ERR_ARG;
+    }
+    if (precision == 64 && p->classify_mode == MADI_CLASSIFY_FP64_CACHED && !graph64_ready(g->dev)) {
+        set_error("%s: the FP64 walk's graph labels (classify_mode 5) need the geometry's KD tree and Delaunay graph",
+                  who);
         return MADI_ERR_ARG;
     }
     if (plan->n_pairs < 0 || plan->n_pairs > MADI_MAX_PAIRS) {
@@ -1167,6 +1210,10 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
     W.use_golden = golden;
     if (!kd_ready(g->dev)) {
         set_error("madi_walk_f64: geometry has no KD tree");
+        return MADI_ERR_ARG;
+    }
+    if (p->classify_mode == MADI_CLASSIFY_FP64_CACHED && !graph64_ready(g->dev)) {
+        set_error("madi_walk_f64: graph labels (classify_mode 5) need the geometry's KD tree and Delaunay graph");
         return MADI_ERR_ARG;
     }
     const Gf2Col* jp = nullptr;

@@ -94,6 +94,8 @@ struct GeomDev {
     const int4* nb_krec;        // per edge slot: neighbour k and k's (list offset, list end, hmax bits)
     const float* nb_kalpha;     // per edge slot: alpha of the neighbour
     const double* cell_xyz;     // per cell world coordinates
+    const double4* nb_u64;      // per edge slot (ê, h) in FP64 (with a KD tree only; sentinels h = +inf)
+    const int8_t* cell_flags;   // per cell: 1 = the FP64 graph labels defer to the KD classifier
     const int32_t* hint;        // hint_n^3 cells: a seed near each hint-grid box centre
     int32_t n_cells;
     int32_t hint_n;

@@ -26,6 +26,9 @@ struct WalkDev {
     // u = k * 2^-53 >= thr  <=>  k >= rej[m-1].  rej = 2^53 + 1 disables.
     uint64_t rej1;
     uint64_t rej2;
+    // FP64 walk: 1 = graph labels with a certified ball
+    // (MADI_CLASSIFY_FP64_CACHED), 0 = the exact classifier at every endpoint
+    int32_t graph_labels;
 };
 
 // The walk's exit certificate (an inside walker's proposal past its cell,

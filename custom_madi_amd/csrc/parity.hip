@@ -34,6 +34,78 @@ __global__ __launch_bounds__(kBlock) void k_classify_f64(GeomDev g, const double
     }
 }
 
+// ------------------------------------------------------ FP64 graph labels
+
+// classify_mode 5 (the reference's "exact_cached"): labels from the Delaunay
+// graph in FP64, equal to the KD classifier's wherever they are given.
+//
+// From anchor a (its seed image at sa, world coordinates), the margin of edge
+// a -> k at q is m = h - (q - sa) . ê, the distance from q to the a-k bisector
+// (ê, h in FP64; the edges of a complete list are a's Voronoi facets).  The
+// most negative margin below -eps moves the anchor across that facet (a
+// nearer seed, the greedy walk to the nearest one).  Otherwise q lies in
+// a's Voronoi cell by more than eps, a is its nearest seed, and the smallest
+// margin is the distance to the cell's boundary: q is inside the contracted
+// cell iff it is at least alpha_a.  eps (1e-9 um, scaled with the box)
+// exceeds the FP64 rounding of these margins and of the KD classifier's
+// (both below 1e-12 um: coordinates below 1e4 um, seeds at least 2e-3 um
+// apart), so any label given here is the one the KD classifier computes;
+// a margin within eps of 0 or alpha, a cell whose list may be incomplete
+// (cell_flags) or a walk that does not settle returns 0, and the caller runs
+// the KD classifier.  The smallest margin also certifies a ball around q
+// with q's label: an inside label holds within m1 - alpha_a, an outside one
+// within min(m1, alpha_a - m1) (margins move by at most the distance moved).
+
+// The image of cell c's seed nearest to the world point q.
+__device__ __forceinline__ void seed_image(const GeomDev& g, int c, double qx, double qy, double qz, double& sx,
+                                           double& sy, double& sz)
+{
+    sx = g.cell_xyz[3 * c];
+    sy = g.cell_xyz[3 * c + 1];
+    sz = g.cell_xyz[3 * c + 2];
+    if (g.periodic) {
+        sx += g.L * rint((qx - sx) / g.L);
+        sy += g.L * rint((qy - sy) / g.L);
+        sz += g.L * rint((qz - sz) / g.L);
+    }
+}
+
+__device__ __forceinline__ int graph_label_f64(const GeomDev& g, int& a, double& sx, double& sy, double& sz, double qx,
+                                               double qy, double qz, double eps, int& inside, double& m1,
+                                               long long& moves)
+{
+    for (int it = 0; it <= kMaxGraphIters; ++it) {
+        if ((unsigned)a >= (unsigned)g.n_cells || g.cell_flags[a]) return 0;
+        const int lo = g.nb_off[a], hi = g.nb_off[a + 1];
+        const double rx = qx - sx, ry = qy - sy, rz = qz - sz;
+        double mm = __builtin_inf();
+        int km = -1;
+        for (int k = lo; k < hi; ++k) {
+            const double4 u = g.nb_u64[k];
+            const double m = u.w - (rx * u.x + ry * u.y + rz * u.z);
+            if (m < mm) {
+                mm = m;
+                km = k;
+            }
+        }
+        if (km >= 0 && mm < -eps) {
+            // a nearer seed across the facet of edge km
+            a = g.nb_id[km];
+            if ((unsigned)a >= (unsigned)g.n_cells) return 0;
+            seed_image(g, a, sx + g.nb_e64[3 * km], sy + g.nb_e64[3 * km + 1], sz + g.nb_e64[3 * km + 2], sx, sy,
+                       sz);
+            ++moves;
+            continue;
+        }
+        const double al = g.cell_alpha64[a];
+        if (!(mm > eps) || fabs(mm - al) <= eps) return 0;
+        inside = mm > al;
+        m1 = mm;
+        return 1;
+    }
+    return 0;
+}
+
 // ----------------------------------------------------------------- RNG
 
 __global__ __launch_bounds__(kBlock) void k_rng_init(uint64_t seed, uint64_t first, int64_t n,
@@ -69,7 +141,16 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
     const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = w < p.n_walkers;
     const int G = p.n_grid;
-    long long nodes = 0, crossings = 0, rejects = 0;
+    long long nodes = 0, crossings = 0, rejects = 0, classified = 0, cache_hits = 0, moves = 0, kd_calls = 0;
+    // graph labels (classify_mode 5): the walker's anchor (the cell of its
+    // position and that seed's image, asx..), and the certified ball of the
+    // last graph label: every point within crho of (ccx, ccy, ccz) has label
+    // (ccell, cin) and anchor image (csx, ..)
+    const bool graph = p.graph_labels != 0 && !g.free_water;
+    const double eps = 1e-9 * fmax(1.0, g.L / 100.0);
+    double ccx = 0.0, ccy = 0.0, ccz = 0.0, crho = -1.0, csx = 0.0, csy = 0.0, csz = 0.0;
+    double asx = 0.0, asy = 0.0, asz = 0.0;
+    int ccell = 0, cin = 0;
 
     double px = 0.0, py = 0.0, pz = 0.0;
     Rng r{0, 0};
@@ -88,9 +169,15 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
     }
     int cell = 0, in = 0;
     int err = 0, escaped = 0;
-    if (live && !classify_point_f64(g, v, px, py, pz, cell, in, nodes)) {
-        err = 1;
-        escaped = 1;
+    if (live) {
+        ++classified;
+        ++kd_calls;
+        if (!classify_point_f64(g, v, px, py, pz, cell, in, nodes)) {
+            err = 1;
+            escaped = 1;
+        } else if (graph) {
+            seed_image(g, cell, px, py, pz, asx, asy, asz);
+        }
     }
     const double half = 0.5 * g.L;
     double xp = px - half, yp = py - half, zp = pz - half;
@@ -128,8 +215,44 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
             if (!g.periodic && (nx < 0.0 || nx >= g.L || ny < 0.0 || ny >= g.L || nz < 0.0 || nz >= g.L)) {
                 escaped = 1;
             } else {
-                int ncell = 0, nin = 0;
-                if (!classify_point_f64(g, v, nx, ny, nz, ncell, nin, nodes)) {
+                int ncell = 0, nin = 0, ok = 1;
+                double nsx = asx, nsy = asy, nsz = asz;  // the proposal's anchor image
+                const double ex = nx - ccx, ey = ny - ccy, ez = nz - ccz;
+                if (graph && crho > 0.0 && ex * ex + ey * ey + ez * ez < crho * crho) {
+                    ncell = ccell;
+                    nin = cin;
+                    nsx = csx;
+                    nsy = csy;
+                    nsz = csz;
+                    ++cache_hits;
+                } else if (graph) {
+                    ++classified;
+                    int na = cell;  // from the walker's own cell
+                    double m1 = 0.0;
+                    if (graph_label_f64(g, na, nsx, nsy, nsz, nx, ny, nz, eps, nin, m1, moves)) {
+                        ncell = na;
+                        const double al = g.cell_alpha64[na];
+                        crho = (nin ? m1 - al : fmin(m1, al - m1)) - 2.0 * eps;
+                    } else {
+                        ++kd_calls;
+                        ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, nodes);
+                        if (ok) seed_image(g, ncell, nx, ny, nz, nsx, nsy, nsz);
+                        crho = -1.0;
+                    }
+                    ccx = nx;
+                    ccy = ny;
+                    ccz = nz;
+                    ccell = ncell;
+                    cin = nin;
+                    csx = nsx;
+                    csy = nsy;
+                    csz = nsz;
+                } else {
+                    ++classified;
+                    ++kd_calls;
+                    ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, nodes);
+                }
+                if (!ok) {
                     err = 1;
                     escaped = 1;
                 } else {
@@ -145,9 +268,15 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                             nz = pz;
                             ncell = cell;
                             nin = in;
+                            nsx = asx;
+                            nsy = asy;
+                            nsz = asz;
                             ++rejects;
                         }
                     }
+                    asx = nsx;
+                    asy = nsy;
+                    asz = nsz;
                     px = nx;
                     py = ny;
                     pz = nz;
@@ -211,11 +340,23 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
         const double a = wave_sum_f64((double)nodes);
         const double b = wave_sum_f64((double)crossings);
         const double c = wave_sum_f64((double)rejects);
+        const double d = wave_sum_f64((double)classified);
+        const double e = wave_sum_f64((double)cache_hits);
+        const double f = wave_sum_f64((double)moves);
+        const double h = wave_sum_f64((double)kd_calls);
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)a);
             atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)b);
             atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)c);
-            // Every live walker classifies its start and each unescaped step.
+            // every live walker's start and unescaped step: a classification
+            // (graph or KD) or, with graph labels, a label from the certified
+            // ball; KD classifications (all of them without graph labels:
+            // the starts and the graph's deferrals with them) count as FP64
+            // re-decisions
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)d);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)e);
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)f);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)h);
         }
     }
 }

@@ -104,8 +104,18 @@ def spec_from_columns(j_delta, j_Delta, j_sum, pcs, dirs=AXIS_DIRECTIONS):
     return spec, col_pair, col_b
 
 
+MADI_CLASSIFY_FP64_CACHED = 5
+
+
 def walk_params(n_walkers: int, cfg: SimConfig, pp: float, seed: int, first_walker: int = 0,
                 tier_cap_um: float = 0.0, scan_rounds: int = 0, classify_mode: int = 0) -> _hip.WalkParams:
+    """madi_walk_params of one walk.  An FP64 configuration with
+    classifier_mode "exact_cached" (the reference's cached classifier) selects
+    the FP64 walk's graph labels with their certified ball (labels equal to
+    the exact classifier's), unless the caller picks a classify_mode."""
+    if (classify_mode == 0 and getattr(cfg, "classifier_mode", "exact") == "exact_cached"
+            and getattr(cfg, "precision", "fp64") != "fp32"):
+        classify_mode = MADI_CLASSIFY_FP64_CACHED
     p = _hip.WalkParams()
     p.tier_cap_um = float(tier_cap_um)
     p.scan_rounds = int(scan_rounds)

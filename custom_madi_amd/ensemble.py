@@ -691,6 +691,7 @@ class GeometryArrays:
     periodic: bool
     free_water: bool
     pad_margin: float = 0.0
+    cell_boxed: Optional[np.ndarray] = None  # 1 where a cell's neighbour list may miss a facet
 
 
 def geometry_arrays(ens: Ensemble, with_kd: bool = True) -> GeometryArrays:
@@ -714,7 +715,7 @@ def geometry_arrays(ens: Ensemble, with_kd: bool = True) -> GeometryArrays:
     if not ens.periodic:
         # cells at the population's hull are cut by the box one spacing
         # beyond the seeds; walks stay in Omega_sim, deep inside it
-        offs, ids, _, _ = voronoi_graph(seeds, n, pad=spacing)
+        offs, ids, _, boxed = voronoi_graph(seeds, n, pad=spacing)
         nb_vec = seeds[ids] - np.repeat(seeds, np.diff(offs), axis=0)
         nodes = (ens.kd_node_seed, ens.kd_node_axis, ens.kd_node_left, ens.kd_node_right, ens.kd_node_parent)
         return GeometryArrays(
@@ -723,6 +724,7 @@ def geometry_arrays(ens: Ensemble, with_kd: bool = True) -> GeometryArrays:
             cell_xyz=seeds, cell_annulus=ann, nb_offsets=offs, nb_ids=ids,
             nb_vec=np.ascontiguousarray(nb_vec), hint_n=n_hint, hint=_hint_grid(seeds, L, n_hint, False), L=L,
             source_lo=float(ens.source_lo), source_hi=float(ens.source_hi), periodic=False, free_water=False,
+            cell_boxed=boxed,
         )
     # Periodic: pad with images.  The margin must cover (a) the exact
     # classifier's ball d1 + 2 alpha1 around any point of the box and (b)
@@ -820,6 +822,8 @@ class DeviceGeometry:
             d.nb_vec = f64(arr.nb_vec)
             d.hint_n = arr.hint_n
             d.hint = i32(arr.hint)
+            if arr.cell_boxed is not None:
+                d.cell_incomplete = i8(arr.cell_boxed)
         handle = C.c_void_p()
         _hip.check(lib.madi_geom_create(C.byref(d), C.byref(handle)), "madi_geom_create")
         self._lib = lib

@@ -12,8 +12,13 @@ The walk itself runs in libmadi_hip.so:
   Delaunay-graph classifier certified in FP64, compensated Y, FP64 sums).
 
 There is no CPU walk: ``use_gpu=False`` raises, as does a missing GPU or
-library.  ``exact_cached`` is accepted and gives the exact classifier's
-results (the reference defines it as an equivalence-preserving shortcut).
+library.  ``classifier_mode="exact_cached"`` (the reference's
+equivalence-preserving shortcut) runs the FP64 walk with graph labels
+(``MADI_CLASSIFY_FP64_CACHED``): endpoints are labelled from the Delaunay
+graph in FP64 from the walker's own cell, a label also certifies a ball that
+later endpoints take it from, and decisions within rounding of a threshold
+go to the exact KD classifier, so the results are the exact walk's bit for
+bit.
 """
 
 from __future__ import annotations

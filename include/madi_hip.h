@@ -128,6 +128,10 @@ typedef struct {
     double source_hi;
     int32_t periodic;
     int32_t free_water;
+    /* optional (NULL: none), n_cells: 1 where a cell's neighbour list may
+     * miss a facet (madi_voronoi_graph's `boxed`); the FP64 walk's graph
+     * labels leave such cells to the KD classifier */
+    const int8_t* cell_incomplete;
 } madi_geom_desc;
 
 typedef struct madi_geom madi_geom;
@@ -236,8 +240,23 @@ typedef struct {
                                classify_mode 4: 1 = 3 walker + 1 classifier waves per workgroup, 2 = 2 + 2 */
     int32_t classify_mode;  /* FP32 walk: 0 default (4), 1 per-lane state machine per tile, 3 per-lane state
                                machine with lane refill (streamed), 4 streamed walker waves with classifier
-                               waves (split); 2 (a removed lock-step variant) is rejected */
+                               waves (split); 2 (a removed lock-step variant) is rejected.  FP64 walk: 0
+                               (or 1, 3, 4) the exact classifier at every endpoint, 5 graph labels with a
+                               certified ball (MADI_CLASSIFY_FP64_CACHED; rejected by the FP32 walks) */
 } madi_walk_params;
+
+/* The FP64 walk's graph labels (the reference's "exact_cached" classifier
+ * mode, madi/walker_gpu.py:337-518): each endpoint is labelled from the
+ * geometry's Delaunay graph in FP64, starting at the walker's own cell (the
+ * margins of its facets, greedy moves to a nearer seed), and the label also
+ * certifies a ball around the endpoint (the distance to the nearest facet,
+ * or to the contracted cell's boundary) within which later endpoints take it
+ * without a scan.  A margin within 1e-9 um of a decision, a cell whose list
+ * may be incomplete (madi_geom_desc.cell_incomplete) or seeds nearer than
+ * 2e-3 um are left to the exact KD classifier, so every label is the exact
+ * classifier's and the walk's results are its bit for bit.  Needs a geometry
+ * with both the KD tree and the graph; tier_cap_um is not used. */
+#define MADI_CLASSIFY_FP64_CACHED 5
 
 /* Reference-semantics FP64 walk with the exact classifier at every endpoint.
  * Replaces _walk_kernel (madi/walker_gpu.py:520-656).  Random inputs come

@@ -1,0 +1,141 @@
+"""The FP64 walk's graph labels against the exact classifier, on the GPU.
+
+classify_mode 5 (the reference's "exact_cached" classifier mode,
+madi/walker_gpu.py:337-518) labels endpoints from the Delaunay graph in FP64
+from the walker's own cell, takes labels from the certified ball of the last
+one, and leaves decisions within rounding of a threshold (and cells whose
+lists may be incomplete) to the exact KD classifier.  The reference proves
+its cache equal to the exact classifier bit for bit
+(scripts/validate_exact_classifier_cache.py:73-140); here the graph-labelled
+walk must give the exact walk's Y, inside trace, escape and failure flags
+bit for bit (and the fused walk its sums and occupancy), with most endpoints
+labelled without the KD classifier: on a non-periodic Eq. S2-style box
+(walkers escape; hull cells defer to the KD classifier), a periodic box, a
+dense box with thin annuli, free water, and the reference's golden case
+through run_walk_Y(classifier="exact_cached").
+"""
+
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from custom_madi_amd import engine, walker_gpu
+from custom_madi_amd.config import SimConfig
+from custom_madi_amd.ensemble import DeviceGeometry, create_dummy_ensemble, geometry_arrays
+from tests import _geom
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+class _Cfg:
+    def __init__(self, n_steps, steps_per_h, n_grid, sigma, ts):
+        self.n_steps, self.steps_per_h, self.n_grid, self.sigma, self.ts = n_steps, steps_per_h, n_grid, sigma, ts
+
+
+def _pair(dg, n, cfg, pp, seed):
+    exact = engine.walk_y(dg, engine.walk_params(n, cfg, pp, seed))
+    cached = engine.walk_y(dg, engine.walk_params(n, cfg, pp, seed, classify_mode=engine.MADI_CLASSIFY_FP64_CACHED))
+    return exact, cached
+
+
+def _assert_same(exact, cached, what):
+    c = cached.counters
+    say(f"[fp64 graph] {what}: scans {c['classify']} ball answers {c['safe_hit']} KD {c['fp64_recheck']} "
+        f"moves {c['graph_iters']} (exact: {exact.counters['classify']} KD); kd nodes {c['kd_nodes']} vs "
+        f"{exact.counters['kd_nodes']}")
+    assert np.array_equal(cached.Y.cpu().numpy(), exact.Y.cpu().numpy())
+    assert np.array_equal(cached.inside_trace, exact.inside_trace)
+    assert np.array_equal(cached.escaped, exact.escaped) and np.array_equal(cached.err, exact.err)
+    # every endpoint is a classification or an answer from the certified ball
+    assert c["classify"] + c["safe_hit"] == exact.counters["classify"]
+    assert c["crossings"] == exact.counters["crossings"] and c["rejects"] == exact.counters["rejects"]
+
+
+def test_graph_labelled_walk_equals_exact_walk_non_periodic():
+    ens = _geom.cellular(L=40.0, seed=5)
+    dg = DeviceGeometry(geometry_arrays(ens))
+    cfg = _Cfg(1500, 50, 31, float(np.sqrt(2 * 2.0 * 0.02)), 0.02)
+    exact, cached = _pair(dg, 8192, cfg, 0.05, 11)
+    _assert_same(exact, cached, "cellular L 40")
+    assert exact.escaped.any()  # the box is small enough that some walkers leave it
+    assert cached.counters["safe_hit"] > 0
+    # the KD classifier runs for the starts and a small share of the endpoints
+    assert cached.counters["fp64_recheck"] < 0.05 * exact.counters["classify"]
+
+
+def test_cached_walk_equals_exact_walk_periodic_and_dense():
+    for name, ens, sigma, ts in (
+        ("periodic 64 cells", _geom.periodic(n_cells=64, seed=0), np.sqrt(2 * 2.0 * 0.05), 0.05),
+        ("periodic dense v_i 0.95", _geom.periodic(n_cells=300, rho_per_uL=4.0e6, vi=0.95, seed=2),
+         np.sqrt(2 * 2.0 * 0.01), 0.01),
+    ):
+        dg = DeviceGeometry(geometry_arrays(ens))
+        cfg = _Cfg(1200, 40, 31, float(sigma), ts)
+        exact, cached = _pair(dg, 8192, cfg, 0.2, 23)
+        _assert_same(exact, cached, name)
+        assert cached.counters["safe_hit"] > 0
+        assert cached.counters["fp64_recheck"] < 0.05 * exact.counters["classify"]
+
+
+def test_cached_fused_walk_equals_exact_fused_walk():
+    ens = _geom.cellular(L=150.0, seed=9)
+    dg = DeviceGeometry(geometry_arrays(ens))
+    G = 14
+    cfg = _Cfg(650, 50, G, float(np.sqrt(2 * 2.0 * 0.02)), 0.02)
+    spec = engine.PhaseSpec(j_delta=np.array([1, 2]), j_Delta=np.array([5, 9]), j_sum=np.array([6, 11]),
+                            pc=np.array([[0.0, 0.05, 0.2], [0.0, 0.1, 0.3]]), dirs=engine.AXIS_DIRECTIONS)
+    a = engine.walk_fused(dg, engine.walk_params(4096, cfg, 0.03, 321), spec, precision=64)
+    b = engine.walk_fused(dg, engine.walk_params(4096, cfg, 0.03, 321, classify_mode=engine.MADI_CLASSIFY_FP64_CACHED),
+                          spec, precision=64)
+    assert np.array_equal(a.cos, b.cos) and np.array_equal(a.sin, b.sin)
+    assert np.array_equal(a.occupancy, b.occupancy)
+    assert a.n_escaped == b.n_escaped == 0 and a.n_error == b.n_error == 0
+    assert b.counters["safe_hit"] > 0
+
+
+def test_free_water_walk_is_unchanged():
+    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=12.0, n_walkers=1024, n_ensembles=1, small_deltas=[2.0],
+                    big_deltas=[5.0], b_values=[0.0, 1000.0])
+    ens = create_dummy_ensemble(cfg)
+    dg = DeviceGeometry(geometry_arrays(ens))
+    exact, cached = _pair(dg, 1024, cfg, 0.5, 4)
+    _assert_same(exact, cached, "free water")
+
+
+def test_run_walk_Y_exact_cached_equals_the_reference_golden():
+    """classifier="exact_cached" through the drop-in API: the reference's own
+    CPU walk (tests/golden/walk_vectors.npz) within its golden tolerance, and
+    bitwise the exact mode's Y."""
+    ens, cfg, pp, g = _geom.reference_walk_golden()
+    stream = walker_gpu.make_walk_random_stream(cfg, int(g["scalars"][13]), ensemble=ens)
+    Yc, n_esc, tel = walker_gpu.run_walk_Y(ens, 0.0, cfg, verbose=False, pp=pp, return_telemetry=True,
+                                           classifier="exact_cached", random_stream=stream)
+    Ye = walker_gpu.run_walk_Y(ens, 0.0, cfg, verbose=False, pp=pp, random_stream=stream)[0]
+    assert n_esc == 0 and np.array_equal(Yc, Ye)
+    assert np.allclose(Yc, g["Y"], rtol=5e-11, atol=5e-12)
+    assert tel["classifier_cache"]["tier1"] > 0
+    assert tel["classifier_cache"]["requests"] == cfg.n_walkers * (cfg.n_steps + 1)
+    say(f"[fp64 graph] golden case: {tel['classifier_cache']}")
+
+
+def test_fp32_walks_reject_the_fp64_cache_mode():
+    ens = _geom.periodic(n_cells=64, seed=0)
+    dg = DeviceGeometry(geometry_arrays(ens))
+    cfg = _Cfg(100, 50, 3, float(np.sqrt(2 * 2.0 * 0.02)), 0.02)
+    spec = engine.PhaseSpec(j_delta=np.array([0]), j_Delta=np.array([1]), j_sum=np.array([2]),
+                            pc=np.array([[0.0]]), dirs=engine.AXIS_DIRECTIONS)
+    p = engine.walk_params(256, cfg, 0.1, 1, classify_mode=engine.MADI_CLASSIFY_FP64_CACHED)
+    with pytest.raises(ValueError, match="FP64 walk's label cache"):
+        engine.walk_fused(dg, p, spec, precision=32)
+    cfg.classifier_mode, cfg.precision = "exact_cached", "fp32"
+    assert engine.walk_params(256, cfg, 0.1, 1).classify_mode == 0  # FP32 never selects the FP64 cache
+    cfg.precision = "fp64"
+    assert engine.walk_params(256, cfg, 0.1, 1).classify_mode == engine.MADI_CLASSIFY_FP64_CACHED
