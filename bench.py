@@ -791,7 +791,21 @@ def measure_c1(args, log):
             "occupancy_max_abs_d": float(np.abs(res.occupancy - f64.occupancy).max() / N),
             "fp64_walk_s": t64, "fp64_kd_nodes": f64.counters["kd_nodes"],
         }
-        log(f"fp64 check: max |dS| {dS.max():.3e}, max |dS|/SE {ratio.max():.3f}")
+        # the same FP64 walk with graph labels (the reference's exact_cached
+        # mode, classify_mode 5): bitwise the exact walk's results
+        pg = engine.walk_params(N, cfg, pp, params.seed, classify_mode=engine.MADI_CLASSIFY_FP64_CACHED)
+        tg = time.perf_counter()
+        fg = engine.walk_fused(dg, pg, specv, precision=64)
+        tg = time.perf_counter() - tg
+        out["fp64_check"]["fp64_graph_labels"] = {
+            "walk_s": tg, "walker_steps_per_s": N * cfg.n_steps / tg, "speedup_over_exact": t64 / tg,
+            "bitwise_equal_to_exact": bool(np.array_equal(fg.cos, f64.cos) and np.array_equal(fg.sin, f64.sin)
+                                           and np.array_equal(fg.occupancy, f64.occupancy)),
+            "kd_classifications": fg.counters["fp64_recheck"], "graph_scans": fg.counters["classify"],
+            "ball_answers": fg.counters["safe_hit"],
+        }
+        log(f"fp64 check: max |dS| {dS.max():.3e}, max |dS|/SE {ratio.max():.3f}; graph-labelled FP64 walk "
+            f"{tg:.3f} s vs {t64:.3f} s exact")
     return out, (ens, cfg, pp)
 
 
