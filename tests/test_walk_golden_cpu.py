@@ -44,3 +44,17 @@ def test_oracle_walk_equals_the_reference_cpu_walk():
     assert np.count_nonzero(np.diff(g["inside_trace"].astype(int), axis=1)) >= 10
     cs, sn = O.reduce(g["Y"], g["cols_jd"], g["cols_jD"], g["cols_js"], g["cols_pc"])
     assert np.allclose(cs, g["cos"], rtol=1e-12, atol=1e-12) and np.allclose(sn, g["sin"], rtol=1e-12, atol=1e-12)
+
+
+def test_cpu_baseline_restatement_equals_the_reference_cpu_walk():
+    """oracle.walk_cpu_reference (the CPU algorithm the benchmark times as the
+    reference's CPU fallback) gives the reference's _walk_cpu Y and inside
+    trace bit for bit on the golden case."""
+    ens, cfg, pp, g = _geom.reference_walk_golden()
+    stream = walker_gpu.make_walk_random_stream(cfg, int(g["scalars"][13]), ensemble=ens)
+    Y, itr, esc = O.walk_cpu_reference(ens.seeds, ens.annulus, ens.L, ens.source_lo, ens.source_hi, cfg.n_walkers,
+                                       cfg.n_steps, cfg.steps_per_h, cfg.n_grid, cfg.sigma, cfg.ts, pp,
+                                       stream=(stream.initial_positions, stream.increments,
+                                               stream.acceptance_uniforms))
+    assert np.array_equal(Y, g["Y"]) and np.array_equal(itr, g["inside_trace"])
+    assert np.array_equal(esc.astype(bool), g["escaped"].astype(bool))
