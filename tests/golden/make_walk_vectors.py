@@ -73,5 +73,45 @@ def main() -> None:
     print(f"wrote {OUT}")
 
 
+def time_cpu_walks() -> None:
+    """With --time: the reference's _walk_cpu beside this repository's
+    restatement (oracle.walk_cpu_reference, the benchmark's CPU baseline) on a
+    larger seeded case, same stream; prints both rates and whether Y agrees."""
+    import time
+
+    sys.path.insert(1, str(Path(__file__).resolve().parents[2]))
+    from oracle import oracle as O
+
+    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=4.0, h_ms=0.2, n_walkers=400, small_deltas=[0.4], big_deltas=[1.2],
+                    b_values=[0.0])
+    rho, side, vi = 5.0e5, 60.0, 0.6
+    seeds, _, nn, _ = RE._make_population(RE._rho_um3(rho), side, cfg, np.random.default_rng(3))
+    annulus = np.minimum(RE.alpha_star_from_vi(vi, rho), cfg.kappa * nn / 2.0)
+    nodes = RE._build_gpu_kdtree(seeds)
+    src = cfg.source_fraction * side
+    lo = 0.5 * (side - src)
+    ens = RE.Ensemble(seeds=seeds, annulus=annulus, rho=rho, V=float("nan"), vi=float("nan"), alpha_star=0.0,
+                      L=side, source_lo=lo, source_hi=lo + src, mean_AV=1.0, geometry=None, rho_requested=rho,
+                      V_requested=float("nan"), kd_node_seed=nodes[0], kd_node_axis=nodes[1], kd_node_left=nodes[2],
+                      kd_node_right=nodes[3], kd_node_parent=nodes[4])
+    stream = RW.make_walk_random_stream(cfg, 9, ensemble=ens)
+    t = time.perf_counter()
+    Y, _, itr, _, _ = RW._walk_cpu(ens, 0.02, cfg, random_stream=stream)
+    t_ref = time.perf_counter() - t
+    t = time.perf_counter()
+    Y2, itr2, _ = O.walk_cpu_reference(seeds, annulus, side, lo, lo + src, cfg.n_walkers, cfg.n_steps,
+                                       cfg.steps_per_h, cfg.n_grid, cfg.sigma, cfg.ts, 0.02,
+                                       stream=(stream.initial_positions, stream.increments,
+                                               stream.acceptance_uniforms))
+    t_port = time.perf_counter() - t
+    ws = cfg.n_walkers * cfg.n_steps
+    print(f"{len(seeds)} seeds, {ws} walker-steps: reference _walk_cpu {t_ref:.2f} s ({ws / t_ref:.3e}/s), "
+          f"restatement {t_port:.2f} s ({ws / t_port:.3e}/s); Y equal {np.array_equal(Y, Y2)}, "
+          f"traces equal {np.array_equal(itr, itr2)}")
+
+
 if __name__ == "__main__":
-    main()
+    if "--time" in sys.argv:
+        time_cpu_walks()
+    else:
+        main()
