@@ -21,13 +21,20 @@ resident on the device.  The same line carries:
   processes, and the FP64 C oracle with OpenMP.
 
 * ``universal_reduce``: the universal library's FP64 column reduce at
-  production size (50,000 walkers x 129 Y samples x 31,125 columns).
+  production size (50,000 walkers x 129 Y samples x 31,125 columns);
+* ``production``: the reference's own published benchmark (walk plus
+  31,125-column reduction of one Eq. S8 ensemble, 50,000 walkers x 128,000
+  steps, k_io 20 /s, at the production grid's low / centre / high rho)
+  through the drop-in API in three ways -- the FP64 exact walk, the FP64
+  exact_cached walk (bitwise the exact one) and the FP32 walk -- each beside
+  the published A100 time.
 
 Other workloads (--workload): c2, c3 on their own, c4 (a seeded sample of the
 full-range sweep), map (MAP matching of 1e6 voxels against a 14,105-entry
 library, FP64), reduce (the column reduce alone, with its CPU baseline),
 library (the drop-in library build at the default configuration, group by
-group against the batched scheduler).  --gpus N > 1 exits: this benchmark uses one GPU.
+group against the batched scheduler), production (the production object
+alone).  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
 from __future__ import annotations
@@ -444,6 +451,107 @@ def run_reduce(args):
     return 0
 
 
+# The reference's published walk-plus-reduction times on a full NVIDIA
+# A100-SXM4-80GB, one ensemble x one k_io at the production grid's low,
+# centre and high rho (docs/v5_fast_classifier_launch_readiness.md:210-219).
+PUBLISHED_A100_S = {"exact": {"low": 34.825, "center": 52.957, "high": 87.590},
+                    "exact_cached": {"low": 3.533, "center": 5.105, "high": 16.260}}
+
+
+def production_coordinate(which: str):
+    """The reference benchmark's coordinate (analysis/v5_runtime_benchmark.py:101-123):
+    rho index 0 / 21 / last of the production log grid, the admissible V whose
+    v_i is nearest (in log) to the geometric centre of the grid's v_i range."""
+    from custom_madi_amd.library import make_remediation_log_grid
+    grid = make_remediation_log_grid()
+    idx = {"low": 0, "center": 21, "high": len(grid.rhos) - 1}[which]
+    target = float(np.sqrt(grid.vi_min * grid.vi_max))
+    rho = float(grid.rhos[idx])
+    cands = [(float(V), rho * float(V) * 1e-6) for V in grid.Vs if grid.vi_min <= rho * float(V) * 1e-6 <= grid.vi_max]
+    V, vi = min(cands, key=lambda c: abs(np.log(c[1] / target)))
+    return rho, V, vi
+
+
+def measure_production(args, log):
+    """The reference's own production benchmark, through the drop-in API:
+    _walk_and_reduce_one_ensemble of one Eq. S8 ensemble (50,000 walkers x
+    128,000 steps of 1 us, D0 3.0) at k_io 20 /s, reduced onto the 31,125
+    production columns, at the three density coordinates; geometry and the
+    first launch are outside the timed region, as in the reference's timing.
+    Three ways: the FP64 exact walk (reference semantics), the FP64
+    exact_cached walk (graph labels, bitwise the exact one) and the FP32
+    production walk; each against the published A100 times."""
+    from dataclasses import replace
+    from custom_madi_amd import _hip, signal, walker_gpu
+    from custom_madi_amd.config import SimConfig
+    _hip.require_gpu()
+    torch = _hip.torch_mod()
+    base = SimConfig(n_ensembles=1)
+    cols = signal.build_columns(base)
+    args_cols = (cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    modes = {"exact": base, "exact_cached": replace(base, classifier_mode="exact_cached"),
+             "fp32": replace(base, precision="fp32")}
+    out = {}
+    for which in ("low", "center", "high"):
+        rho, V, vi = production_coordinate(which)
+        t0 = time.perf_counter()
+        ens = walker_gpu._build_ensembles_for_entry(rho, V, base, seed=args.prod_seed, verbose=False)[0]
+        ens.device_geometry()
+        geom_s = time.perf_counter() - t0
+        n_pop = int(len(ens.seeds))
+        log(f"production {which}: rho {rho:.4g}/uL, V {V:.4g} pL, v_i {vi:.3f}: {n_pop} seeds, geometry {geom_s:.1f} s")
+        pp = walker_gpu._checked_pp(walker_gpu.kio_to_pp(20.0, ens.mean_AV, base))
+        seed = walker_gpu._walk_seed(args.prod_seed, 0)
+        row = {"rho_per_uL": rho, "V_pL": V, "target_vi": vi, "n_seeds_pop": n_pop, "geometry_s": geom_s, "pp": pp}
+        res = {}
+        for mode, cfg in modes.items():
+            # first launch of each kernel outside the timed region
+            walker_gpu._walk_and_reduce_one_ensemble(ens, pp, replace(cfg, n_walkers=64), *args_cols, seed)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            r = walker_gpu._walk_and_reduce_one_ensemble(ens, pp, cfg, *args_cols, seed)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            res[mode] = r
+            steps = cfg.n_walkers * cfg.n_steps
+            entry = {"walk_and_reduction_s": dt, "walker_steps_per_s": steps / dt, "n_escaped": int(r.n_escaped)}
+            if mode in PUBLISHED_A100_S:
+                pub = PUBLISHED_A100_S[mode][which]
+                entry["published_a100_s"] = pub
+                entry["speedup_over_published_a100"] = pub / dt
+            entry["speedup_over_published_a100_cached"] = PUBLISHED_A100_S["exact_cached"][which] / dt
+            row[mode] = entry
+            log(f"production {which} {mode}: {dt:.3f} s ({steps / dt:.3e} walker-steps/s)")
+        a, b = res["exact"], res["exact_cached"]
+        row["exact_cached_bitwise_equal_to_exact"] = bool(
+            np.array_equal(a.cos_sum, b.cos_sum) and np.array_equal(a.sin_sum, b.sin_sum)
+            and np.array_equal(a.occupancy_counts, b.occupancy_counts))
+        n3 = 3.0 * base.n_walkers
+        row["fp32_vs_fp64_max_abs_dS"] = float(np.max(np.abs(res["fp32"].cos_sum - a.cos_sum)) / n3)
+        out[which] = row
+        del ens
+    return {"workload": "production: the reference's walk-plus-reduction benchmark (one Eq. S8 ensemble, 50,000 "
+                        "walkers x 128,000 steps of 1 us, D0 3.0, k_io 20/s, 31,125 columns) at the production "
+                        "grid's low / centre / high rho", "coordinates": out}
+
+
+def run_production(args):
+    log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
+    r = measure_production(args, log)
+    c = r["coordinates"]
+    hi = c["high"]["exact_cached"]
+    out = {
+        "metric": "walk + reduction seconds per ensemble x k_io (the reference's production benchmark)",
+        "value": hi["walk_and_reduction_s"], "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 1,
+        "ms_per_step": 1e3 * hi["walk_and_reduction_s"], "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": hi["speedup_over_published_a100"], "dtype": "f64 (reference semantics)", "data": "synthetic",
+        "config": {"workload": r["workload"], "headline": "high rho, FP64 exact_cached"},
+        "coordinates": c,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def run_library(args):
     """The drop-in library build (build_library_from_triplets) at the default
     production configuration, group by group and on the batched scheduler."""
@@ -618,7 +726,9 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="target seconds per CPU baseline sample")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for HBM counter scaling)")
-    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce", "library"], default="c1")
+    ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce", "library", "production"],
+                    default="c1")
+    ap.add_argument("--prod-seed", type=int, default=0, help="production: geometry and walk seed")
     ap.add_argument("--voxels", type=int, default=1_000_000, help="map: voxels matched per step")
     ap.add_argument("--library-entries", type=int, default=14_105, help="map: library entries")
     ap.add_argument("--entries", type=int, default=256)
@@ -640,6 +750,7 @@ def main(argv=None):
     ap.add_argument("--no-c2", action="store_true", help="default line without the c2 object")
     ap.add_argument("--no-c3", action="store_true", help="default line without the c3 object")
     ap.add_argument("--no-reduce", action="store_true", help="default line without the universal_reduce object")
+    ap.add_argument("--no-production", action="store_true", help="default line without the production object")
     ap.add_argument("--library-groups", type=int, default=2, help="library: (rho, V) groups (at most 3)")
     ap.add_argument("--library-kios", type=int, default=3, help="library: k_io values per group (at most 3)")
     ap.add_argument("--library-out", default=None)
@@ -659,6 +770,8 @@ def main(argv=None):
         return run_map(args)
     if args.workload == "reduce":
         return run_reduce(args)
+    if args.workload == "production":
+        return run_production(args)
     if args.workload == "library":
         return run_library(args)
 
@@ -677,6 +790,9 @@ def main(argv=None):
         out["c3"] = measure_c3(args, pool, lambda m: log(f"c3: {m}"), args.c3_duration)
     if not args.no_reduce:
         out["universal_reduce"] = measure_reduce(args, log, cpu_baseline=False)
+    if not args.no_production:
+        log("production: the reference's walk-plus-reduction benchmark at its three densities")
+        out["production"] = measure_production(args, log)
     if not args.no_cpu_baseline:
         procs, vis = max(1, int(args.workers)), visible_cpus()
         single, multi = cpu_baseline_reference(ens, cfg, pp, pool, procs, args.cpu_seconds, log)
