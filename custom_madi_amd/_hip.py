@@ -29,6 +29,7 @@ MADI_MAX_PAIRS = 8
 MADI_MAX_B = 16
 MADI_MAX_DIRS = 64
 MADI_MAX_REDUCE_COLS = 2_000_000
+MADI_MAX_MULTI_WALKS = 64
 N_COUNTERS = 13
 N_FLAGS = 2
 MADI_FLAG_ESCAPED = 0
@@ -117,6 +118,7 @@ SIGNATURES = {
     "madi_classify_graph": (C.c_int, [_p, _p, C.c_int64, _p, _p, _p, _p, _p]),
     "madi_rng_init": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _p, _p]),
     "madi_walk_f64": (C.c_int, [_p, C.POINTER(WalkParams), _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "madi_walk_f64_multi": (C.c_int, [_p, C.POINTER(WalkParams), C.c_int32, _p, _p, _p, _p, _p, _p, _p]),
     "madi_reduce_f64": (C.c_int, [_p, C.c_int64, C.c_int32, _p, _p, _p, _p, C.c_int32, _p, _p, _p]),
     "madi_walk_fused_workspace": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_walk_fused": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, C.c_int64,

@@ -1223,6 +1223,43 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
     return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
+int madi_walk_f64_multi(const madi_geom* g, const madi_walk_params* p, int32_t n_walks, const double* pp,
+                        double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err, int64_t* d_counters,
+                        void* stream)
+{
+    WalkDev W;
+    int rc = make_walk(p, W);
+    if (rc) return rc;
+    if (!g || n_walks < 1 || n_walks > MADI_MAX_MULTI_WALKS || !pp ||
+        (W.n_walkers > 0 && (!d_Y || !d_inside_trace || !d_escaped || !d_err))) {
+        set_error("madi_walk_f64_multi: invalid arguments (n_walks must lie in 1..%d)", MADI_MAX_MULTI_WALKS);
+        return MADI_ERR_ARG;
+    }
+    MultiPP mp;
+    memset(&mp, 0, sizeof mp);
+    mp.n = n_walks;
+    for (int k = 0; k < n_walks; ++k) {
+        if (!(pp[k] >= 0.0 && pp[k] <= 1.0)) {
+            set_error("p_p must be finite and in [0, 1], got %g (walk %d)", pp[k], k);
+            return MADI_ERR_ARG;
+        }
+        mp.pp[k] = pp[k];
+    }
+    if (!kd_ready(g->dev)) {
+        set_error("madi_walk_f64_multi: geometry has no KD tree");
+        return MADI_ERR_ARG;
+    }
+    if (p->classify_mode == MADI_CLASSIFY_FP64_CACHED && !graph64_ready(g->dev)) {
+        set_error("madi_walk_f64_multi: graph labels (classify_mode 5) need the geometry's KD tree and Delaunay graph");
+        return MADI_ERR_ARG;
+    }
+    const Gf2Col* jp = nullptr;
+    if ((rc = get_jump_pow(&jp))) return rc;
+    rc = launch_walk_f64_multi(g->dev, W, jp, mp, d_Y, d_inside_trace, d_escaped, d_err,
+                               (unsigned long long*)d_counters, (hipStream_t)stream);
+    return rc ? rc : mark_used(g, (hipStream_t)stream);
+}
+
 int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
                     const int32_t* d_js, const double* d_pc, int32_t n_cols, double* d_cos, double* d_sin,
                     void* stream)

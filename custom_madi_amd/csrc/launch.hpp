@@ -91,6 +91,12 @@ struct WorkDev {
     double* y_out;  // NULL, or Y at every sample: walker x n_grid x 3 (streamed walks only)
 };
 
+// p_p of each of several FP64 walks run in one launch (MADI_MAX_MULTI_WALKS)
+struct MultiPP {
+    int32_t n;
+    double pp[MADI_MAX_MULTI_WALKS];
+};
+
 void set_error(const char* fmt, ...);
 int check_launch(const char* what);
 
@@ -102,6 +108,8 @@ int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, 
                     hipStream_t s);
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
                           unsigned long long* counters, hipStream_t s);
+int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const MultiPP& mp, double* Y,
+                          int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters, hipStream_t s);
 // Sample chunks of the column reduce for n walkers and n_cols columns (the
 // partial buffer holds n_cols x chunks x 2 doubles).
 int32_t reduce_chunks(int64_t n, int32_t n_cols);

@@ -131,8 +131,19 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                                                     const double* __restrict__ gu, double* __restrict__ Y,
                                                     int8_t* __restrict__ itr, int8_t* __restrict__ esc_out,
                                                     int8_t* __restrict__ err_out, FusedOut fo,
-                                                    unsigned long long* counters)
+                                                    unsigned long long* counters, MultiPP mp)
 {
+    // several walks that differ only in p_p (mp.n > 0, kY): walk blockIdx.y
+    // takes mp.pp[blockIdx.y] and its own slices of the outputs and counters
+    if (MODE == kY && mp.n > 0) {
+        const size_t kw = blockIdx.y, n = (size_t)p.n_walkers;
+        p.pp = mp.pp[kw];
+        Y += kw * n * (size_t)p.n_grid * 3;
+        itr += kw * n * (size_t)p.n_grid;
+        esc_out += kw * n;
+        err_out += kw * n;
+        if (counters) counters += kw * MADI_N_COUNTERS;
+    }
     __shared__ KdNodeF sf[kLdsNodes];
     __shared__ KdNodeI si[kLdsNodes];
     __shared__ double lds_cols[kWaves * kColChunk * 2];
@@ -625,9 +636,21 @@ int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, 
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
     FusedOut fo{};
+    MultiPP mp{};
     hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, gi, gd, gu, Y,
-                       itr, esc, err, fo, counters);
+                       itr, esc, err, fo, counters, mp);
     return check_launch("k_walk_f64");
+}
+
+int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const MultiPP& mp, double* Y,
+                          int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters, hipStream_t s)
+{
+    if (p.n_walkers == 0 || mp.n == 0) return MADI_OK;
+    const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
+    FusedOut fo{};
+    hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks, (unsigned)mp.n), dim3(kBlock), 0, s, g, p, jump_pow,
+                       nullptr, nullptr, nullptr, Y, itr, esc, err, fo, counters, mp);
+    return check_launch("k_walk_f64 (several walks)");
 }
 
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
@@ -635,8 +658,9 @@ int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
 {
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
+    MultiPP mp{};
     hipLaunchKernelGGL(k_walk_f64<kFused>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, fo, counters);
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, fo, counters, mp);
     return check_launch("k_walk_f64_fused");
 }
 

@@ -9,7 +9,7 @@ from __future__ import annotations
 import ctypes as C
 import math
 from dataclasses import dataclass
-from typing import Optional, Sequence
+from typing import List, Optional, Sequence
 
 import numpy as np
 
@@ -176,6 +176,31 @@ def walk_y(geom: DeviceGeometry, params: _hip.WalkParams, stream=None) -> WalkY:
     torch.cuda.synchronize()
     return WalkY(Y, itr.cpu().numpy(), esc.cpu().numpy(), err.cpu().numpy(),
                  dict(zip(_hip.COUNTER_NAMES, cnt.cpu().numpy().tolist())))
+
+
+def walk_y_multi(geom: DeviceGeometry, params: _hip.WalkParams, pps) -> List[WalkY]:
+    """Several FP64 walks that differ only in p_p, in one launch
+    (madi_walk_f64_multi): walk k equals walk_y with params.pp = pps[k], bit
+    for bit."""
+    torch = _hip.require_gpu()
+    K = len(pps)
+    if not 1 <= K <= _hip.MADI_MAX_MULTI_WALKS:
+        raise ValueError(f"1 to {_hip.MADI_MAX_MULTI_WALKS} walks per launch, got {K}")
+    n, G = int(params.n_walkers), int(params.n_grid)
+    Y = torch.empty((K, n, G, 3), dtype=torch.float64, device="cuda")
+    itr = torch.empty((K, n, G), dtype=torch.int8, device="cuda")
+    esc = torch.empty((K, n), dtype=torch.int8, device="cuda")
+    err = torch.empty((K, n), dtype=torch.int8, device="cuda")
+    cnt = torch.zeros((K, _hip.N_COUNTERS), dtype=torch.int64, device="cuda")
+    pp = np.ascontiguousarray(pps, dtype=np.float64)
+    lib = _hip.load_library()
+    rc = lib.madi_walk_f64_multi(geom.handle, C.byref(params), K, pp.ctypes.data, _hip.dptr(Y), _hip.dptr(itr),
+                                 _hip.dptr(esc), _hip.dptr(err), _hip.dptr(cnt), _hip.stream_ptr())
+    _hip.check(rc, "madi_walk_f64_multi")
+    torch.cuda.synchronize()
+    itr_h, esc_h, err_h, cnt_h = itr.cpu().numpy(), esc.cpu().numpy(), err.cpu().numpy(), cnt.cpu().numpy()
+    return [WalkY(Y[k], itr_h[k], esc_h[k], err_h[k], dict(zip(_hip.COUNTER_NAMES, cnt_h[k].tolist())))
+            for k in range(K)]
 
 
 @dataclass

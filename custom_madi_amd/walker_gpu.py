@@ -367,6 +367,67 @@ def _walk_and_reduce_one_ensemble(ens: Ensemble, pp: float, cfg: SimConfig, j_de
     )
 
 
+# Y of the FP64 walks run together in one launch (the rest of the sweep
+# follows in further launches)
+_MULTI_WALK_Y_BYTES = 16 << 30
+
+
+def _walk_and_reduce_kio_sweep(ens: Ensemble, pps: Sequence[float], cfg: SimConfig, j_delta, j_Delta, j_sum,
+                               phase_coef, walk_seed: int) -> list:
+    """[_walk_and_reduce_one_ensemble(ens, pp, ...) for pp in pps], bit for bit.
+
+    The reference walks an ensemble once per k_io with one walk seed
+    (madi/walker_gpu.py:1193-1226).  The FP64 walks whose columns are reduced
+    from Y run several per launch (madi_walk_f64_multi: the same kernel per
+    walk, so the same results): one walk of 50,000 walkers fills a tenth of
+    the GPU, the sweep together fills it."""
+    pps = [_checked_pp(pp) for pp in pps]
+    fused = engine.spec_from_columns(j_delta, j_Delta, j_sum, phase_coef)
+    if cfg.precision == "fp32" or fused is not None or len(pps) < 2:
+        return [_walk_and_reduce_one_ensemble(ens, pp, cfg, j_delta, j_Delta, j_sum, phase_coef, walk_seed)
+                for pp in pps]
+    _require_device(None)
+    K, n_cols, G = len(pps), len(phase_coef), int(cfg.n_grid)
+    cos_t = [np.zeros(n_cols) for _ in range(K)]
+    sin_t = [np.zeros(n_cols) for _ in range(K)]
+    occ_t = [np.zeros(G, dtype=np.int64) for _ in range(K)]
+    stats = [ClassifierCacheStats() for _ in range(K)]
+    total = 0
+    chunk = int(cfg.walker_chunk or cfg.n_walkers)
+    geom = ens.device_geometry()
+    for offset in range(0, int(cfg.n_walkers), chunk):
+        n_this = min(chunk, int(cfg.n_walkers) - offset)
+        params = engine.walk_params(n_this, cfg, pps[0], walk_seed + offset)
+        per_launch = int(max(1, min(_hip.MADI_MAX_MULTI_WALKS, _MULTI_WALK_Y_BYTES // max(1, n_this * G * 24))))
+        for k0 in range(0, K, per_launch):
+            outs = engine.walk_y_multi(geom, params, pps[k0:k0 + per_launch])
+            for k, out in zip(range(k0, k0 + per_launch), outs):
+                if np.any(out.err):
+                    raise RuntimeError("GPU full-facet KD-tree classification failed")
+                n_esc = int(np.sum(out.escaped))
+                if n_esc:
+                    raise RuntimeError(f"SI §S.III fatal boundary condition: {n_esc}/{n_this} walkers exited Ω_sim.")
+                c, sn = engine.reduce_columns(out.Y, j_delta, j_Delta, j_sum, phase_coef)
+                cos_t[k] += c
+                sin_t[k] += sn
+                occ_t[k] += np.sum(out.inside_trace, axis=0, dtype=np.int64)
+                st = _stats_from_counters(out.counters, n_this * (cfg.n_steps + 1))
+                stats[k].full += st.full
+                stats[k].tier1 += st.tier1
+                for key, v in st.device_counters.items():
+                    stats[k].device_counters[key] = stats[k].device_counters.get(key, 0) + v
+            del outs
+        total += n_this
+    return [ReducedResult(
+        cos_sum=cos_t[k], sin_sum=sin_t[k], n_walkers=total, n_escaped=0, occupancy_counts=occ_t[k],
+        pp_values=[float(pps[k])],
+        analytic_kio_eq5_values=[pp_to_kio_eq5(pps[k], ens.mean_AV, cfg)] if not ens.is_free_water else [0.0],
+        geometry_stats=[ens.geometry.to_dict()],
+        ensemble_cos_means=[cos_t[k] / float(3 * total)], ensemble_sin_means=[sin_t[k] / float(3 * total)],
+        classifier_cache_stats=[stats[k].as_dict()],
+    ) for k in range(K)]
+
+
 def _merge_results(results: Sequence[ReducedResult]) -> ReducedResult:
     """Merge per-ensemble results in the given (ensemble) order (madi/walker_gpu.py:1066-1090)."""
     if not results:
@@ -457,12 +518,10 @@ def run_simulation_multi_kio_reduced(rho: float, V: float, kios: Sequence[float]
     outputs = {k: [] for k in kios}
     for e, ens in enumerate(ensembles):
         ws = _walk_seed(seed, e)
-        for kio in kios:
-            if ens.is_free_water or kio == 0.0:
-                pp = 0.0
-            else:
-                pp = _checked_pp(kio_to_pp(kio, ens.mean_AV, cfg))
-            res = _walk_and_reduce_one_ensemble(ens, pp, cfg, j_delta, j_Delta, j_sum, phase_coef, ws)
+        pps = [0.0 if (ens.is_free_water or kio == 0.0) else _checked_pp(kio_to_pp(kio, ens.mean_AV, cfg))
+               for kio in kios]
+        for kio, pp, res in zip(kios, pps, _walk_and_reduce_kio_sweep(ens, pps, cfg, j_delta, j_Delta, j_sum,
+                                                                       phase_coef, ws)):
             outputs[kio].append(res)
             if verbose:
                 print(f"    ensemble {e + 1}/{len(ensembles)}, k_io={kio:g}: p_p={pp:.7g}, "

@@ -269,6 +269,21 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
                   const double* d_gold_u, double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err,
                   int64_t* d_counters, void* stream);
 
+/* Several FP64 walks of one geometry that differ only in p_p, in one launch:
+ * walk k is madi_walk_f64 with p->pp replaced by pp[k] (host array), bit for
+ * bit, device RNG only.  The reference walks an ensemble once per k_io of its
+ * sweep with one walk seed (run_simulation_multi_kio_reduced,
+ * madi/walker_gpu.py:1170-1233, common random numbers); a walk of 50,000
+ * walkers occupies a tenth of the GPU, and the k_io walks together fill it.
+ * Outputs hold the walks one after another: d_Y n_walks x n x n_grid x 3,
+ * d_inside_trace n_walks x n x n_grid, d_escaped and d_err n_walks x n,
+ * d_counters (or NULL) n_walks x MADI_N_COUNTERS, zeroed by the caller.
+ * 1 <= n_walks <= MADI_MAX_MULTI_WALKS. */
+#define MADI_MAX_MULTI_WALKS 64
+int madi_walk_f64_multi(const madi_geom* g, const madi_walk_params* p, int32_t n_walks, const double* pp,
+                        double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err, int64_t* d_counters,
+                        void* stream);
+
 /* Workspace bytes of madi_walk_y_f32 for n_walkers. */
 int madi_walk_y_f32_workspace(int64_t n_walkers, int64_t* bytes);
 

@@ -139,3 +139,29 @@ def test_fp32_walks_reject_the_fp64_cache_mode():
     assert engine.walk_params(256, cfg, 0.1, 1).classify_mode == 0  # FP32 never selects the FP64 cache
     cfg.precision = "fp64"
     assert engine.walk_params(256, cfg, 0.1, 1).classify_mode == engine.MADI_CLASSIFY_FP64_CACHED
+
+
+@pytest.mark.parametrize("mode", ["exact", "exact_cached"])
+def test_fp64_kio_sweep_in_one_launch_equals_walks_one_by_one(mode, monkeypatch):
+    """The FP64 walks of a k_io sweep run several per launch
+    (madi_walk_f64_multi) and give each walk's results bit for bit, across
+    walker chunks and with the sweep split over launches."""
+    from custom_madi_amd import signal
+    ens = _geom.periodic(n_cells=64, seed=3)
+    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=12.0, n_walkers=2048, walker_chunk=1024, n_ensembles=1,
+                    small_deltas=[1.0, 2.0, 3.0], big_deltas=[4.0, 5.0, 6.0, 7.0, 8.0],
+                    b_values=[0.0, 1000.0, 2500.0], classifier_mode=mode)
+    cols = signal.build_columns(cfg)
+    args = (cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    assert engine.spec_from_columns(*args) is None  # reduced from Y
+    pps = [0.0, 0.01, 0.05, 0.3, 1.0]
+    one = [walker_gpu._walk_and_reduce_one_ensemble(ens, pp, cfg, *args, 77) for pp in pps]
+    # two walks per launch: the sweep takes three launches per walker chunk
+    monkeypatch.setattr(walker_gpu, "_MULTI_WALK_Y_BYTES", 2 * 1024 * cfg.n_grid * 24)
+    many = walker_gpu._walk_and_reduce_kio_sweep(ens, pps, cfg, *args, 77)
+    for a, b in zip(one, many):
+        assert np.array_equal(a.cos_sum, b.cos_sum) and np.array_equal(a.sin_sum, b.sin_sum)
+        assert np.array_equal(a.occupancy_counts, b.occupancy_counts) and a.n_walkers == b.n_walkers
+        assert a.classifier_cache_stats == b.classifier_cache_stats
+        assert a.pp_values == b.pp_values and a.analytic_kio_eq5_values == b.analytic_kio_eq5_values
+    say(f"[fp64 sweep] {mode}: {len(pps)} walks equal one by one and together")
