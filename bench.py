@@ -472,7 +472,7 @@ def production_coordinate(which: str):
     return rho, V, vi
 
 
-def measure_production(args, log):
+def measure_production(args, log, sweep=False):
     """The reference's own production benchmark, through the drop-in API:
     _walk_and_reduce_one_ensemble of one Eq. S8 ensemble (50,000 walkers x
     128,000 steps of 1 us, D0 3.0) at k_io 20 /s, reduced onto the 31,125
@@ -480,10 +480,14 @@ def measure_production(args, log):
     first launch are outside the timed region, as in the reference's timing.
     Three ways: the FP64 exact walk (reference semantics), the FP64
     exact_cached walk (graph labels, bitwise the exact one) and the FP32
-    production walk; each against the published A100 times."""
+    production walk; each against the published A100 times.  The FP32 |dS|
+    is set beside the FP64 walk's own seed-to-seed |dS|.  With sweep, also
+    the library build's 51-k_io sweep of the ensemble in FP64 exact_cached
+    (its walks run together, madi_walk_f64_multi)."""
     from dataclasses import replace
     from custom_madi_amd import _hip, signal, walker_gpu
     from custom_madi_amd.config import SimConfig
+    from custom_madi_amd.library import make_remediation_log_grid
     _hip.require_gpu()
     torch = _hip.torch_mod()
     base = SimConfig(n_ensembles=1)
@@ -527,7 +531,38 @@ def measure_production(args, log):
             np.array_equal(a.cos_sum, b.cos_sum) and np.array_equal(a.sin_sum, b.sin_sum)
             and np.array_equal(a.occupancy_counts, b.occupancy_counts))
         n3 = 3.0 * base.n_walkers
-        row["fp32_vs_fp64_max_abs_dS"] = float(np.max(np.abs(res["fp32"].cos_sum - a.cos_sum)) / n3)
+        d32 = (res["fp32"].cos_sum - a.cos_sum) / n3
+        row["fp32_vs_fp64_max_abs_dS"] = float(np.max(np.abs(d32)))
+        row["fp32_vs_fp64_rms_dS"] = float(np.sqrt(np.mean(d32 ** 2)))
+        # the Monte Carlo noise of S at this size: the same FP64 walk with another seed
+        r2 = walker_gpu._walk_and_reduce_one_ensemble(ens, pp, modes["exact_cached"], *args_cols,
+                                                      walker_gpu._walk_seed(args.prod_seed, 1))
+        dseed = (r2.cos_sum - b.cos_sum) / n3
+        row["fp64_seed_to_seed_max_abs_dS"] = float(np.max(np.abs(dseed)))
+        row["fp64_seed_to_seed_rms_dS"] = float(np.sqrt(np.mean(dseed ** 2)))
+        log(f"production {which}: |dS| fp32 vs fp64 max {row['fp32_vs_fp64_max_abs_dS']:.2e} rms "
+            f"{row['fp32_vs_fp64_rms_dS']:.2e}; fp64 seed to seed max {row['fp64_seed_to_seed_max_abs_dS']:.2e} rms "
+            f"{row['fp64_seed_to_seed_rms_dS']:.2e}")
+        if sweep:
+            # the library build's sweep: every production k_io of this ensemble, FP64 exact_cached
+            kios = make_remediation_log_grid().kios
+            pps = [walker_gpu._checked_pp(walker_gpu.kio_to_pp(float(k), ens.mean_AV, base)) for k in kios]
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            rs = walker_gpu._walk_and_reduce_kio_sweep(ens, pps, modes["exact_cached"], *args_cols, seed)
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t
+            k20 = int(np.flatnonzero(kios == 20.0)[0])
+            one = row["exact_cached"]["walk_and_reduction_s"]
+            row["kio_sweep"] = {
+                "n_kio": len(kios), "s": dt, "s_per_kio": dt / len(kios), "speedup_over_one_walk": one * len(kios) / dt,
+                "speedup_over_published_a100_cached": PUBLISHED_A100_S["exact_cached"][which] * len(kios) / dt,
+                "kio_20_bitwise_equal_to_single_walk": bool(
+                    np.array_equal(rs[k20].cos_sum, b.cos_sum) and np.array_equal(rs[k20].sin_sum, b.sin_sum)
+                    and np.array_equal(rs[k20].occupancy_counts, b.occupancy_counts)),
+            }
+            log(f"production {which}: k_io sweep of {len(kios)} walks {dt:.2f} s ({dt / len(kios):.3f} s per k_io)")
+            del rs
         out[which] = row
         del ens
     return {"workload": "production: the reference's walk-plus-reduction benchmark (one Eq. S8 ensemble, 50,000 "
@@ -537,7 +572,7 @@ def measure_production(args, log):
 
 def run_production(args):
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
-    r = measure_production(args, log)
+    r = measure_production(args, log, sweep=True)
     c = r["coordinates"]
     hi = c["high"]["exact_cached"]
     out = {

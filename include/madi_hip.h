@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 7
+#define MADI_ABI_VERSION 8
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -274,7 +274,8 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
  * bit, device RNG only.  The reference walks an ensemble once per k_io of its
  * sweep with one walk seed (run_simulation_multi_kio_reduced,
  * madi/walker_gpu.py:1170-1233, common random numbers); a walk of 50,000
- * walkers occupies a tenth of the GPU, and the k_io walks together fill it.
+ * walkers is 196 blocks, fewer than the GPU's 256 CUs, and the k_io walks
+ * together fill it.
  * Outputs hold the walks one after another: d_Y n_walks x n x n_grid x 3,
  * d_inside_trace n_walks x n x n_grid, d_escaped and d_err n_walks x n,
  * d_counters (or NULL) n_walks x MADI_N_COUNTERS, zeroed by the caller.
