@@ -495,8 +495,14 @@ def measure_production(args, log, sweep=False):
     args_cols = (cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
     modes = {"exact": base, "exact_cached": replace(base, classifier_mode="exact_cached"),
              "fp32": replace(base, precision="fp32")}
+    want = [m.strip() for m in args.prod_modes.split(",") if m.strip()]
+    coords = [c.strip() for c in args.prod_coords.split(",") if c.strip()]
+    if not want or any(m not in modes for m in want) or not coords or \
+            any(c not in ("low", "center", "high") for c in coords):
+        raise SystemExit(f"--prod-modes takes {sorted(modes)}, --prod-coords low, center, high")
+    modes = {m: modes[m] for m in modes if m in want}
     out = {}
-    for which in ("low", "center", "high"):
+    for which in coords:
         rho, V, vi = production_coordinate(which)
         t0 = time.perf_counter()
         ens = walker_gpu._build_ensembles_for_entry(rho, V, base, seed=args.prod_seed, verbose=False)[0]
@@ -526,24 +532,29 @@ def measure_production(args, log, sweep=False):
             entry["speedup_over_published_a100_cached"] = PUBLISHED_A100_S["exact_cached"][which] / dt
             row[mode] = entry
             log(f"production {which} {mode}: {dt:.3f} s ({steps / dt:.3e} walker-steps/s)")
-        a, b = res["exact"], res["exact_cached"]
-        row["exact_cached_bitwise_equal_to_exact"] = bool(
-            np.array_equal(a.cos_sum, b.cos_sum) and np.array_equal(a.sin_sum, b.sin_sum)
-            and np.array_equal(a.occupancy_counts, b.occupancy_counts))
+        a, b = res.get("exact"), res.get("exact_cached")
+        if a is not None and b is not None:
+            row["exact_cached_bitwise_equal_to_exact"] = bool(
+                np.array_equal(a.cos_sum, b.cos_sum) and np.array_equal(a.sin_sum, b.sin_sum)
+                and np.array_equal(a.occupancy_counts, b.occupancy_counts))
         n3 = 3.0 * base.n_walkers
-        d32 = (res["fp32"].cos_sum - a.cos_sum) / n3
-        row["fp32_vs_fp64_max_abs_dS"] = float(np.max(np.abs(d32)))
-        row["fp32_vs_fp64_rms_dS"] = float(np.sqrt(np.mean(d32 ** 2)))
-        # the Monte Carlo noise of S at this size: the same FP64 walk with another seed
-        r2 = walker_gpu._walk_and_reduce_one_ensemble(ens, pp, modes["exact_cached"], *args_cols,
-                                                      walker_gpu._walk_seed(args.prod_seed, 1))
-        dseed = (r2.cos_sum - b.cos_sum) / n3
-        row["fp64_seed_to_seed_max_abs_dS"] = float(np.max(np.abs(dseed)))
-        row["fp64_seed_to_seed_rms_dS"] = float(np.sqrt(np.mean(dseed ** 2)))
-        log(f"production {which}: |dS| fp32 vs fp64 max {row['fp32_vs_fp64_max_abs_dS']:.2e} rms "
-            f"{row['fp32_vs_fp64_rms_dS']:.2e}; fp64 seed to seed max {row['fp64_seed_to_seed_max_abs_dS']:.2e} rms "
-            f"{row['fp64_seed_to_seed_rms_dS']:.2e}")
-        if sweep:
+        ref64 = a if a is not None else b
+        if ref64 is not None and "fp32" in res:
+            d32 = (res["fp32"].cos_sum - ref64.cos_sum) / n3
+            row["fp32_vs_fp64_max_abs_dS"] = float(np.max(np.abs(d32)))
+            row["fp32_vs_fp64_rms_dS"] = float(np.sqrt(np.mean(d32 ** 2)))
+        if b is not None:
+            # the Monte Carlo noise of S at this size: the same FP64 walk with another seed
+            r2 = walker_gpu._walk_and_reduce_one_ensemble(ens, pp, modes["exact_cached"], *args_cols,
+                                                          walker_gpu._walk_seed(args.prod_seed, 1))
+            dseed = (r2.cos_sum - b.cos_sum) / n3
+            row["fp64_seed_to_seed_max_abs_dS"] = float(np.max(np.abs(dseed)))
+            row["fp64_seed_to_seed_rms_dS"] = float(np.sqrt(np.mean(dseed ** 2)))
+        if "fp32_vs_fp64_max_abs_dS" in row and "fp64_seed_to_seed_max_abs_dS" in row:
+            log(f"production {which}: |dS| fp32 vs fp64 max {row['fp32_vs_fp64_max_abs_dS']:.2e} rms "
+                f"{row['fp32_vs_fp64_rms_dS']:.2e}; fp64 seed to seed max {row['fp64_seed_to_seed_max_abs_dS']:.2e} "
+                f"rms {row['fp64_seed_to_seed_rms_dS']:.2e}")
+        if sweep and b is not None:
             # the library build's sweep: every production k_io of this ensemble, FP64 exact_cached
             kios = make_remediation_log_grid().kios
             pps = [walker_gpu._checked_pp(walker_gpu.kio_to_pp(float(k), ens.mean_AV, base)) for k in kios]
@@ -574,6 +585,9 @@ def run_production(args):
     log = lambda m: print(f"[bench] {m}", file=sys.stderr, flush=True)  # noqa: E731
     r = measure_production(args, log, sweep=True)
     c = r["coordinates"]
+    if "high" not in c or "exact_cached" not in c["high"]:
+        print(json.dumps({"metric": "production walks (partial run)", "coordinates": c}), flush=True)
+        return 0
     hi = c["high"]["exact_cached"]
     out = {
         "metric": "walk + reduction seconds per ensemble x k_io (the reference's production benchmark)",
@@ -764,6 +778,11 @@ def main(argv=None):
     ap.add_argument("--workload", choices=["c1", "c2", "c3", "c4", "map", "reduce", "library", "production"],
                     default="c1")
     ap.add_argument("--prod-seed", type=int, default=0, help="production: geometry and walk seed")
+    ap.add_argument("--prod-coords", default="low,center,high",
+                    help="production: comma list of the density coordinates to run (low, center, high)")
+    ap.add_argument("--prod-modes", default="exact,exact_cached,fp32",
+                    help="production: comma list of walks to time (exact, exact_cached, fp32); the FP32 |dS| "
+                         "needs exact and fp32, the seed-to-seed |dS| and the sweep exact_cached")
     ap.add_argument("--voxels", type=int, default=1_000_000, help="map: voxels matched per step")
     ap.add_argument("--library-entries", type=int, default=14_105, help="map: library entries")
     ap.add_argument("--entries", type=int, default=256)

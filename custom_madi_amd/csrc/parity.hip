@@ -133,10 +133,20 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                                                     int8_t* __restrict__ err_out, FusedOut fo,
                                                     unsigned long long* counters, MultiPP mp)
 {
-    // several walks that differ only in p_p (mp.n > 0, kY): walk blockIdx.y
-    // takes mp.pp[blockIdx.y] and its own slices of the outputs and counters
-    if (MODE == kY && mp.n > 0) {
-        const size_t kw = blockIdx.y, n = (size_t)p.n_walkers;
+    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // several walks that differ only in p_p (mp.n = K > 0, kY), walk-major:
+    // thread t runs walker t / K in walk t % K with that walk's p_p and its
+    // slices of the outputs and counters.  The walks share the walk seed, so
+    // the K lanes of one walker follow one path until an acceptance draw
+    // falls between their p_p thresholds, and stay near each other after:
+    // while they agree their geometry reads are one coalesced access, and
+    // after that they still share cache lines.
+    const bool multi = MODE == kY && mp.n > 0;
+    if (multi) {
+        const int64_t t = w;
+        w = t / mp.n;
+        const int kw = (int)(t - w * mp.n);
+        const size_t n = (size_t)p.n_walkers;
         p.pp = mp.pp[kw];
         Y += kw * n * (size_t)p.n_grid * 3;
         itr += kw * n * (size_t)p.n_grid;
@@ -149,7 +159,6 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
     __shared__ double lds_cols[kWaves * kColChunk * 2];
     const KdView v = kd_stage(g, sf, si, kLdsNodes);
 
-    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = w < p.n_walkers;
     const int G = p.n_grid;
     long long nodes = 0, crossings = 0, rejects = 0, classified = 0, cache_hits = 0, moves = 0, kd_calls = 0;
@@ -347,7 +356,18 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
         double* slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
         emit_columns<double>(*fo.plan, dM, live && !escaped, lds_cols, slot);
     }
-    if (counters) {
+    if (counters && multi) {
+        // the lanes of a wave belong to different walks: one add per lane
+        if (live) {
+            atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)nodes);
+            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)crossings);
+            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)rejects);
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)classified);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)cache_hits);
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)moves);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)kd_calls);
+        }
+    } else if (counters) {
         const double a = wave_sum_f64((double)nodes);
         const double b = wave_sum_f64((double)crossings);
         const double c = wave_sum_f64((double)rejects);
@@ -646,10 +666,15 @@ int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
                           int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters, hipStream_t s)
 {
     if (p.n_walkers == 0 || mp.n == 0) return MADI_OK;
-    const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
+    // one thread per (walker, walk), walk-major
+    const int64_t blocks = (p.n_walkers * mp.n + kBlock - 1) / kBlock;
+    if (blocks > 0x7fffffff) {
+        set_error("k_walk_f64 (several walks): %lld walkers x %d walks exceed one grid", (long long)p.n_walkers, mp.n);
+        return MADI_ERR_ARG;
+    }
     FusedOut fo{};
-    hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks, (unsigned)mp.n), dim3(kBlock), 0, s, g, p, jump_pow,
-                       nullptr, nullptr, nullptr, Y, itr, esc, err, fo, counters, mp);
+    hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, nullptr, nullptr,
+                       nullptr, Y, itr, esc, err, fo, counters, mp);
     return check_launch("k_walk_f64 (several walks)");
 }
 
