@@ -120,9 +120,149 @@ __global__ __launch_bounds__(kBlock) void k_rng_init(uint64_t seed, uint64_t fir
 
 // ---------------------------------------------------------------- walk
 
-// One walker of the reference step loop (madi/walker_gpu.py:531-656).
-// Mode kY writes Y / inside samples; mode kFused accumulates the plan's
-// moments instead and reduces phases at the end.
+// One walker of the reference step loop (madi/walker_gpu.py:531-656): its
+// position, its anchor (the cell of its position and that seed's image,
+// asx..; graph labels), the certified ball of the last graph label (every
+// point within crho of (ccx, ccy, ccz) has label (ccell, cin) and anchor
+// image (csx, ..)), the previous position less L/2 and Y, and its counters.
+struct WalkerF64 {
+    double px, py, pz, asx, asy, asz;
+    double ccx, ccy, ccz, crho, csx, csy, csz;
+    double xp, yp, zp, Yx, Yy, Yz;
+    int cell, in, ccell, cin, err, escaped;
+    long long nodes, crossings, rejects, classified, cache_hits, moves, kd_calls;
+};
+
+__device__ __forceinline__ void walker_clear(WalkerF64& s)
+{
+    s.px = s.py = s.pz = s.asx = s.asy = s.asz = 0.0;
+    s.ccx = s.ccy = s.ccz = s.csx = s.csy = s.csz = 0.0;
+    s.crho = -1.0;
+    s.xp = s.yp = s.zp = s.Yx = s.Yy = s.Yz = 0.0;
+    s.cell = s.in = s.ccell = s.cin = s.err = s.escaped = 0;
+    s.nodes = s.crossings = s.rejects = s.classified = s.cache_hits = s.moves = s.kd_calls = 0;
+}
+
+// The start at (px, py, pz): the exact classification, the anchor image,
+// and the trapezoid's first point.
+__device__ __forceinline__ void walker_start(const GeomDev& g, const KdView& v, bool graph, double half, WalkerF64& s)
+{
+    ++s.classified;
+    ++s.kd_calls;
+    if (!classify_point_f64(g, v, s.px, s.py, s.pz, s.cell, s.in, s.nodes)) {
+        s.err = 1;
+        s.escaped = 1;
+    } else if (graph) {
+        seed_image(g, s.cell, s.px, s.py, s.pz, s.asx, s.asy, s.asz);
+    }
+    s.xp = s.px - half;
+    s.yp = s.py - half;
+    s.zp = s.pz - half;
+}
+
+// One step with displacement (dx, dy, dz) and acceptance uniform u: the
+// proposal's label (the certified ball, the graph or the KD classifier),
+// the membrane count m and the acceptance test u < p_p^m, the move, and the
+// trapezoid of Y.
+__device__ __forceinline__ void walker_step(const GeomDev& g, const KdView& v, const WalkDev& p, bool graph,
+                                            double eps, double half, double dx, double dy, double dz, double u,
+                                            WalkerF64& s)
+{
+    double nx = s.px + dx, ny = s.py + dy, nz = s.pz + dz;
+    if (!g.periodic && (nx < 0.0 || nx >= g.L || ny < 0.0 || ny >= g.L || nz < 0.0 || nz >= g.L)) {
+        s.escaped = 1;
+        return;
+    }
+    int ncell = 0, nin = 0, ok = 1;
+    double nsx = s.asx, nsy = s.asy, nsz = s.asz;  // the proposal's anchor image
+    const double ex = nx - s.ccx, ey = ny - s.ccy, ez = nz - s.ccz;
+    if (graph && s.crho > 0.0 && ex * ex + ey * ey + ez * ez < s.crho * s.crho) {
+        ncell = s.ccell;
+        nin = s.cin;
+        nsx = s.csx;
+        nsy = s.csy;
+        nsz = s.csz;
+        ++s.cache_hits;
+    } else if (graph) {
+        ++s.classified;
+        int na = s.cell;  // from the walker's own cell
+        double m1 = 0.0;
+        if (graph_label_f64(g, na, nsx, nsy, nsz, nx, ny, nz, eps, nin, m1, s.moves)) {
+            ncell = na;
+            const double al = g.cell_alpha64[na];
+            s.crho = (nin ? m1 - al : fmin(m1, al - m1)) - 2.0 * eps;
+        } else {
+            ++s.kd_calls;
+            ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, s.nodes);
+            if (ok) seed_image(g, ncell, nx, ny, nz, nsx, nsy, nsz);
+            s.crho = -1.0;
+        }
+        s.ccx = nx;
+        s.ccy = ny;
+        s.ccz = nz;
+        s.ccell = ncell;
+        s.cin = nin;
+        s.csx = nsx;
+        s.csy = nsy;
+        s.csz = nsz;
+    } else {
+        ++s.classified;
+        ++s.kd_calls;
+        ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, s.nodes);
+    }
+    if (!ok) {
+        s.err = 1;
+        s.escaped = 1;
+        return;
+    }
+    int m = 0;
+    if (s.in != nin) m = 1;
+    else if (s.in && nin && s.cell != ncell) m = 2;
+    if (m > 0) ++s.crossings;
+    if (m > 0 && p.pp < 1.0) {
+        const double thr = m == 1 ? p.pp : p.pp * p.pp;
+        if (u >= thr) {
+            nx = s.px;
+            ny = s.py;
+            nz = s.pz;
+            ncell = s.cell;
+            nin = s.in;
+            nsx = s.asx;
+            nsy = s.asy;
+            nsz = s.asz;
+            ++s.rejects;
+        }
+    }
+    s.asx = nsx;
+    s.asy = nsy;
+    s.asz = nsz;
+    s.px = nx;
+    s.py = ny;
+    s.pz = nz;
+    s.cell = ncell;
+    s.in = nin;
+    const double xn = s.px - half, yn = s.py - half, zn = s.pz - half;
+    s.Yx += 0.5 * (s.xp + xn) * p.ts;
+    s.Yy += 0.5 * (s.yp + yn) * p.ts;
+    s.Yz += 0.5 * (s.zp + zn) * p.ts;
+    s.xp = xn;
+    s.yp = yn;
+    s.zp = zn;
+}
+
+// The walker's Y and inside flag at grid point j (mode kY).
+__device__ __forceinline__ void walker_sample(const WalkerF64& s, int64_t w, int G, int j, double* __restrict__ Y,
+                                              int8_t* __restrict__ itr)
+{
+    double* Yw = Y + ((size_t)w * G + j) * 3;
+    Yw[0] = s.Yx;
+    Yw[1] = s.Yy;
+    Yw[2] = s.Yz;
+    itr[(size_t)w * G + j] = (int8_t)s.in;
+}
+
+// One walker per thread.  Mode kY writes Y / inside samples; mode kFused
+// accumulates the plan's moments instead and reduces phases at the end.
 enum WalkMode { kY = 0, kFused = 1 };
 
 template <int MODE>
@@ -131,93 +271,52 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                                                     const double* __restrict__ gu, double* __restrict__ Y,
                                                     int8_t* __restrict__ itr, int8_t* __restrict__ esc_out,
                                                     int8_t* __restrict__ err_out, FusedOut fo,
-                                                    unsigned long long* counters, MultiPP mp)
+                                                    unsigned long long* counters)
 {
-    int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    // several walks that differ only in p_p (mp.n = K > 0, kY), walk-major:
-    // thread t runs walker t / K in walk t % K with that walk's p_p and its
-    // slices of the outputs and counters.  The walks share the walk seed, so
-    // the K lanes of one walker follow one path until an acceptance draw
-    // falls between their p_p thresholds, and stay near each other after:
-    // while they agree their geometry reads are one coalesced access, and
-    // after that they still share cache lines.
-    const bool multi = MODE == kY && mp.n > 0;
-    if (multi) {
-        const int64_t t = w;
-        w = t / mp.n;
-        const int kw = (int)(t - w * mp.n);
-        const size_t n = (size_t)p.n_walkers;
-        p.pp = mp.pp[kw];
-        Y += kw * n * (size_t)p.n_grid * 3;
-        itr += kw * n * (size_t)p.n_grid;
-        esc_out += kw * n;
-        err_out += kw * n;
-        if (counters) counters += kw * MADI_N_COUNTERS;
-    }
     __shared__ KdNodeF sf[kLdsNodes];
     __shared__ KdNodeI si[kLdsNodes];
     __shared__ double lds_cols[kWaves * kColChunk * 2];
     const KdView v = kd_stage(g, sf, si, kLdsNodes);
 
+    const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const bool live = w < p.n_walkers;
     const int G = p.n_grid;
-    long long nodes = 0, crossings = 0, rejects = 0, classified = 0, cache_hits = 0, moves = 0, kd_calls = 0;
-    // graph labels (classify_mode 5): the walker's anchor (the cell of its
-    // position and that seed's image, asx..), and the certified ball of the
-    // last graph label: every point within crho of (ccx, ccy, ccz) has label
-    // (ccell, cin) and anchor image (csx, ..)
     const bool graph = p.graph_labels != 0 && !g.free_water;
     const double eps = 1e-9 * fmax(1.0, g.L / 100.0);
-    double ccx = 0.0, ccy = 0.0, ccz = 0.0, crho = -1.0, csx = 0.0, csy = 0.0, csz = 0.0;
-    double asx = 0.0, asy = 0.0, asz = 0.0;
-    int ccell = 0, cin = 0;
+    const double half = 0.5 * g.L;
 
-    double px = 0.0, py = 0.0, pz = 0.0;
+    WalkerF64 s;
+    walker_clear(s);
     Rng r{0, 0};
     if (live) {
         if (p.use_golden) {
-            px = gi[3 * w];
-            py = gi[3 * w + 1];
-            pz = gi[3 * w + 2];
+            s.px = gi[3 * w];
+            s.py = gi[3 * w + 1];
+            s.pz = gi[3 * w + 2];
         } else {
             r = rng_for_walker(p.seed, p.first_walker + (uint64_t)w, jump_pow);
             const double span = g.source_hi - g.source_lo;
-            px = g.source_lo + uniform_f64(r) * span;
-            py = g.source_lo + uniform_f64(r) * span;
-            pz = g.source_lo + uniform_f64(r) * span;
+            s.px = g.source_lo + uniform_f64(r) * span;
+            s.py = g.source_lo + uniform_f64(r) * span;
+            s.pz = g.source_lo + uniform_f64(r) * span;
         }
+        walker_start(g, v, graph, half, s);
     }
-    int cell = 0, in = 0;
-    int err = 0, escaped = 0;
-    if (live) {
-        ++classified;
-        ++kd_calls;
-        if (!classify_point_f64(g, v, px, py, pz, cell, in, nodes)) {
-            err = 1;
-            escaped = 1;
-        } else if (graph) {
-            seed_image(g, cell, px, py, pz, asx, asy, asz);
-        }
-    }
-    const double half = 0.5 * g.L;
-    double xp = px - half, yp = py - half, zp = pz - half;
-    double Yx = 0.0, Yy = 0.0, Yz = 0.0;
     double dM[MADI_MAX_PAIRS][3];
 #pragma unroll
     for (int q = 0; q < MADI_MAX_PAIRS; ++q) dM[q][0] = dM[q][1] = dM[q][2] = 0.0;
     if (MODE == kY && live) {
         double* Yw = Y + (size_t)w * G * 3;
         Yw[0] = Yw[1] = Yw[2] = 0.0;
-        itr[(size_t)w * G] = (int8_t)(err ? 0 : in);
+        itr[(size_t)w * G] = (int8_t)(s.err ? 0 : s.in);
     }
     if (MODE == kFused) {
-        if (live && !err && fo.occupancy) { /* sample j = 0 counted below */ }
-        const long long c0 = wave_popc(live && !err && in);
+        const long long c0 = wave_popc(live && !s.err && s.in);
         if ((threadIdx.x & 63) == 0 && c0) atomicAdd((unsigned long long*)&fo.occupancy[0], (unsigned long long)c0);
     }
     int next_ev = 0;
     for (int32_t step = 0; step < p.n_steps; ++step) {
-        if (live && !escaped) {
+        if (live && !s.escaped) {
             double dx, dy, dz, u;
             if (p.use_golden) {
                 const size_t k = (size_t)step * p.n_walkers + w;
@@ -231,100 +330,15 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                 dz = normal_f64(r) * p.sigma;
                 u = uniform_f64(r);
             }
-            double nx = px + dx, ny = py + dy, nz = pz + dz;
-            if (!g.periodic && (nx < 0.0 || nx >= g.L || ny < 0.0 || ny >= g.L || nz < 0.0 || nz >= g.L)) {
-                escaped = 1;
-            } else {
-                int ncell = 0, nin = 0, ok = 1;
-                double nsx = asx, nsy = asy, nsz = asz;  // the proposal's anchor image
-                const double ex = nx - ccx, ey = ny - ccy, ez = nz - ccz;
-                if (graph && crho > 0.0 && ex * ex + ey * ey + ez * ez < crho * crho) {
-                    ncell = ccell;
-                    nin = cin;
-                    nsx = csx;
-                    nsy = csy;
-                    nsz = csz;
-                    ++cache_hits;
-                } else if (graph) {
-                    ++classified;
-                    int na = cell;  // from the walker's own cell
-                    double m1 = 0.0;
-                    if (graph_label_f64(g, na, nsx, nsy, nsz, nx, ny, nz, eps, nin, m1, moves)) {
-                        ncell = na;
-                        const double al = g.cell_alpha64[na];
-                        crho = (nin ? m1 - al : fmin(m1, al - m1)) - 2.0 * eps;
-                    } else {
-                        ++kd_calls;
-                        ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, nodes);
-                        if (ok) seed_image(g, ncell, nx, ny, nz, nsx, nsy, nsz);
-                        crho = -1.0;
-                    }
-                    ccx = nx;
-                    ccy = ny;
-                    ccz = nz;
-                    ccell = ncell;
-                    cin = nin;
-                    csx = nsx;
-                    csy = nsy;
-                    csz = nsz;
-                } else {
-                    ++classified;
-                    ++kd_calls;
-                    ok = classify_point_f64(g, v, nx, ny, nz, ncell, nin, nodes);
-                }
-                if (!ok) {
-                    err = 1;
-                    escaped = 1;
-                } else {
-                    int m = 0;
-                    if (in != nin) m = 1;
-                    else if (in && nin && cell != ncell) m = 2;
-                    if (m > 0) ++crossings;
-                    if (m > 0 && p.pp < 1.0) {
-                        const double thr = m == 1 ? p.pp : p.pp * p.pp;
-                        if (u >= thr) {
-                            nx = px;
-                            ny = py;
-                            nz = pz;
-                            ncell = cell;
-                            nin = in;
-                            nsx = asx;
-                            nsy = asy;
-                            nsz = asz;
-                            ++rejects;
-                        }
-                    }
-                    asx = nsx;
-                    asy = nsy;
-                    asz = nsz;
-                    px = nx;
-                    py = ny;
-                    pz = nz;
-                    cell = ncell;
-                    in = nin;
-                    const double xn = px - half, yn = py - half, zn = pz - half;
-                    Yx += 0.5 * (xp + xn) * p.ts;
-                    Yy += 0.5 * (yp + yn) * p.ts;
-                    Yz += 0.5 * (zp + zn) * p.ts;
-                    xp = xn;
-                    yp = yn;
-                    zp = zn;
-                }
-            }
+            walker_step(g, v, p, graph, eps, half, dx, dy, dz, u, s);
         }
         const int32_t s1 = step + 1;
         if (s1 % p.steps_per_h == 0) {
             const int j = s1 / p.steps_per_h;
             if (MODE == kY) {
-                if (live) {
-                    double* Yw = Y + ((size_t)w * G + j) * 3;
-                    Yw[0] = Yx;
-                    Yw[1] = Yy;
-                    Yw[2] = Yz;
-                    itr[(size_t)w * G + j] = (int8_t)in;
-                }
+                if (live) walker_sample(s, w, G, j, Y, itr);
             } else {
-                const long long c = wave_popc(live && !err && in);
+                const long long c = wave_popc(live && !s.err && s.in);
                 if ((threadIdx.x & 63) == 0 && c) atomicAdd((unsigned long long*)&fo.occupancy[j], (unsigned long long)c);
                 while (next_ev < fo.ev.n && fo.ev.j[next_ev] == j) {
                     const int q = fo.ev.pair[next_ev];
@@ -332,9 +346,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
 #pragma unroll
                     for (int t = 0; t < MADI_MAX_PAIRS; ++t)
                         if (t == q) {
-                            dM[t][0] += sg * Yx;
-                            dM[t][1] += sg * Yy;
-                            dM[t][2] += sg * Yz;
+                            dM[t][0] += sg * s.Yx;
+                            dM[t][1] += sg * s.Yy;
+                            dM[t][2] += sg * s.Yz;
                         }
                     ++next_ev;
                 }
@@ -343,38 +357,27 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
     }
     if (MODE == kY) {
         if (live) {
-            esc_out[w] = (int8_t)escaped;
-            err_out[w] = (int8_t)err;
+            esc_out[w] = (int8_t)s.escaped;
+            err_out[w] = (int8_t)s.err;
         }
     } else {
-        const long long ne = wave_popc(live && escaped);
-        const long long nr = wave_popc(live && err);
+        const long long ne = wave_popc(live && s.escaped);
+        const long long nr = wave_popc(live && s.err);
         if ((threadIdx.x & 63) == 0) {
             if (ne) atomicAdd((unsigned long long*)&fo.flags[MADI_FLAG_ESCAPED], (unsigned long long)ne);
             if (nr) atomicAdd((unsigned long long*)&fo.flags[MADI_FLAG_CLASSIFIER_ERR], (unsigned long long)nr);
         }
         double* slot = fo.partial + (size_t)(fo.tile_base + blockIdx.x) * fo.plan->n_cols * 2;
-        emit_columns<double>(*fo.plan, dM, live && !escaped, lds_cols, slot);
+        emit_columns<double>(*fo.plan, dM, live && !s.escaped, lds_cols, slot);
     }
-    if (counters && multi) {
-        // the lanes of a wave belong to different walks: one add per lane
-        if (live) {
-            atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)nodes);
-            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)crossings);
-            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)rejects);
-            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)classified);
-            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)cache_hits);
-            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)moves);
-            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)kd_calls);
-        }
-    } else if (counters) {
-        const double a = wave_sum_f64((double)nodes);
-        const double b = wave_sum_f64((double)crossings);
-        const double c = wave_sum_f64((double)rejects);
-        const double d = wave_sum_f64((double)classified);
-        const double e = wave_sum_f64((double)cache_hits);
-        const double f = wave_sum_f64((double)moves);
-        const double h = wave_sum_f64((double)kd_calls);
+    if (counters) {
+        const double a = wave_sum_f64((double)s.nodes);
+        const double b = wave_sum_f64((double)s.crossings);
+        const double c = wave_sum_f64((double)s.rejects);
+        const double d = wave_sum_f64((double)s.classified);
+        const double e = wave_sum_f64((double)s.cache_hits);
+        const double f = wave_sum_f64((double)s.moves);
+        const double h = wave_sum_f64((double)s.kd_calls);
         if ((threadIdx.x & 63) == 0) {
             atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)a);
             atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)b);
@@ -388,6 +391,116 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
             atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)e);
             atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)f);
             atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)h);
+        }
+    }
+}
+
+// The k_io sweep (madi_walk_f64_multi): K = mp.n walks of one geometry that
+// differ only in p_p, walk-major.  A block holds M = kBlock / K walkers as K
+// consecutive threads each (thread m K + k runs walker m in walk k; the
+// threads past M K idle).  The walks share the walk seed, so the K threads
+// of a walker make the same draws at every step.  In each batch of K steps
+// every thread advances its copy of the walker's stream and keeps the draws
+// of step base + k, turns them into that step's displacement and acceptance
+// uniform (the FP64 logarithm, square root and cosine: once per step and
+// walker instead of once per walk) in the step table in LDS, and after a
+// barrier the walker's threads take each step's values from the table.  A
+// value is computed by the same code as in k_walk_f64 on the same draws, so
+// every walk's results are bitwise its k_walk_f64 walk's.  The threads of a
+// walker also follow one path until an acceptance draw falls between their
+// p_p thresholds, and stay near each other after: their geometry reads are
+// one coalesced access, or share cache lines.
+constexpr int kSweepLdsNodes = 255;  // top 8 KD levels: the step table shares the LDS
+
+__global__ __launch_bounds__(kBlock) void k_walk_f64_sweep(GeomDev g, WalkDev p, const Gf2Col* __restrict__ jump_pow,
+                                                          MultiPP mp, double* __restrict__ Y,
+                                                          int8_t* __restrict__ itr, int8_t* __restrict__ esc_out,
+                                                          int8_t* __restrict__ err_out, unsigned long long* counters)
+{
+    __shared__ KdNodeF sf[kSweepLdsNodes];
+    __shared__ KdNodeI si[kSweepLdsNodes];
+    __shared__ double4 table[kBlock];  // walker m's step base + t at m K + t: (dx, dy, dz, u)
+    const KdView v = kd_stage(g, sf, si, kSweepLdsNodes);
+
+    const int K = mp.n;
+    const int M = kBlock / K;
+    const int m = (int)threadIdx.x / K;
+    const int k = (int)threadIdx.x - m * K;
+    const int64_t w = (int64_t)blockIdx.x * M + m;
+    const bool slot = m < M;  // a (walker, walk) thread; its walker may lie past the last
+    const bool live = slot && w < p.n_walkers;
+    const int G = p.n_grid;
+    if (slot) {
+        const size_t n = (size_t)p.n_walkers;
+        p.pp = mp.pp[k];
+        Y += k * n * (size_t)G * 3;
+        itr += k * n * (size_t)G;
+        esc_out += k * n;
+        err_out += k * n;
+        if (counters) counters += k * MADI_N_COUNTERS;
+    }
+    const bool graph = p.graph_labels != 0 && !g.free_water;
+    const double eps = 1e-9 * fmax(1.0, g.L / 100.0);
+    const double half = 0.5 * g.L;
+
+    WalkerF64 s;
+    walker_clear(s);
+    Rng r{0, 0};
+    if (live) {
+        r = rng_for_walker(p.seed, p.first_walker + (uint64_t)w, jump_pow);
+        const double span = g.source_hi - g.source_lo;
+        s.px = g.source_lo + uniform_f64(r) * span;
+        s.py = g.source_lo + uniform_f64(r) * span;
+        s.pz = g.source_lo + uniform_f64(r) * span;
+        walker_start(g, v, graph, half, s);
+        double* Yw = Y + (size_t)w * G * 3;
+        Yw[0] = Yw[1] = Yw[2] = 0.0;
+        itr[(size_t)w * G] = (int8_t)(s.err ? 0 : s.in);
+    }
+    for (int32_t base = 0; base < p.n_steps; base += K) {
+        const int B = p.n_steps - base < K ? p.n_steps - base : K;
+        // the draws of step base + k (7 per step: three normals from two
+        // each, then the acceptance uniform)
+        uint64_t d0 = 0, d1 = 0, d2 = 0, d3 = 0, d4 = 0, d5 = 0, d6 = 0;
+        for (int t = 0; t < B; ++t) {
+            const uint64_t a0 = rng_next(r), a1 = rng_next(r), a2 = rng_next(r), a3 = rng_next(r);
+            const uint64_t a4 = rng_next(r), a5 = rng_next(r), a6 = rng_next(r);
+            if (t == k) {
+                d0 = a0;
+                d1 = a1;
+                d2 = a2;
+                d3 = a3;
+                d4 = a4;
+                d5 = a5;
+                d6 = a6;
+            }
+        }
+        __syncthreads();  // every read of the previous batch's table is done
+        if (live && k < B)
+            table[threadIdx.x] = make_double4(normal_f64_of(d0, d1) * p.sigma, normal_f64_of(d2, d3) * p.sigma,
+                                              normal_f64_of(d4, d5) * p.sigma, u53_to_f64(d6));
+        __syncthreads();
+        for (int t = 0; t < B; ++t) {
+            if (live && !s.escaped) {
+                const double4 d = table[m * K + t];
+                walker_step(g, v, p, graph, eps, half, d.x, d.y, d.z, d.w, s);
+            }
+            const int32_t s1 = base + t + 1;
+            if (s1 % p.steps_per_h == 0 && live) walker_sample(s, w, G, s1 / p.steps_per_h, Y, itr);
+        }
+    }
+    if (live) {
+        esc_out[w] = (int8_t)s.escaped;
+        err_out[w] = (int8_t)s.err;
+        if (counters) {
+            // the threads of a wave belong to different walks: one add each
+            atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)s.nodes);
+            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)s.crossings);
+            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)s.rejects);
+            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)s.classified);
+            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)s.cache_hits);
+            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)s.moves);
+            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)s.kd_calls);
         }
     }
 }
@@ -656,9 +769,8 @@ int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, 
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
     FusedOut fo{};
-    MultiPP mp{};
     hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, gi, gd, gu, Y,
-                       itr, esc, err, fo, counters, mp);
+                       itr, esc, err, fo, counters);
     return check_launch("k_walk_f64");
 }
 
@@ -666,16 +778,19 @@ int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
                           int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters, hipStream_t s)
 {
     if (p.n_walkers == 0 || mp.n == 0) return MADI_OK;
-    // one thread per (walker, walk), walk-major
-    const int64_t blocks = (p.n_walkers * mp.n + kBlock - 1) / kBlock;
-    if (blocks > 0x7fffffff) {
-        set_error("k_walk_f64 (several walks): %lld walkers x %d walks exceed one grid", (long long)p.n_walkers, mp.n);
+    if (mp.n < 1 || mp.n > MADI_MAX_MULTI_WALKS) {
+        set_error("k_walk_f64_sweep: 1 to %d walks per launch, got %d", MADI_MAX_MULTI_WALKS, mp.n);
         return MADI_ERR_ARG;
     }
-    FusedOut fo{};
-    hipLaunchKernelGGL(k_walk_f64<kY>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, nullptr, nullptr,
-                       nullptr, Y, itr, esc, err, fo, counters, mp);
-    return check_launch("k_walk_f64 (several walks)");
+    const int64_t per_block = kBlock / mp.n;  // walkers per block
+    const int64_t blocks = (p.n_walkers + per_block - 1) / per_block;
+    if (blocks > 0x7fffffff) {
+        set_error("k_walk_f64_sweep: %lld walkers exceed one grid", (long long)p.n_walkers);
+        return MADI_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_walk_f64_sweep, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, mp, Y, itr, esc,
+                       err, counters);
+    return check_launch("k_walk_f64_sweep");
 }
 
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
@@ -683,9 +798,8 @@ int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
 {
     if (p.n_walkers == 0) return MADI_OK;
     const int64_t blocks = (p.n_walkers + kBlock - 1) / kBlock;
-    MultiPP mp{};
     hipLaunchKernelGGL(k_walk_f64<kFused>, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, nullptr,
-                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, fo, counters, mp);
+                       nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, fo, counters);
     return check_launch("k_walk_f64_fused");
 }
 

@@ -54,13 +54,21 @@ __device__ __forceinline__ double uniform_f64(Rng& r) { return u53_to_f64(rng_ne
 
 // numba normal_float64 (numba-src:199-221): two float32 uniforms, Box-Muller
 // first output, FP64 arithmetic; the float32 logarithm is taken as the
-// correctly rounded value float(log(double(u1))).
-__device__ __forceinline__ double normal_f64(Rng& r)
+// correctly rounded value float(log(double(u1))).  normal_f64_of takes the
+// two draws, normal_f64 makes them.
+__device__ __forceinline__ double normal_f64_of(uint64_t a1, uint64_t a2)
 {
-    const float u1 = u53_to_f32(rng_next(r));
-    const float u2 = u53_to_f32(rng_next(r));
+    const float u1 = u53_to_f32(a1);
+    const float u2 = u53_to_f32(a2);
     const float lg = (float)log((double)u1);
     return sqrt(-2.0 * (double)lg) * cos(6.283185307179586 * (double)u2);
+}
+
+__device__ __forceinline__ double normal_f64(Rng& r)
+{
+    const uint64_t a1 = rng_next(r);
+    const uint64_t a2 = rng_next(r);
+    return normal_f64_of(a1, a2);
 }
 
 }  // namespace madi
