@@ -70,22 +70,35 @@ __device__ __forceinline__ void seed_image(const GeomDev& g, int c, double qx, d
     }
 }
 
+template <int GRP = 8>
 __device__ __forceinline__ int graph_label_f64(const GeomDev& g, int& a, double& sx, double& sy, double& sz, double qx,
                                                double qy, double qz, double eps, int& inside, double& m1,
                                                long long& moves)
 {
     for (int it = 0; it <= kMaxGraphIters; ++it) {
-        if ((unsigned)a >= (unsigned)g.n_cells || g.cell_flags[a]) return 0;
+        if ((unsigned)a >= (unsigned)g.n_cells) return 0;
+        // the cell's flag, list and annulus in one round trip
+        const int8_t flag = g.cell_flags[a];
         const int lo = g.nb_off[a], hi = g.nb_off[a + 1];
+        const double al = g.cell_alpha64[a];
+        if (flag) return 0;
         const double rx = qx - sx, ry = qy - sy, rz = qz - sz;
         double mm = __builtin_inf();
         int km = -1;
-        for (int k = lo; k < hi; ++k) {
-            const double4 u = g.nb_u64[k];
-            const double m = u.w - (rx * u.x + ry * u.y + rz * u.z);
-            if (m < mm) {
-                mm = m;
-                km = k;
+        // edges in groups of GRP, a group's records fetched before any is
+        // used (one round trip per group; the tail re-reads edge lo), then
+        // compared in list order: the first smallest margin, as one by one
+        for (int k0 = lo; k0 < hi; k0 += GRP) {
+            double4 u[GRP];
+#pragma unroll
+            for (int j = 0; j < GRP; ++j) u[j] = g.nb_u64[k0 + j < hi ? k0 + j : lo];
+#pragma unroll
+            for (int j = 0; j < GRP; ++j) {
+                const double m = u[j].w - (rx * u[j].x + ry * u[j].y + rz * u[j].z);
+                if (k0 + j < hi && m < mm) {
+                    mm = m;
+                    km = k0 + j;
+                }
             }
         }
         if (km >= 0 && mm < -eps) {
@@ -97,7 +110,6 @@ __device__ __forceinline__ int graph_label_f64(const GeomDev& g, int& a, double&
             ++moves;
             continue;
         }
-        const double al = g.cell_alpha64[a];
         if (!(mm > eps) || fabs(mm - al) <= eps) return 0;
         inside = mm > al;
         m1 = mm;
