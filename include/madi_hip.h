@@ -275,7 +275,8 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
  * sweep with one walk seed (run_simulation_multi_kio_reduced,
  * madi/walker_gpu.py:1170-1233, common random numbers); a walk of 50,000
  * walkers is 196 blocks, fewer than the GPU's 256 CUs, and the k_io walks
- * together fill it.
+ * together fill it.  The walks of one walker run in adjacent lanes and share
+ * each step's draws and their FP64 arithmetic (the same values as alone).
  * Outputs hold the walks one after another: d_Y n_walks x n x n_grid x 3,
  * d_inside_trace n_walks x n x n_grid, d_escaped and d_err n_walks x n,
  * d_counters (or NULL) n_walks x MADI_N_COUNTERS, zeroed by the caller.
