@@ -25,16 +25,19 @@ resident on the device.  The same line carries:
 * ``production``: the reference's own published benchmark (walk plus
   31,125-column reduction of one Eq. S8 ensemble, 50,000 walkers x 128,000
   steps, k_io 20 /s, at the production grid's low / centre / high rho)
-  through the drop-in API in three ways -- the FP64 exact walk, the FP64
-  exact_cached walk (bitwise the exact one) and the FP32 walk -- each beside
-  the published A100 time.
+  through the drop-in API: the FP64 exact_cached walk (graph labels) and the
+  FP32 walk at each coordinate, the FP64 exact walk at low rho (bitwise the
+  exact_cached one), the FP64 walk's seed-to-seed |dS| beside FP32's, and
+  the library's 51-k_io sweep of the ensemble in one launch, each beside the
+  published A100 time.
 
 Other workloads (--workload): c2, c3 on their own, c4 (a seeded sample of the
 full-range sweep), map (MAP matching of 1e6 voxels against a 14,105-entry
 library, FP64), reduce (the column reduce alone, with its CPU baseline),
 library (the drop-in library build at the default configuration, group by
 group against the batched scheduler), production (the production object
-alone).  --gpus N > 1 exits: this benchmark uses one GPU.
+alone, with the exact walk at all three coordinates; --prod-coords and
+--prod-modes narrow it).  --gpus N > 1 exits: this benchmark uses one GPU.
 """
 
 from __future__ import annotations
@@ -472,7 +475,7 @@ def production_coordinate(which: str):
     return rho, V, vi
 
 
-def measure_production(args, log, sweep=False):
+def measure_production(args, log, sweep=False, exact_coords=("low", "center", "high")):
     """The reference's own production benchmark, through the drop-in API:
     _walk_and_reduce_one_ensemble of one Eq. S8 ensemble (50,000 walkers x
     128,000 steps of 1 us, D0 3.0) at k_io 20 /s, reduced onto the 31,125
@@ -483,7 +486,9 @@ def measure_production(args, log, sweep=False):
     production walk; each against the published A100 times.  The FP32 |dS|
     is set beside the FP64 walk's own seed-to-seed |dS|.  With sweep, also
     the library build's 51-k_io sweep of the ensemble in FP64 exact_cached
-    (its walks run together, madi_walk_f64_multi)."""
+    (its walks run together, madi_walk_f64_multi).  The exact walk runs at
+    exact_coords only; FP32 is compared with it there, else with
+    exact_cached (bitwise the same walk)."""
     from dataclasses import replace
     from custom_madi_amd import _hip, signal, walker_gpu
     from custom_madi_amd.config import SimConfig
@@ -515,6 +520,8 @@ def measure_production(args, log, sweep=False):
         row = {"rho_per_uL": rho, "V_pL": V, "target_vi": vi, "n_seeds_pop": n_pop, "geometry_s": geom_s, "pp": pp}
         res = {}
         for mode, cfg in modes.items():
+            if mode == "exact" and which not in exact_coords:
+                continue
             # first launch of each kernel outside the timed region
             walker_gpu._walk_and_reduce_one_ensemble(ens, pp, replace(cfg, n_walkers=64), *args_cols, seed)
             torch.cuda.synchronize()
@@ -846,7 +853,9 @@ def main(argv=None):
         out["universal_reduce"] = measure_reduce(args, log, cpu_baseline=False)
     if not args.no_production:
         log("production: the reference's walk-plus-reduction benchmark at its three densities")
-        out["production"] = measure_production(args, log)
+        # the exact walk (reference semantics, ~20 s a coordinate) at low rho only;
+        # --workload production runs it at all three
+        out["production"] = measure_production(args, log, sweep=True, exact_coords=("low",))
     if not args.no_cpu_baseline:
         procs, vis = max(1, int(args.workers)), visible_cpus()
         single, multi = cpu_baseline_reference(ens, cfg, pp, pool, procs, args.cpu_seconds, log)
