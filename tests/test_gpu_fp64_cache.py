@@ -165,3 +165,38 @@ def test_fp64_kio_sweep_in_one_launch_equals_walks_one_by_one(mode, monkeypatch)
         assert a.classifier_cache_stats == b.classifier_cache_stats
         assert a.pp_values == b.pp_values and a.analytic_kio_eq5_values == b.analytic_kio_eq5_values
     say(f"[fp64 sweep] {mode}: {len(pps)} walks equal one by one and together")
+
+
+@pytest.mark.parametrize("case", ["escapes K3", "ragged K7", "max K64"])
+def test_walk_y_multi_equals_walk_y_per_walk(case):
+    """madi_walk_f64_multi against madi_walk_f64 walk by walk, bit for bit:
+    Y, inside traces, escape and failure flags and counters.  A
+    non-periodic box whose walkers escape (a walk's escaped threads still
+    make the walker's shared draws for its other walks), walker counts that
+    leave the last block part-filled, step counts that are not a multiple of
+    the walk count, and the largest walk count."""
+    mode = {"escapes K3": engine.MADI_CLASSIFY_FP64_CACHED, "ragged K7": 0,
+            "max K64": engine.MADI_CLASSIFY_FP64_CACHED}[case]
+    if case == "escapes K3":
+        ens, n, pps, cfg = _geom.cellular(L=40.0, seed=5), 1000, [0.0, 0.05, 1.0], \
+            _Cfg(1500, 50, 31, float(np.sqrt(2 * 2.0 * 0.02)), 0.02)
+    elif case == "ragged K7":
+        ens, n, pps, cfg = _geom.periodic(n_cells=64, seed=1), 333, [0.0, 0.01, 0.02, 0.1, 0.3, 0.7, 1.0], \
+            _Cfg(703, 50, 15, float(np.sqrt(2 * 2.0 * 0.05)), 0.05)
+    else:
+        ens, n, cfg = _geom.periodic(n_cells=64, seed=2), 130, _Cfg(400, 40, 11, float(np.sqrt(2 * 2.0 * 0.05)), 0.05)
+        pps = list(np.linspace(0.0, 1.0, 64))
+    dg = DeviceGeometry(geometry_arrays(ens))
+    params = engine.walk_params(n, cfg, pps[0], 19, classify_mode=mode)
+    many = engine.walk_y_multi(dg, params, pps)
+    escaped_any = False
+    for pp, b in zip(pps, many):
+        a = engine.walk_y(dg, engine.walk_params(n, cfg, pp, 19, classify_mode=mode))
+        assert np.array_equal(a.Y.cpu().numpy(), b.Y.cpu().numpy())
+        assert np.array_equal(a.inside_trace, b.inside_trace)
+        assert np.array_equal(a.escaped, b.escaped) and np.array_equal(a.err, b.err)
+        assert a.counters == b.counters
+        escaped_any |= bool(a.escaped.any())
+    if case == "escapes K3":
+        assert escaped_any  # the box is small enough that walkers leave it
+    say(f"[fp64 multi] {case}: {len(pps)} walks of {n} walkers equal one by one")
