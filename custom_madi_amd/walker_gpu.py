@@ -25,6 +25,7 @@ from __future__ import annotations
 
 import math
 import sys
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field, replace
 from typing import Optional, Sequence, Tuple
 
@@ -32,7 +33,10 @@ import numpy as np
 
 from . import _hip, engine
 from .config import PRODUCTION_AXIS_WALKS_PER_ENTRY, SimConfig
-from .ensemble import Ensemble, create_dummy_ensemble, create_ensemble, create_periodic_ensemble
+from .ensemble import (
+    Ensemble, build_ensemble_host, create_dummy_ensemble, create_ensemble, create_periodic_ensemble, finish_ensemble,
+    geometry_arrays,
+)
 
 PAPER_WALKS_PER_ENTRY = PRODUCTION_AXIS_WALKS_PER_ENTRY
 
@@ -476,6 +480,34 @@ def _build_ensemble_for_entry(rho: float, V: float, cfg: SimConfig, seed: int, i
     return create_ensemble(rho, V, cfg, seed=_ensemble_seed(seed, index), verbose=verbose)
 
 
+def _ensembles_in_turn(rho: float, V: float, cfg: SimConfig, seed: int, verbose: bool):
+    """The entry's ensembles one at a time, in order (the reference builds
+    them all before the first walk, madi/walker_gpu.py:1136-1142, 1193).
+    The CPU part of ensemble i + 1 (population, KD tree, Voronoi graph) is
+    built on a worker thread while ensemble i walks, and an ensemble's
+    device geometry is freed once the caller drops it.  Each ensemble
+    depends only on its own seed, so the ensembles, and every result, are
+    those of building them all first."""
+    n = int(cfg.n_ensembles)
+    if n <= 1 or rho <= 0.0 or V <= 0.0 or cfg.periodic:
+        for i in range(n):
+            yield _build_ensemble_for_entry(rho, V, cfg, seed, i, verbose)
+        return
+
+    def host(i):
+        draft = build_ensemble_host(rho, V, cfg, seed=_ensemble_seed(seed, i))
+        return draft, geometry_arrays(draft.ens)
+
+    with ThreadPoolExecutor(max_workers=1) as pool:
+        pending = pool.submit(host, 0)
+        for i in range(n):
+            draft, arrays = pending.result()
+            if i + 1 < n:
+                pending = pool.submit(host, i + 1)
+            # the v_i measurement runs on the GPU, here in the calling thread
+            yield finish_ensemble(draft, cfg, verbose=verbose, arrays=arrays)
+
+
 def run_simulation_reduced(rho: float, V: float, kio: float, j_delta, j_Delta, j_sum, phase_coef,
                            cfg: Optional[SimConfig] = None, seed: int = 0, verbose: bool = True, *,
                            narrow_snapshot_steps=None, narrow_snapshot_index=None,
@@ -514,9 +546,9 @@ def run_simulation_multi_kio_reduced(rho: float, V: float, kios: Sequence[float]
         sys.stderr.write(
             f"    WARNING: only {total_walks:,} axis-walks per entry; production reference is "
             f"~{PAPER_WALKS_PER_ENTRY:,}.\n")
-    ensembles = _build_ensembles_for_entry(rho, V, cfg, seed, verbose)
     outputs = {k: [] for k in kios}
-    for e, ens in enumerate(ensembles):
+    n_ens = int(cfg.n_ensembles)
+    for e, ens in enumerate(_ensembles_in_turn(rho, V, cfg, seed, verbose)):
         ws = _walk_seed(seed, e)
         pps = [0.0 if (ens.is_free_water or kio == 0.0) else _checked_pp(kio_to_pp(kio, ens.mean_AV, cfg))
                for kio in kios]
@@ -524,7 +556,7 @@ def run_simulation_multi_kio_reduced(rho: float, V: float, kios: Sequence[float]
                                                                        phase_coef, ws)):
             outputs[kio].append(res)
             if verbose:
-                print(f"    ensemble {e + 1}/{len(ensembles)}, k_io={kio:g}: p_p={pp:.7g}, "
+                print(f"    ensemble {e + 1}/{n_ens}, k_io={kio:g}: p_p={pp:.7g}, "
                       f"Eq.5={res.analytic_kio_eq5:.5g} s^-1")
     return {k: _merge_results(v) for k, v in outputs.items()}
 

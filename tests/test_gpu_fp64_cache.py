@@ -17,6 +17,7 @@ through run_walk_Y(classifier="exact_cached").
 
 from __future__ import annotations
 
+import json
 import sys
 
 import numpy as np
@@ -200,3 +201,33 @@ def test_walk_y_multi_equals_walk_y_per_walk(case):
     if case == "escapes K3":
         assert escaped_any  # the box is small enough that walkers leave it
     say(f"[fp64 multi] {case}: {len(pps)} walks of {n} walkers equal one by one")
+
+
+def test_multi_kio_ensembles_in_turn_equal_ensembles_built_first():
+    """run_simulation_multi_kio_reduced builds an entry's Eq. S8 ensembles one
+    at a time (the next one's CPU geometry on a worker thread while the
+    current one walks); its results equal those of building every ensemble
+    first, as the reference does, bit for bit."""
+    from custom_madi_amd import signal
+    # a short walk's Eq. S8 box is small: its realised v_i is within 0.03
+    cfg = SimConfig(D0=2.0, ts=0.02, T_max_ms=12.0, n_walkers=1024, n_ensembles=3, small_deltas=[1.0, 2.0],
+                    big_deltas=[4.0, 6.0], b_values=[0.0, 1000.0, 2500.0], classifier_mode="exact_cached",
+                    geometry_validation_points=20_000, geometry_vi_tolerance=0.03)
+    cols = signal.build_columns(cfg)
+    args = (cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    rho, V, kios, seed = 1.0e5, 6.3, [0.0, 5.0, 20.0, 60.0], 7
+    got = walker_gpu.run_simulation_multi_kio_reduced(rho, V, kios, *args, cfg=cfg, seed=seed, verbose=False)
+    want = {k: [] for k in kios}
+    for e, ens in enumerate(walker_gpu._build_ensembles_for_entry(rho, V, cfg, seed, False)):
+        pps = [0.0 if k == 0.0 else walker_gpu._checked_pp(walker_gpu.kio_to_pp(k, ens.mean_AV, cfg)) for k in kios]
+        for k, r in zip(kios, walker_gpu._walk_and_reduce_kio_sweep(ens, pps, cfg, *args,
+                                                                     walker_gpu._walk_seed(seed, e))):
+            want[k].append(r)
+    for k in kios:
+        w = walker_gpu._merge_results(want[k])
+        g = got[k]
+        assert np.array_equal(g.cos_sum, w.cos_sum) and np.array_equal(g.sin_sum, w.sin_sum)
+        assert np.array_equal(g.occupancy_counts, w.occupancy_counts) and g.n_walkers == w.n_walkers == 3 * 1024
+        assert g.pp_values == w.pp_values
+        assert json.dumps(g.geometry_stats, sort_keys=True) == json.dumps(w.geometry_stats, sort_keys=True)
+    say(f"[fp64 entry] {len(kios)} k_io x 3 ensembles built in turn equal those built first")
