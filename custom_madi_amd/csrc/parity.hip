@@ -70,7 +70,7 @@ __device__ __forceinline__ void seed_image(const GeomDev& g, int c, double qx, d
     }
 }
 
-template <int GRP = 8>
+template <int GRP>
 __device__ __forceinline__ int graph_label_f64(const GeomDev& g, int& a, double& sx, double& sy, double& sz, double qx,
                                                double qy, double qz, double eps, int& inside, double& m1,
                                                long long& moves)
@@ -118,6 +118,12 @@ __device__ __forceinline__ int graph_label_f64(const GeomDev& g, int& a, double&
     return 0;
 }
 
+// Edges per memory round trip of a graph scan: the one-walk kernel's waves
+// run about one per SIMD and gain from the deeper fetch; the k_io sweep fills
+// the GPU, where four edges keep it at three waves per SIMD instead of two.
+constexpr int kGraphGroupWalk = 8;
+constexpr int kGraphGroupSweep = 4;
+
 // ----------------------------------------------------------------- RNG
 
 __global__ __launch_bounds__(kBlock) void k_rng_init(uint64_t seed, uint64_t first, int64_t n,
@@ -136,11 +142,11 @@ __global__ __launch_bounds__(kBlock) void k_rng_init(uint64_t seed, uint64_t fir
 // position, its anchor (the cell of its position and that seed's image,
 // asx..; graph labels), the certified ball of the last graph label (every
 // point within crho of (ccx, ccy, ccz) has label (ccell, cin) and anchor
-// image (csx, ..)), the previous position less L/2 and Y, and its counters.
+// image (csx, ..)), Y, and its counters.
 struct WalkerF64 {
     double px, py, pz, asx, asy, asz;
     double ccx, ccy, ccz, crho, csx, csy, csz;
-    double xp, yp, zp, Yx, Yy, Yz;
+    double Yx, Yy, Yz;
     int cell, in, ccell, cin, err, escaped;
     long long nodes, crossings, rejects, classified, cache_hits, moves, kd_calls;
 };
@@ -150,14 +156,13 @@ __device__ __forceinline__ void walker_clear(WalkerF64& s)
     s.px = s.py = s.pz = s.asx = s.asy = s.asz = 0.0;
     s.ccx = s.ccy = s.ccz = s.csx = s.csy = s.csz = 0.0;
     s.crho = -1.0;
-    s.xp = s.yp = s.zp = s.Yx = s.Yy = s.Yz = 0.0;
+    s.Yx = s.Yy = s.Yz = 0.0;
     s.cell = s.in = s.ccell = s.cin = s.err = s.escaped = 0;
     s.nodes = s.crossings = s.rejects = s.classified = s.cache_hits = s.moves = s.kd_calls = 0;
 }
 
-// The start at (px, py, pz): the exact classification, the anchor image,
-// and the trapezoid's first point.
-__device__ __forceinline__ void walker_start(const GeomDev& g, const KdView& v, bool graph, double half, WalkerF64& s)
+// The start at (px, py, pz): the exact classification and the anchor image.
+__device__ __forceinline__ void walker_start(const GeomDev& g, const KdView& v, bool graph, WalkerF64& s)
 {
     ++s.classified;
     ++s.kd_calls;
@@ -167,15 +172,14 @@ __device__ __forceinline__ void walker_start(const GeomDev& g, const KdView& v, 
     } else if (graph) {
         seed_image(g, s.cell, s.px, s.py, s.pz, s.asx, s.asy, s.asz);
     }
-    s.xp = s.px - half;
-    s.yp = s.py - half;
-    s.zp = s.pz - half;
 }
 
 // One step with displacement (dx, dy, dz) and acceptance uniform u: the
 // proposal's label (the certified ball, the graph or the KD classifier),
 // the membrane count m and the acceptance test u < p_p^m, the move, and the
-// trapezoid of Y.
+// trapezoid of Y (its previous point: the position before the step less
+// L/2).
+template <int GRP>
 __device__ __forceinline__ void walker_step(const GeomDev& g, const KdView& v, const WalkDev& p, bool graph,
                                             double eps, double half, double dx, double dy, double dz, double u,
                                             WalkerF64& s)
@@ -199,7 +203,7 @@ __device__ __forceinline__ void walker_step(const GeomDev& g, const KdView& v, c
         ++s.classified;
         int na = s.cell;  // from the walker's own cell
         double m1 = 0.0;
-        if (graph_label_f64(g, na, nsx, nsy, nsz, nx, ny, nz, eps, nin, m1, s.moves)) {
+        if (graph_label_f64<GRP>(g, na, nsx, nsy, nsz, nx, ny, nz, eps, nin, m1, s.moves)) {
             ncell = na;
             const double al = g.cell_alpha64[na];
             s.crho = (nin ? m1 - al : fmin(m1, al - m1)) - 2.0 * eps;
@@ -245,6 +249,7 @@ __device__ __forceinline__ void walker_step(const GeomDev& g, const KdView& v, c
             ++s.rejects;
         }
     }
+    const double xp = s.px - half, yp = s.py - half, zp = s.pz - half;
     s.asx = nsx;
     s.asy = nsy;
     s.asz = nsz;
@@ -254,12 +259,9 @@ __device__ __forceinline__ void walker_step(const GeomDev& g, const KdView& v, c
     s.cell = ncell;
     s.in = nin;
     const double xn = s.px - half, yn = s.py - half, zn = s.pz - half;
-    s.Yx += 0.5 * (s.xp + xn) * p.ts;
-    s.Yy += 0.5 * (s.yp + yn) * p.ts;
-    s.Yz += 0.5 * (s.zp + zn) * p.ts;
-    s.xp = xn;
-    s.yp = yn;
-    s.zp = zn;
+    s.Yx += 0.5 * (xp + xn) * p.ts;
+    s.Yy += 0.5 * (yp + yn) * p.ts;
+    s.Yz += 0.5 * (zp + zn) * p.ts;
 }
 
 // The walker's Y and inside flag at grid point j (mode kY).
@@ -312,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
             s.py = g.source_lo + uniform_f64(r) * span;
             s.pz = g.source_lo + uniform_f64(r) * span;
         }
-        walker_start(g, v, graph, half, s);
+        walker_start(g, v, graph, s);
     }
     double dM[MADI_MAX_PAIRS][3];
 #pragma unroll
@@ -342,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64(GeomDev g, WalkDev p, const
                 dz = normal_f64(r) * p.sigma;
                 u = uniform_f64(r);
             }
-            walker_step(g, v, p, graph, eps, half, dx, dy, dz, u, s);
+            walker_step<kGraphGroupWalk>(g, v, p, graph, eps, half, dx, dy, dz, u, s);
         }
         const int32_t s1 = step + 1;
         if (s1 % p.steps_per_h == 0) {
@@ -442,15 +444,9 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64_sweep(GeomDev g, WalkDev p,
     const bool slot = m < M;  // a (walker, walk) thread; its walker may lie past the last
     const bool live = slot && w < p.n_walkers;
     const int G = p.n_grid;
-    if (slot) {
-        const size_t n = (size_t)p.n_walkers;
-        p.pp = mp.pp[k];
-        Y += k * n * (size_t)G * 3;
-        itr += k * n * (size_t)G;
-        esc_out += k * n;
-        err_out += k * n;
-        if (counters) counters += k * MADI_N_COUNTERS;
-    }
+    // the thread's row of the outputs: walk k's slice holds walker w at row k n + w
+    const int64_t row = (int64_t)k * p.n_walkers + w;
+    if (slot) p.pp = mp.pp[k];
     const bool graph = p.graph_labels != 0 && !g.free_water;
     const double eps = 1e-9 * fmax(1.0, g.L / 100.0);
     const double half = 0.5 * g.L;
@@ -464,10 +460,10 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64_sweep(GeomDev g, WalkDev p,
         s.px = g.source_lo + uniform_f64(r) * span;
         s.py = g.source_lo + uniform_f64(r) * span;
         s.pz = g.source_lo + uniform_f64(r) * span;
-        walker_start(g, v, graph, half, s);
-        double* Yw = Y + (size_t)w * G * 3;
+        walker_start(g, v, graph, s);
+        double* Yw = Y + (size_t)row * G * 3;
         Yw[0] = Yw[1] = Yw[2] = 0.0;
-        itr[(size_t)w * G] = (int8_t)(s.err ? 0 : s.in);
+        itr[(size_t)row * G] = (int8_t)(s.err ? 0 : s.in);
     }
     for (int32_t base = 0; base < p.n_steps; base += K) {
         const int B = p.n_steps - base < K ? p.n_steps - base : K;
@@ -495,24 +491,25 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64_sweep(GeomDev g, WalkDev p,
         for (int t = 0; t < B; ++t) {
             if (live && !s.escaped) {
                 const double4 d = table[m * K + t];
-                walker_step(g, v, p, graph, eps, half, d.x, d.y, d.z, d.w, s);
+                walker_step<kGraphGroupSweep>(g, v, p, graph, eps, half, d.x, d.y, d.z, d.w, s);
             }
             const int32_t s1 = base + t + 1;
-            if (s1 % p.steps_per_h == 0 && live) walker_sample(s, w, G, s1 / p.steps_per_h, Y, itr);
+            if (s1 % p.steps_per_h == 0 && live) walker_sample(s, row, G, s1 / p.steps_per_h, Y, itr);
         }
     }
     if (live) {
-        esc_out[w] = (int8_t)s.escaped;
-        err_out[w] = (int8_t)s.err;
+        esc_out[row] = (int8_t)s.escaped;
+        err_out[row] = (int8_t)s.err;
         if (counters) {
             // the threads of a wave belong to different walks: one add each
-            atomicAdd(&counters[MADI_CNT_KD_NODES], (unsigned long long)s.nodes);
-            atomicAdd(&counters[MADI_CNT_CROSSINGS], (unsigned long long)s.crossings);
-            atomicAdd(&counters[MADI_CNT_REJECTS], (unsigned long long)s.rejects);
-            atomicAdd(&counters[MADI_CNT_CLASSIFY], (unsigned long long)s.classified);
-            atomicAdd(&counters[MADI_CNT_SAFE_HIT], (unsigned long long)s.cache_hits);
-            atomicAdd(&counters[MADI_CNT_GRAPH_ITERS], (unsigned long long)s.moves);
-            atomicAdd(&counters[MADI_CNT_FP64_RECHECK], (unsigned long long)s.kd_calls);
+            unsigned long long* c = counters + k * MADI_N_COUNTERS;
+            atomicAdd(&c[MADI_CNT_KD_NODES], (unsigned long long)s.nodes);
+            atomicAdd(&c[MADI_CNT_CROSSINGS], (unsigned long long)s.crossings);
+            atomicAdd(&c[MADI_CNT_REJECTS], (unsigned long long)s.rejects);
+            atomicAdd(&c[MADI_CNT_CLASSIFY], (unsigned long long)s.classified);
+            atomicAdd(&c[MADI_CNT_SAFE_HIT], (unsigned long long)s.cache_hits);
+            atomicAdd(&c[MADI_CNT_GRAPH_ITERS], (unsigned long long)s.moves);
+            atomicAdd(&c[MADI_CNT_FP64_RECHECK], (unsigned long long)s.kd_calls);
         }
     }
 }
