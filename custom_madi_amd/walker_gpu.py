@@ -382,9 +382,11 @@ def _walk_and_reduce_kio_sweep(ens: Ensemble, pps: Sequence[float], cfg: SimConf
 
     The reference walks an ensemble once per k_io with one walk seed
     (madi/walker_gpu.py:1193-1226).  The FP64 walks whose columns are reduced
-    from Y run several per launch (madi_walk_f64_multi: the same kernel per
-    walk, so the same results): one walk of 50,000 walkers fills a tenth of
-    the GPU, the sweep together fills it."""
+    from Y run up to 64 per launch (madi_walk_f64_multi): a walker's walks in
+    adjacent lanes, sharing each step's draws and their FP64 arithmetic,
+    every value computed as in the one-walk kernel, so the results are the
+    same.  One walk of 50,000 walkers is 196 blocks, under one per CU; the
+    sweep together fills the GPU."""
     pps = [_checked_pp(pp) for pp in pps]
     fused = engine.spec_from_columns(j_delta, j_Delta, j_sum, phase_coef)
     if cfg.precision == "fp32" or fused is not None or len(pps) < 2:
