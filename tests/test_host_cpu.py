@@ -331,6 +331,36 @@ def test_y_walk_and_reduce_abi_argument_checks():
     assert lib.madi_reduce_f64(fake, 10, 3, fake, fake, fake, fake, 0, C.byref(cs), C.byref(cs), None) == _hip.MADI_OK
 
 
+def test_walk_f64_multi_abi_argument_checks():
+    """madi_walk_f64_multi's argument errors (no geometry, walk counts outside
+    1..MADI_MAX_MULTI_WALKS, missing or invalid p_p, missing outputs), which
+    return before any device call or geometry access."""
+    import ctypes as C
+    from custom_madi_amd import _hip
+    lib = _hip.load_library()
+    p = _hip.WalkParams()
+    p.n_walkers, p.n_steps, p.steps_per_h, p.n_grid, p.pp = 10, 20, 10, 3, 0.5
+    fake = 4096  # never dereferenced: each call below returns before touching memory
+    K = _hip.MADI_MAX_MULTI_WALKS
+
+    def call(g, k, pps, Y=fake):
+        arr = None if pps is None else C.cast((C.c_double * len(pps))(*pps), C.c_void_p)
+        return lib.madi_walk_f64_multi(g, C.byref(p), k, arr, Y, fake, fake, fake, None, None)
+
+    assert call(None, 2, [0.1, 0.2]) == _hip.MADI_ERR_ARG
+    assert call(fake, 0, [0.1]) == _hip.MADI_ERR_ARG
+    assert call(fake, K + 1, [0.1] * (K + 1)) == _hip.MADI_ERR_ARG
+    assert f"1..{K}" in _hip.last_error()
+    assert call(fake, 2, None) == _hip.MADI_ERR_ARG
+    assert call(fake, 2, [0.1, 0.2], Y=None) == _hip.MADI_ERR_ARG
+    assert call(fake, 2, [0.1, float("nan")]) == _hip.MADI_ERR_ARG
+    assert "walk 1" in _hip.last_error()
+    assert call(fake, 3, [0.1, 0.2, 1.5]) == _hip.MADI_ERR_ARG
+    assert call(fake, 2, [-0.1, 0.2]) == _hip.MADI_ERR_ARG
+    p.pp = 2.0  # the params' own p_p is checked too
+    assert call(fake, 2, [0.1, 0.2]) == _hip.MADI_ERR_ARG
+
+
 def test_matcher_and_finalize_abi_argument_checks():
     """Workspace sizes and argument errors of the matcher and tile-sum entry
     points, which return before any device call."""
