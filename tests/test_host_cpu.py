@@ -295,6 +295,33 @@ def test_log_grid_weights_and_free_water():
 
 # ------------------------------------------------------------------ C ABI
 
+def test_production_grid_points_and_benchmark_coordinates():
+    """The production log grid's rho and V at indices the reference's
+    replicate subset lists with their values
+    (data/madi_v5_replicate_entry_subset.json:17-37), and the benchmark's
+    density coordinates derived from it (analysis/v5_runtime_benchmark.py:101-123:
+    rho index 0 / 21 / last, the admissible V whose v_i is nearest in log to
+    the geometric centre of the v_i range)."""
+    import bench
+    from custom_madi_amd.library import make_remediation_log_grid
+    grid = make_remediation_log_grid()
+    assert (len(grid.rhos), len(grid.Vs), grid.vi_min, grid.vi_max) == (64, 64, 0.4, 0.99)
+    for ri, vj, rho, V in [(0, 53, 10000.0, 41.52664275536708), (3, 54, 13894.954943731389, 48.59562131612258),
+                           (9, 47, 26826.957952797275, 16.169810020515047), (18, 41, 71968.56730011514,
+                                                                            6.2962652107424715),
+                           (21, 42, 100000.0, 7.368062997280774), (42, 29, 1000000.0, 0.9546390696098913),
+                           (63, 14, 10000000.0, 0.09032010701366826)]:
+        assert grid.rhos[ri] == pytest.approx(rho, rel=1e-14) and grid.Vs[vj] == pytest.approx(V, rel=1e-14)
+    centre = np.sqrt(0.4 * 0.99)
+    for which, ri in (("low", 0), ("center", 21), ("high", 63)):
+        rho, V, vi = bench.production_coordinate(which)
+        assert rho == grid.rhos[ri] and vi == pytest.approx(rho * V * 1e-6)
+        admissible = [float(v) for v in grid.Vs if 0.4 <= rho * float(v) * 1e-6 <= 0.99]
+        assert V in admissible
+        assert all(abs(np.log(vi / centre)) <= abs(np.log(rho * v * 1e-6 / centre)) for v in admissible)
+    assert bench.production_coordinate("center")[1] == grid.Vs[41]  # 6.296 pL, v_i 0.630
+
+
 def test_hip_library_exports_every_declared_symbol():
     import ctypes
     from custom_madi_amd import _hip
