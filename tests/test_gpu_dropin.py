@@ -157,6 +157,19 @@ def test_an_escape_through_run_simulation_reduced_is_fatal(precision):
                                           cfg=cfg, seed=1, verbose=False)
 
 
+def test_an_escape_in_a_kio_sweep_is_fatal():
+    """The FP64 k_io sweep (several walks per launch, columns reduced from Y)
+    keeps the reference's fatal escape (madi/walker_gpu.py:1023-1028)."""
+    cfg = _cfg(L=3.0, small_deltas=[1.0, 2.0, 3.0], big_deltas=[4.0, 5.0, 6.0, 7.0, 8.0])
+    cols = signal.build_columns(cfg)
+    args = (cols.j_delta, cols.j_Delta, cols.j_sum, cols.phase_coef)
+    assert engine.spec_from_columns(*args) is None  # the sweep path, not the fused walk
+    say("[dropin] escaping walks, run_simulation_multi_kio_reduced k_io sweep")
+    with pytest.raises(RuntimeError, match="fatal boundary condition"):
+        walker_gpu.run_simulation_multi_kio_reduced(0.0, 0.0, [0.0, 20.0, 60.0], *args, cfg=cfg, seed=1,
+                                                    verbose=False)
+
+
 def test_coincident_seeds_are_a_classifier_failure_of_the_exact_walk():
     ens = _geom.cellular(L=50.0, seed=3)
     mid = np.all((ens.seeds > ens.source_lo + 1.0) & (ens.seeds < ens.source_hi - 1.0), axis=1)
