@@ -119,6 +119,7 @@ SIGNATURES = {
     "madi_rng_init": (C.c_int, [C.c_uint64, C.c_uint64, C.c_int64, _p, _p]),
     "madi_walk_f64": (C.c_int, [_p, C.POINTER(WalkParams), _p, _p, _p, _p, _p, _p, _p, _p, _p]),
     "madi_walk_f64_multi": (C.c_int, [_p, C.POINTER(WalkParams), C.c_int32, _p, _p, _p, _p, _p, _p, _p]),
+    "madi_count_inside": (C.c_int, [_p, C.c_int64, C.c_int64, C.c_int32, _p, _p]),
     "madi_reduce_f64": (C.c_int, [_p, C.c_int64, C.c_int32, _p, _p, _p, _p, C.c_int32, _p, _p, _p]),
     "madi_walk_fused_workspace": (C.c_int, [C.c_int64, C.POINTER(PhasePlan), _i64p]),
     "madi_walk_fused": (C.c_int, [_p, C.POINTER(WalkParams), C.POINTER(PhasePlan), C.c_int32, _p, C.c_int64,

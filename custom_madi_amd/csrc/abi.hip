@@ -1260,6 +1260,18 @@ int madi_walk_f64_multi(const madi_geom* g, const madi_walk_params* p, int32_t n
     return rc ? rc : mark_used(g, (hipStream_t)stream);
 }
 
+int madi_count_inside(const int8_t* d_inside_trace, int64_t n_walks, int64_t n_walkers, int32_t n_grid,
+                      int64_t* d_counts, void* stream)
+{
+    if (n_walks < 0 || n_walkers < 0 || n_grid < 0 || n_walks > 0x7fffffff ||
+        (n_walks > 0 && n_grid > 0 && (!d_counts || (n_walkers > 0 && !d_inside_trace)))) {
+        set_error("madi_count_inside: invalid arguments");
+        return MADI_ERR_ARG;
+    }
+    return launch_count_inside(d_inside_trace, n_walks, n_walkers, n_grid, (long long*)d_counts,
+                               (hipStream_t)stream);
+}
+
 int madi_reduce_f64(const double* d_Y, int64_t n, int32_t n_grid, const int32_t* d_jd, const int32_t* d_jD,
                     const int32_t* d_js, const double* d_pc, int32_t n_cols, double* d_cos, double* d_sin,
                     void* stream)

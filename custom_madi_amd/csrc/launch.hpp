@@ -108,6 +108,7 @@ int launch_walk_f64(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, 
                     hipStream_t s);
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,
                           unsigned long long* counters, hipStream_t s);
+int launch_count_inside(const int8_t* itr, int64_t n_walks, int64_t n, int32_t G, long long* counts, hipStream_t s);
 int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const MultiPP& mp, double* Y,
                           int8_t* itr, int8_t* esc, int8_t* err, unsigned long long* counters, hipStream_t s);
 // Sample chunks of the column reduce for n walkers and n_cols columns (the

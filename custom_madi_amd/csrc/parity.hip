@@ -514,6 +514,37 @@ __global__ __launch_bounds__(kBlock) void k_walk_f64_sweep(GeomDev g, WalkDev p,
     }
 }
 
+// Occupancy of stacked walks: counts[k][j] = sum over walkers w of
+// inside[k][w][j] (the reference's np.sum(inside, axis=0),
+// madi/walker_gpu.py:1054).  Block (walk k, 64 grid points) with 4 rows of
+// threads: consecutive threads read consecutive bytes of a walker's row,
+// each row of threads strides over the walkers, then the rows are summed in
+// LDS.  Integer sums, so the order does not matter.
+constexpr int kCountCols = 64;
+constexpr int kCountRows = kBlock / kCountCols;
+
+__global__ __launch_bounds__(kBlock) void k_count_inside(const int8_t* __restrict__ itr, int64_t n, int32_t G,
+                                                        long long* __restrict__ counts)
+{
+    __shared__ long long part[kCountRows][kCountCols];
+    const int tx = threadIdx.x % kCountCols, ty = threadIdx.x / kCountCols;
+    const int64_t k = blockIdx.y;
+    const int j = blockIdx.x * kCountCols + tx;
+    long long c = 0;
+    if (j < G) {
+        const int8_t* col = itr + k * n * G + j;
+        for (int64_t w = ty; w < n; w += kCountRows) c += col[w * G];
+    }
+    part[ty][tx] = c;
+    __syncthreads();
+    if (ty == 0 && j < G) {
+        long long t = 0;
+#pragma unroll
+        for (int r = 0; r < kCountRows; ++r) t += part[r][tx];
+        counts[k * G + j] = t;
+    }
+}
+
 // -------------------------------------------------------------- reduce
 
 // sin and cos of x in FP64 for |x| <= kSinCosFast.  Cody and Waite's
@@ -800,6 +831,14 @@ int launch_walk_f64_multi(const GeomDev& g, const WalkDev& p, const Gf2Col* jump
     hipLaunchKernelGGL(k_walk_f64_sweep, dim3((unsigned)blocks), dim3(kBlock), 0, s, g, p, jump_pow, mp, Y, itr, esc,
                        err, counters);
     return check_launch("k_walk_f64_sweep");
+}
+
+int launch_count_inside(const int8_t* itr, int64_t n_walks, int64_t n, int32_t G, long long* counts, hipStream_t s)
+{
+    if (n_walks == 0 || G == 0) return MADI_OK;
+    hipLaunchKernelGGL(k_count_inside, dim3((unsigned)((G + kCountCols - 1) / kCountCols), (unsigned)n_walks),
+                       dim3(kBlock), 0, s, itr, n, G, counts);
+    return check_launch("k_count_inside");
 }
 
 int launch_walk_f64_fused(const GeomDev& g, const WalkDev& p, const Gf2Col* jump_pow, const FusedOut& fo,

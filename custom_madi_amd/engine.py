@@ -146,10 +146,11 @@ def rng_states(seed: int, first: int, n: int) -> np.ndarray:
 @dataclass
 class WalkY:
     Y: object            # device tensor (n, n_grid, 3) float64
-    inside_trace: np.ndarray
+    inside_trace: Optional[np.ndarray]  # (n, n_grid) int8 (None when only the occupancy was asked for)
     escaped: np.ndarray
     err: np.ndarray
     counters: dict
+    occupancy: Optional[np.ndarray] = None  # (n_grid,) int64 inside counts, counted on the device
 
 
 def walk_y(geom: DeviceGeometry, params: _hip.WalkParams, stream=None) -> WalkY:
@@ -178,10 +179,12 @@ def walk_y(geom: DeviceGeometry, params: _hip.WalkParams, stream=None) -> WalkY:
                  dict(zip(_hip.COUNTER_NAMES, cnt.cpu().numpy().tolist())))
 
 
-def walk_y_multi(geom: DeviceGeometry, params: _hip.WalkParams, pps) -> List[WalkY]:
+def walk_y_multi(geom: DeviceGeometry, params: _hip.WalkParams, pps, traces: bool = True) -> List[WalkY]:
     """Several FP64 walks that differ only in p_p, in one launch
     (madi_walk_f64_multi): walk k equals walk_y with params.pp = pps[k], bit
-    for bit."""
+    for bit.  traces=False leaves the inside traces on the device and
+    returns each walk's occupancy, counted there (madi_count_inside),
+    instead."""
     torch = _hip.require_gpu()
     K = len(pps)
     if not 1 <= K <= _hip.MADI_MAX_MULTI_WALKS:
@@ -197,9 +200,17 @@ def walk_y_multi(geom: DeviceGeometry, params: _hip.WalkParams, pps) -> List[Wal
     rc = lib.madi_walk_f64_multi(geom.handle, C.byref(params), K, pp.ctypes.data, _hip.dptr(Y), _hip.dptr(itr),
                                  _hip.dptr(esc), _hip.dptr(err), _hip.dptr(cnt), _hip.stream_ptr())
     _hip.check(rc, "madi_walk_f64_multi")
+    occ_h = None
+    if not traces:
+        occ = torch.empty((K, G), dtype=torch.int64, device="cuda")
+        _hip.check(lib.madi_count_inside(_hip.dptr(itr), K, n, G, _hip.dptr(occ), _hip.stream_ptr()),
+                   "madi_count_inside")
+        occ_h = occ.cpu().numpy()
     torch.cuda.synchronize()
-    itr_h, esc_h, err_h, cnt_h = itr.cpu().numpy(), esc.cpu().numpy(), err.cpu().numpy(), cnt.cpu().numpy()
-    return [WalkY(Y[k], itr_h[k], esc_h[k], err_h[k], dict(zip(_hip.COUNTER_NAMES, cnt_h[k].tolist())))
+    esc_h, err_h, cnt_h = esc.cpu().numpy(), err.cpu().numpy(), cnt.cpu().numpy()
+    itr_h = itr.cpu().numpy() if traces else None
+    return [WalkY(Y[k], None if itr_h is None else itr_h[k], esc_h[k], err_h[k],
+                  dict(zip(_hip.COUNTER_NAMES, cnt_h[k].tolist())), None if occ_h is None else occ_h[k])
             for k in range(K)]
 
 

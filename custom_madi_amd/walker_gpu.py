@@ -406,7 +406,7 @@ def _walk_and_reduce_kio_sweep(ens: Ensemble, pps: Sequence[float], cfg: SimConf
         params = engine.walk_params(n_this, cfg, pps[0], walk_seed + offset)
         per_launch = int(max(1, min(_hip.MADI_MAX_MULTI_WALKS, _MULTI_WALK_Y_BYTES // max(1, n_this * G * 24))))
         for k0 in range(0, K, per_launch):
-            outs = engine.walk_y_multi(geom, params, pps[k0:k0 + per_launch])
+            outs = engine.walk_y_multi(geom, params, pps[k0:k0 + per_launch], traces=False)
             for k, out in zip(range(k0, k0 + per_launch), outs):
                 if np.any(out.err):
                     raise RuntimeError("GPU full-facet KD-tree classification failed")
@@ -416,7 +416,7 @@ def _walk_and_reduce_kio_sweep(ens: Ensemble, pps: Sequence[float], cfg: SimConf
                 c, sn = engine.reduce_columns(out.Y, j_delta, j_Delta, j_sum, phase_coef)
                 cos_t[k] += c
                 sin_t[k] += sn
-                occ_t[k] += np.sum(out.inside_trace, axis=0, dtype=np.int64)
+                occ_t[k] += out.occupancy
                 st = _stats_from_counters(out.counters, n_this * (cfg.n_steps + 1))
                 stats[k].full += st.full
                 stats[k].tier1 += st.tier1

@@ -24,7 +24,7 @@
 extern "C" {
 #endif
 
-#define MADI_ABI_VERSION 8
+#define MADI_ABI_VERSION 9
 
 #define MADI_OK 0
 #define MADI_ERR_ARG 1
@@ -285,6 +285,16 @@ int madi_walk_f64(const madi_geom* g, const madi_walk_params* p, const double* d
 int madi_walk_f64_multi(const madi_geom* g, const madi_walk_params* p, int32_t n_walks, const double* pp,
                         double* d_Y, int8_t* d_inside_trace, int8_t* d_escaped, int8_t* d_err, int64_t* d_counters,
                         void* stream);
+
+/* Occupancy of stacked walks, on the device: d_counts[k * n_grid + j] = the
+ * sum over walkers w of d_inside_trace[(k * n_walkers + w) * n_grid + j].
+ * The inside trace is as madi_walk_f64_multi writes it (n_walks x n_walkers
+ * x n_grid int8 flags, 0 or 1), so each count is the number of walkers of
+ * walk k inside at grid point j; the reference sums its inside trace on the
+ * host (np.sum(inside, axis=0), madi/walker_gpu.py:1054).  d_counts: n_walks
+ * x n_grid int64, overwritten. */
+int madi_count_inside(const int8_t* d_inside_trace, int64_t n_walks, int64_t n_walkers, int32_t n_grid,
+                      int64_t* d_counts, void* stream);
 
 /* Workspace bytes of madi_walk_y_f32 for n_walkers. */
 int madi_walk_y_f32_workspace(int64_t n_walkers, int64_t* bytes);

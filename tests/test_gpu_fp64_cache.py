@@ -190,8 +190,11 @@ def test_walk_y_multi_equals_walk_y_per_walk(case):
     dg = DeviceGeometry(geometry_arrays(ens))
     params = engine.walk_params(n, cfg, pps[0], 19, classify_mode=mode)
     many = engine.walk_y_multi(dg, params, pps)
+    counted = engine.walk_y_multi(dg, params, pps, traces=False)  # occupancy counted on the device
     escaped_any = False
-    for pp, b in zip(pps, many):
+    for pp, b, c in zip(pps, many, counted):
+        assert c.inside_trace is None
+        assert np.array_equal(c.occupancy, np.sum(b.inside_trace, axis=0, dtype=np.int64))
         a = engine.walk_y(dg, engine.walk_params(n, cfg, pp, 19, classify_mode=mode))
         assert np.array_equal(a.Y.cpu().numpy(), b.Y.cpu().numpy())
         assert np.array_equal(a.inside_trace, b.inside_trace)

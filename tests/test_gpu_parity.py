@@ -50,7 +50,7 @@ def periodic_geom():
 
 def test_library_abi_version():
     lib = _hip.load_library()
-    assert lib.madi_abi_version() == 8
+    assert lib.madi_abi_version() == 9
     info = _hip.device_info()
     say(f"device: {info}")
     assert info["cu_count"] > 0

@@ -360,8 +360,9 @@ def test_y_walk_and_reduce_abi_argument_checks():
 
 def test_walk_f64_multi_abi_argument_checks():
     """madi_walk_f64_multi's argument errors (no geometry, walk counts outside
-    1..MADI_MAX_MULTI_WALKS, missing or invalid p_p, missing outputs), which
-    return before any device call or geometry access."""
+    1..MADI_MAX_MULTI_WALKS, missing or invalid p_p, missing outputs) and
+    madi_count_inside's, which return before any device call or geometry
+    access."""
     import ctypes as C
     from custom_madi_amd import _hip
     lib = _hip.load_library()
@@ -386,6 +387,11 @@ def test_walk_f64_multi_abi_argument_checks():
     assert call(fake, 2, [-0.1, 0.2]) == _hip.MADI_ERR_ARG
     p.pp = 2.0  # the params' own p_p is checked too
     assert call(fake, 2, [0.1, 0.2]) == _hip.MADI_ERR_ARG
+    # the device occupancy count of stacked walks
+    assert lib.madi_count_inside(fake, -1, 10, 3, fake, None) == _hip.MADI_ERR_ARG
+    assert lib.madi_count_inside(fake, 2, 10, 3, None, None) == _hip.MADI_ERR_ARG
+    assert lib.madi_count_inside(None, 2, 10, 3, fake, None) == _hip.MADI_ERR_ARG
+    assert lib.madi_count_inside(None, 0, 10, 3, None, None) == _hip.MADI_OK  # nothing to count
 
 
 def test_matcher_and_finalize_abi_argument_checks():
